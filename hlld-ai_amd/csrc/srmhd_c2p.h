@@ -1,0 +1,246 @@
+// Kastaun con2prim on the device (c2p.py:41-363), batch-faithful.
+//
+// The reference's secant (c2p.py:214-233) keeps stepping EVERY element until
+// all elements of the batch satisfy |f| < 1e-15, or 60 steps pass.  A
+// converged element that keeps stepping is usually kicked out again (its
+// next secant step rounds onto x1 and the bracket guard jumps to the
+// midpoint), so the break iteration K is a property of the whole batch.
+//
+// It is reproduced in two passes with no grid-wide barrier:
+//   pass A  every element runs the full 60 steps and records bit k of a
+//           64-bit mask when it is NOT converged after k steps (k = 0..59);
+//           the masks are OR-reduced into one word per batch; the state after
+//           60 steps and mu_+ are stored (24 B per element);
+//   pass B  K = index of the first zero bit of the OR (60 if none).  K = 60,
+//           the common case for large batches, reuses the stored state;
+//           K < 60 recomputes K steps.  Then _finish, the 200-step bisection
+//           redo of unconverged elements with a good bracket, and the
+//           atmosphere floor, exactly as conservative_to_primitive.
+#pragma once
+
+#include "srmhd_common.h"
+
+namespace srmhd {
+
+constexpr double C2P_TOL = 1e-15;
+constexpr int C2P_SECANT = 60;
+constexpr int C2P_BISECT = 200;
+
+struct Kastaun {
+  double D, q, r[3], r2, bt[3], bt2, br, bperp2, v02, B[3];
+  srmhd_eos e;
+
+  // c2p.py:48-106; cons order D, tau, Sx, Sy, Sz, Bx, By, Bz
+  SRMHD_DEV void init(const double c[8], const srmhd_eos& eos) {
+    e = eos;
+    D = c[CD];
+    double S[3] = {c[CS], c[CS + 1], c[CS + 2]};
+    const double tau = clo(c[CT], 0.0);
+    B[0] = c[CB];
+    B[1] = c[CB + 1];
+    B[2] = c[CB + 2];
+    double S2 = sq(S[0]) + sq(S[1]) + sq(S[2]);
+    const double Sn = csqrt(S2);
+    const double Smax = D + tau;
+    const double fac = (Sn > Smax) ? 0.9999 * Smax / (Sn + TINY) : 1.0;
+    S[0] = S[0] * fac;
+    S[1] = S[1] * fac;
+    S[2] = S[2] * fac;
+    S2 = sq(S[0]) + sq(S[1]) + sq(S[2]);
+    const double B2 = sq(B[0]) + sq(B[1]) + sq(B[2]);
+    const double BS = B[0] * S[0] + B[1] * S[1] + B[2] * S[2];
+    q = tau / (D + TINY);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] = S[i] / (D + TINY);
+    r2 = S2 / (D * D + TINY);
+    const double sD = csqrt(D + TINY);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) bt[i] = B[i] / sD;
+    bt2 = B2 / (D + TINY);
+    br = BS / (D * sD + TINY);
+    bperp2 = r2 * bt2 - br * br;
+    v02 = r2 / (r2 + 1.0);
+  }
+
+  SRMHD_DEV double chi(double mu) const { return 1.0 / (1.0 + mu * bt2); }
+
+  SRMHD_DEV double rbar2(double mu, double c) const {
+    return r2 * (c * c) + mu * c * (1.0 + c) * (br * br);
+  }
+
+  // mu sqrt(h_min^2 + rbar^2) - 1   c2p.py:124-127
+  SRMHD_DEV double fa(double mu) const { return mu * csqrt(1.0 + rbar2(mu, chi(mu))) - 1.0; }
+
+  // master function   c2p.py:129-158
+  SRMHD_DEV double master(double mu, double* rho_out = nullptr, double* eps_out = nullptr) const {
+    const double c = chi(mu);
+    const double mc = mu * c;
+    const double qb = q - 0.5 * bt2 - 0.5 * (mc * mc) * bperp2;
+    const double rb2 = rbar2(mu, c);
+    const double vh2 = chi_(mu * mu * rb2, v02);
+    const double Wh = 1.0 / csqrt(clo(1.0 - vh2, 1e-10));
+    const double rho = tmin(tmax(D / Wh, 1e-12), 1e15);
+    const double eps0 = Wh * (qb - mu * rb2) + vh2 * (Wh * Wh) / (1.0 + Wh);
+    const double eps = tmin(tmax(eps0, eos_eps_lo(e, rho)), 1e5);
+    const double p = eos_press(e, eps, rho);
+    const double a = p / (rho * (1.0 + eps) + TINY);
+    const double nuA = (1.0 + a) * (1.0 + eps) / Wh;
+    const double nuB = (1.0 + a) * (1.0 + qb - mu * rb2);
+    const double nu = tmax(nuA, nuB);
+    if (rho_out) *rho_out = rho;
+    if (eps_out) *eps_out = eps;
+    return mu - sdiv(1.0, nu + mu * rb2);
+  }
+
+  // 50-step bisection of fa on [0, 1] (c2p.py:162-184).  The reference
+  // re-evaluates fa(lo) every iteration; fa is deterministic, so carrying
+  // the last value is bit-identical.
+  SRMHD_DEV double mu_plus() const {
+    double lo = 0.0, hi = 1.0 / (1.0 + TINY);
+    double flo = fa(lo);
+#pragma unroll 1
+    for (int it = 0; it < 50; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      const double fm = fa(mid);
+      if ((flo * fm) <= 0.0) {
+        hi = mid;
+      } else {
+        lo = mid;
+        flo = fm;
+      }
+    }
+    return hi;
+  }
+
+  // one guarded secant step   c2p.py:219-232
+  SRMHD_DEV void secant_step(double& x0, double& f0, double& x1, double& f1) const {
+    const double df = f1 - f0;
+    const double mid = 0.5 * (x0 + x1);
+    double x2 = (fabs(df) > 1e-30) ? x1 - f1 * (x1 - x0) / df : mid;
+    x2 = ((x2 - x0) * (x2 - x1) < 0.0) ? x2 : mid;
+    const double f2 = master(x2);
+    if (!((f0 * f2) <= 0.0)) {
+      x0 = x1;
+      f0 = f1;
+    }
+    x1 = x2;
+    f1 = f2;
+  }
+
+  // 200-step bisection continuation   c2p.py:238-266 -> (mu, |f| at mu)
+  SRMHD_DEV double bisection(double mup, double& fmu) const {
+    double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
+    double f0 = master(x0), f1 = master(x1);
+#pragma unroll 1
+    for (int it = 0; it < C2P_BISECT; ++it) {
+      const double xm = 0.5 * (x0 + x1);
+      const double fm = master(xm);
+      if ((f0 * fm) <= 0.0) {
+        x1 = xm;
+        f1 = fm;
+      } else {
+        x0 = xm;
+        f0 = fm;
+      }
+    }
+    const bool pick = fabs(f0) < fabs(f1);
+    fmu = pick ? f0 : f1;
+    return pick ? x0 : x1;
+  }
+
+  // primitives at mu   c2p.py:268-310; returns the v-clamp flag
+  SRMHD_DEV bool finish(double mu, Prim& P) const {
+    double rho, eps;
+    master(mu, &rho, &eps);
+    const double mc = mu * chi(mu);
+    const double mbr = mu * br;
+    double v[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[i] = mc * (r[i] + mbr * bt[i]);
+    const double v2 = sq(v[0]) + sq(v[1]) + sq(v[2]);
+    const double lim = 1.0 - 1e-10;
+    const bool clamped = v2 > lim;
+    const double norm = clamped ? csqrt(chi_(v2, lim) / (v2 + TINY)) : 1.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) P.v[i] = v[i] * norm;
+    P.rho = rho;
+    P.eps = eps;
+    P.p = eos_press(e, eps, rho);
+    P.B[0] = B[0];
+    P.B[1] = B[1];
+    P.B[2] = B[2];
+    return clamped;
+  }
+};
+
+// per-element record of pass A: mu_+ (sign = good bracket), state after 60 steps
+struct C2PRec {
+  double mup_signed, x1, f1;
+};
+
+// Pass A.  Returns the element's not-converged mask (bit k: after k steps).
+SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRec& rec) {
+  const double mup = k.mu_plus();
+  double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
+  double f0 = k.master(x0), f1 = k.master(x1);
+  const bool good = (f0 * f1) < 0.0;
+  uint64_t mask = 0;
+#pragma unroll 1
+  for (int it = 0; it < C2P_SECANT; ++it) {
+    if (!(fabs(f1) < C2P_TOL)) mask |= (uint64_t)1 << it;
+    k.secant_step(x0, f0, x1, f1);
+  }
+  rec.mup_signed = good ? mup : -mup;
+  rec.x1 = x1;
+  rec.f1 = f1;
+  return mask;
+}
+
+// break iteration of the batch from the OR of every element's mask
+SRMHD_DEV int c2p_break_iter(uint64_t ormask) {
+  const uint64_t zeros = ~ormask & ((((uint64_t)1) << C2P_SECANT) - 1);
+  return zeros ? __builtin_ctzll(zeros) : C2P_SECANT;
+}
+
+// status bits written by the con2prim entry points
+enum { C2P_BRACKET_OK = 1, C2P_CONVERGED = 2, C2P_V_CLAMPED = 4, C2P_ATMO = 8 };
+
+// Pass B: full conservative_to_primitive of one element given its pass-A
+// record and the batch break iteration K.   c2p.py:313-363
+SRMHD_DEV int c2p_pass_b(const Kastaun& k, const C2PRec& rec, int K, double atmo_rho,
+                         double atmo_eps, Prim& P) {
+  const bool good = rec.mup_signed > 0.0;
+  const double mup = fabs(rec.mup_signed);
+  double x1 = rec.x1, f1 = rec.f1;
+  if (K < C2P_SECANT) {
+    double x0 = 0.0;
+    x1 = mup * (1.0 - 1e-10);
+    double f0 = k.master(x0);
+    f1 = k.master(x1);
+#pragma unroll 1
+    for (int it = 0; it < K; ++it) k.secant_step(x0, f0, x1, f1);
+  }
+  bool conv = fabs(f1) < C2P_TOL;
+  bool clamped;
+  if (!conv && good) {
+    double fmu;
+    const double mu = k.bisection(mup, fmu);
+    conv = fabs(fmu) < C2P_TOL;
+    clamped = k.finish(mu, P);
+  } else {
+    clamped = k.finish(x1, P);
+  }
+  const bool atmo = P.rho < atmo_rho;
+  if (atmo) {
+    P.rho = atmo_rho;
+    P.eps = atmo_eps;
+    P.v[0] = 0.0;
+    P.v[1] = 0.0;
+    P.v[2] = 0.0;
+  }
+  P.p = eos_press(k.e, P.eps, P.rho);
+  return (good ? C2P_BRACKET_OK : 0) | (conv ? C2P_CONVERGED : 0) | (clamped ? C2P_V_CLAMPED : 0) |
+         (atmo ? C2P_ATMO : 0);
+}
+
+}  // namespace srmhd

@@ -1,0 +1,989 @@
+// Kernels and C-ABI entry points of the gfx950 SRMHD step (see include/srmhd.h).
+//
+// Launch shape: one thread per face / cell, 256-thread blocks (4 wave64s).
+// Reductions are wave-level (shuffles, ballots) followed by one atomic per
+// wave on 64-bit words; every reduced quantity is an integer, a bit mask or a
+// max, so results are deterministic.
+#include "srmhd_recon.h"
+#include "srmhd_riemann.h"
+
+using namespace srmhd;
+
+namespace {
+
+constexpr int BLK = 256;
+
+inline int blocks_for(int64_t n) { return (int)((n + BLK - 1) / BLK); }
+
+inline int hip_status(hipError_t e) { return e == hipSuccess ? SRMHD_OK : SRMHD_EHIP + (int)e; }
+
+inline int launch_status() { return hip_status(hipGetLastError()); }
+
+__device__ inline unsigned long long wave_or(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off);
+  return v;
+}
+__device__ inline unsigned long long wave_max(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ inline bool lane0() { return (threadIdx.x & 63) == 0; }
+
+__device__ inline unsigned long long dbits(double x) { return __double_as_longlong(x); }
+
+// Per-face counters of one flux call (hlld.py:1516-1529), wave-aggregated.
+__device__ void diag_accumulate(srmhd_flux_diag* d, bool active, int st, int iters) {
+  const unsigned long long nc = __popcll(__ballot(active && (st & FS_NOT_CONVERGED)));
+  const unsigned long long nw = __popcll(__ballot(active && (st & FS_WAVE_ORDER)));
+  const unsigned long long nb = __popcll(__ballot(active && (st & FS_BN_ZERO)));
+  const unsigned long long nf = __popcll(__ballot(active && (st & FS_FALLBACK)));
+  const unsigned long long na = __popcll(__ballot(active));
+  const bool conv = active && !(st & FS_NOT_CONVERGED);
+  const unsigned long long ncv = __popcll(__ballot(conv));
+  const unsigned long long si = wave_sum(conv ? (unsigned long long)iters : 0ull);
+  const unsigned long long mx = wave_max(active ? (unsigned long long)iters : 0ull);
+  if (lane0()) {
+    if (na) atomicAdd(&d->n_interfaces, na);
+    if (nc) atomicAdd(&d->n_not_converged, nc);
+    if (nw) atomicAdd(&d->n_wave_order_bad, nw);
+    if (nb) atomicAdd(&d->n_bn_zero, nb);
+    if (nf) atomicAdd(&d->n_hlle_fallback, nf);
+    if (ncv) atomicAdd(&d->n_converged, ncv);
+    if (si) atomicAdd(&d->sum_iters_converged, si);
+    if (mx) atomicMax(&d->max_iters, mx);
+  }
+}
+
+struct PrimPtrs {
+  const double* p[9];
+};
+struct ConsPtrs {
+  double* p[8];
+};
+
+__device__ inline void load_prim(const PrimPtrs& s, int64_t i, Prim& P) {
+  P.rho = s.p[0][i];
+  P.v[0] = s.p[1][i];
+  P.v[1] = s.p[2][i];
+  P.v[2] = s.p[3][i];
+  P.p = s.p[4][i];
+  P.eps = s.p[5][i];
+  P.B[0] = s.p[6][i];
+  P.B[1] = s.p[7][i];
+  P.B[2] = s.p[8][i];
+}
+
+__global__ void k_zero_u64(unsigned long long* w, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = 0ull;
+}
+
+// ── flat flux (flux_fn) ─────────────────────────────────────────────────────
+
+template <int N>
+__global__ __launch_bounds__(BLK) void k_flux_a(int64_t n, PrimPtrs L, PrimPtrs R, srmhd_eos e,
+                                                C2PRec* rec, srmhd_flux_diag* d) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  uint64_t m = 0;
+  if (i < n) {
+    Prim l, r;
+    load_prim(L, i, l);
+    load_prim(R, i, r);
+    HlldPrep<N> h;
+    h.run(l, r, e);
+    Kastaun k;
+    k.init(h.uH, e);
+    m = c2p_pass_a(k, rec[i]);
+  }
+  m = wave_or(m);
+  if (lane0() && m) atomicOr(&d->c2p_mask, (unsigned long long)m);
+}
+
+template <int N, int S>
+__global__ __launch_bounds__(BLK) void k_flux_b(int64_t n, PrimPtrs L, PrimPtrs R, srmhd_eos e,
+                                                const C2PRec* rec, srmhd_flux_diag* d,
+                                                ConsPtrs F, ConsPtrs U, int want_u,
+                                                double* pstar, uint8_t* status) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const bool active = i < n;
+  int st = 0, iters = 0;
+  if (active) {
+    Prim l, r;
+    load_prim(L, i, l);
+    load_prim(R, i, r);
+    double f[8], u[8], ps;
+    if (S == SOLVER_HLLE) {
+      hlle<N>(l, r, e, f, u, ps);
+    } else if (S == SOLVER_HLLC) {
+      hllc<N>(l, r, e, f, u, ps);
+    } else {
+      HlldPrep<N> h;
+      h.run(l, r, e);
+      Kastaun k;
+      k.init(h.uH, e);
+      Prim ph;
+      c2p_pass_b(k, rec[i], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
+      st = hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, u, ps, iters);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) F.p[k][i] = f[k];
+    if (want_u) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) U.p[k][i] = u[k];
+    }
+    if (pstar) pstar[i] = ps;
+    if (status) status[i] = (uint8_t)st;
+  }
+  if (S == SOLVER_HLLD) diag_accumulate(d, active, st, iters);
+}
+
+// ── flat con2prim / prim2con ─────────────────────────────────────────────────
+
+struct CConsPtrs {
+  const double* p[8];
+};
+
+__device__ inline void load_cons(const CConsPtrs& c, int64_t i, double u[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) u[k] = c.p[k][i];
+}
+
+__global__ __launch_bounds__(BLK) void k_c2p_a(int64_t n, CConsPtrs C, srmhd_eos e, C2PRec* rec,
+                                               unsigned long long* mask) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  uint64_t m = 0;
+  if (i < n) {
+    double u[8];
+    load_cons(C, i, u);
+    Kastaun k;
+    k.init(u, e);
+    m = c2p_pass_a(k, rec[i]);
+  }
+  m = wave_or(m);
+  if (lane0() && m) atomicOr(mask, (unsigned long long)m);
+}
+
+struct OutPrimPtrs {
+  double* p[9];
+};
+
+__global__ __launch_bounds__(BLK) void k_c2p_b(int64_t n, CConsPtrs C, srmhd_eos e,
+                                               const C2PRec* rec, const unsigned long long* mask,
+                                               double atmo_rho, double atmo_eps, OutPrimPtrs P,
+                                               uint8_t* status) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  double u[8];
+  load_cons(C, i, u);
+  Kastaun k;
+  k.init(u, e);
+  Prim q;
+  const int st = c2p_pass_b(k, rec[i], c2p_break_iter(*mask), atmo_rho, atmo_eps, q);
+  P.p[0][i] = q.rho;
+  P.p[1][i] = q.v[0];
+  P.p[2][i] = q.v[1];
+  P.p[3][i] = q.v[2];
+  P.p[4][i] = q.p;
+  P.p[5][i] = q.eps;
+  P.p[6][i] = q.B[0];
+  P.p[7][i] = q.B[1];
+  P.p[8][i] = q.B[2];
+  if (status) status[i] = (uint8_t)st;
+}
+
+__global__ __launch_bounds__(BLK) void k_p2c(int64_t n, PrimPtrs P, ConsPtrs C) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= n) return;
+  Prim q;
+  load_prim(P, i, q);
+  double c[8];
+  prim_to_cons(q, c);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) C.p[k][i] = c[k];
+}
+
+// ── 2D ──────────────────────────────────────────────────────────────────────
+
+struct G2 {
+  int nx, ny, ng, nxt, nyt, bcx, bcy;
+  double dx, dy;
+};
+
+struct St2 {  // D, Sx, Sy, Sz, tau, Bz, Bxf, Byf
+  const double* p[8];
+};
+struct St2Out {
+  double* p[8];
+};
+struct Pr2 {  // rho, vx, vy, vz, p, eps
+  const double* p[6];
+};
+struct Pr2Out {
+  double* p[6];
+};
+struct Flx {  // D, Sx, Sy, Sz, tau, Bz, E-component
+  double* p[7];
+};
+
+// BC source index of a cell-centred axis (grid.py:93-113)
+__device__ inline int src_cell(int i, int n, int ng, int bc) {
+  if (i < ng) return bc == SRMHD_BC_PERIODIC ? i + n : ng;
+  if (i >= ng + n) return bc == SRMHD_BC_PERIODIC ? i - n : ng + n - 1;
+  return i;
+}
+// BC source index of a face-staggered axis along its normal (grid.py:116-143)
+__device__ inline int src_face(int f, int n, int ng, int bc) {
+  if (bc == SRMHD_BC_PERIODIC) {
+    if (f < ng) return f + n;
+    if (f == ng + n) return ng;
+    if (f > ng + n) return f - n;
+    return f;
+  }
+  if (f < ng) return ng;
+  if (f > ng + n) return ng + n;
+  return f;
+}
+
+__device__ inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// cell-centred B from the staggered field (ct.py:44-54)
+__device__ inline double bcx(const G2& g, const double* Bxf, int i, int j) {
+  return 0.5 * (Bxf[(int64_t)i * g.nyt + j] + Bxf[(int64_t)(i + 1) * g.nyt + j]);
+}
+__device__ inline double bcy(const G2& g, const double* Byf, int i, int j) {
+  return 0.5 * (Byf[(int64_t)i * (g.nyt + 1) + j] + Byf[(int64_t)i * (g.nyt + 1) + j + 1]);
+}
+
+// Reconstructed face states of face (i, j) of the sweep along N, as
+// reconstruct_prims (reconstruction.py:272-338, vel_var "Wv") followed by
+// B_n := face value (sweep.py:82-89).
+template <int N, int LIM>
+__device__ void face_prims2d(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, int i,
+                             int j, Prim& L, Prim& R) {
+  constexpr int w = stencil_halfwidth(LIM);
+  constexpr int k0 = 3 - w, k1 = 4 + w;
+  const int n = N == 0 ? g.nxt : g.nyt;
+  const int f = N == 0 ? i : j;
+  auto cidx = [&](int k) -> int64_t {
+    const int c = clampi(f + k - 4, 0, n - 1);
+    return N == 0 ? (int64_t)c * g.nyt + j : (int64_t)i * g.nyt + c;
+  };
+  auto ccoord = [&](int k, int& ci, int& cj) {
+    const int c = clampi(f + k - 4, 0, n - 1);
+    ci = N == 0 ? c : i;
+    cj = N == 0 ? j : c;
+  };
+  double q[8];
+  double ql[7], qr[7];   // rho, p, zx, zy, zz, Bt1, Bt2
+
+  // rho and p first: they decide the PCM fallback of every variable
+#pragma unroll
+  for (int v = 0; v < 2; ++v) {
+    const double* a = P.p[v == 0 ? 0 : 4];
+#pragma unroll
+    for (int k = k0; k <= k1; ++k) q[k] = a[cidx(k)];
+    face_states<LIM>(q, f, n, ql[v], qr[v]);
+  }
+  const bool bad = (ql[0] <= 1e-12) || (qr[0] <= 1e-12) || (ql[1] <= 1e-14) || (qr[1] <= 1e-14);
+  if (bad) {
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const double* a = P.p[v == 0 ? 0 : 4];
+      ql[v] = a[cidx(f == 0 ? 4 : 3)] + 0.5 * 0.0;
+      qr[v] = a[cidx(f == n ? 3 : 4)] - 0.5 * 0.0;
+    }
+  }
+  // W v with W from clamp(1 - v^2, 1e-14)
+  double W[8];
+#pragma unroll
+  for (int k = k0; k <= k1; ++k) {
+    const int64_t c = cidx(k);
+    const double vx = P.p[1][c], vy = P.p[2][c], vz = P.p[3][c];
+    W[k] = 1.0 / csqrt(clo(1.0 - (sq(vx) + sq(vy) + sq(vz)), 1e-14));
+  }
+  // 5 more variables: zx, zy, zz, B_t1, B_t2
+#pragma unroll
+  for (int v = 2; v < 7; ++v) {
+#pragma unroll
+    for (int k = k0; k <= k1; ++k) {
+      double val;
+      if (v <= 4) {
+        val = W[k] * P.p[v - 1][cidx(k)];
+      } else {
+        // N=0: t1 = y (faces), t2 = z (cons);  N=1: t1 = z (cons), t2 = x (faces)
+        const bool fromBz = (N == 0) ? (v == 6) : (v == 5);
+        if (fromBz) {
+          val = S.p[5][cidx(k)];
+        } else {
+          int ci, cj;
+          ccoord(k, ci, cj);
+          val = (N == 0) ? bcy(g, S.p[7], ci, cj) : bcx(g, S.p[6], ci, cj);
+        }
+      }
+      q[k] = val;
+    }
+    if (bad) {
+      // PCM: the window slots of cells max(f-1, 0) and min(f, n-1)
+      ql[v] = (f == 0 ? q[4] : q[3]) + 0.5 * 0.0;
+      qr[v] = (f == n ? q[3] : q[4]) - 0.5 * 0.0;
+    } else {
+      face_states<LIM>(q, f, n, ql[v], qr[v]);
+    }
+  }
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+  const double Bn = N == 0 ? S.p[6][(int64_t)i * g.nyt + j] : S.p[7][(int64_t)i * (g.nyt + 1) + j];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const double* x = s ? qr : ql;
+    Prim& Q = s ? R : L;
+    const double z2 = sq(x[2]) + sq(x[3]) + sq(x[4]);
+    const double fac = 1.0 / csqrt(1.0 + z2);
+    Q.v[0] = x[2] * fac;
+    Q.v[1] = x[3] * fac;
+    Q.v[2] = x[4] * fac;
+    Q.rho = clo(x[0], 1e-12);
+    Q.p = clo(x[1], 1e-14);
+    Q.eps = eos_eps(e, Q.p, Q.rho);
+    Q.B[N] = Bn;
+    Q.B[T1] = x[5];
+    Q.B[T2] = x[6];
+  }
+}
+
+template <int N>
+__device__ inline void face_ij(const G2& g, int64_t t, int& i, int& j) {
+  const int ny = N == 0 ? g.nyt : g.nyt + 1;
+  i = (int)(t / ny);
+  j = (int)(t - (int64_t)i * ny);
+}
+
+template <int N>
+__device__ inline int64_t n_faces(const G2& g) {
+  return N == 0 ? (int64_t)(g.nxt + 1) * g.nyt : (int64_t)g.nxt * (g.nyt + 1);
+}
+
+template <int N, int LIM>
+__global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e, C2PRec* rec,
+                                                 srmhd_flux_diag* d) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  uint64_t m = 0;
+  if (t < n_faces<N>(g)) {
+    int i, j;
+    face_ij<N>(g, t, i, j);
+    Prim l, r;
+    face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
+    HlldPrep<N> h;
+    h.run(l, r, e);
+    Kastaun k;
+    k.init(h.uH, e);
+    m = c2p_pass_a(k, rec[t]);
+  }
+  m = wave_or(m);
+  if (lane0() && m) atomicOr(&d->c2p_mask, (unsigned long long)m);
+}
+
+template <int N, int LIM, int SOL>
+__global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e,
+                                                 const C2PRec* rec, srmhd_flux_diag* d, Flx F) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const bool active = t < n_faces<N>(g);
+  int st = 0, iters = 0;
+  double fD = 0.0;
+  if (active) {
+    int i, j;
+    face_ij<N>(g, t, i, j);
+    Prim l, r;
+    face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
+    double f[8], u[8], ps;
+    if (SOL == SOLVER_HLLE) {
+      hlle<N>(l, r, e, f, u, ps);
+    } else if (SOL == SOLVER_HLLC) {
+      hllc<N>(l, r, e, f, u, ps);
+    } else {
+      HlldPrep<N> h;
+      h.run(l, r, e);
+      Kastaun k;
+      k.init(h.uH, e);
+      Prim ph;
+      c2p_pass_b(k, rec[t], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
+      st = hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, nullptr, ps, iters);
+    }
+    F.p[0][t] = f[CD];
+    F.p[1][t] = f[CS];
+    F.p[2][t] = f[CS + 1];
+    F.p[3][t] = f[CS + 2];
+    F.p[4][t] = f[CT];
+    F.p[5][t] = f[CB + 2];
+    F.p[6][t] = N == 0 ? f[CB + 1] : f[CB + 0];   // F^x(By) / F^y(Bx): the face EMF
+    fD = f[CD];
+  }
+  if (SOL == SOLVER_HLLD) diag_accumulate(d, active, st, iters);
+  const unsigned long long mb = wave_max(active ? dbits(fabs(fD)) : 0ull);
+  if (lane0() && mb) atomicMax(&d->absF_D_max, mb);
+}
+
+// Gardiner-Stone corner EMF (ct.py:119-177) with the ring of zeros
+__global__ __launch_bounds__(BLK) void k_corner(G2 g, St2 S, Pr2 P, const double* FxD,
+                                                const double* FxB, const double* FyD,
+                                                const double* FyB, const srmhd_flux_diag* dg,
+                                                int upwind, double* Ez) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const int nc = g.nyt + 1;
+  if (t >= (int64_t)(g.nxt + 1) * nc) return;
+  const int i = (int)(t / nc), j = (int)(t - (int64_t)i * nc);
+  if (i == 0 || j == 0 || i == g.nxt || j == g.nyt) {
+    Ez[t] = 0.0;
+    return;
+  }
+  const double* Bxf = S.p[6];
+  const double* Byf = S.p[7];
+  auto Ec = [&](int ci, int cj) {
+    const int64_t c = (int64_t)ci * g.nyt + cj;
+    return bcx(g, Bxf, ci, cj) * P.p[2][c] - bcy(g, Byf, ci, cj) * P.p[1][c];
+  };
+  const int64_t yf = (int64_t)i * (g.nyt + 1) + j, yfm = (int64_t)(i - 1) * (g.nyt + 1) + j;
+  const int64_t xf = (int64_t)i * g.nyt + j, xfm = (int64_t)i * g.nyt + j - 1;
+  const double eyf = FyB[yf], eyfm = FyB[yfm];
+  const double exf = -FxB[xf], exfm = -FxB[xfm];
+  const double NE = Ec(i, j), NW = Ec(i - 1, j), SE = Ec(i, j - 1), SW = Ec(i - 1, j - 1);
+  const double avg = 0.25 * ((eyf + eyfm) + (exf + exfm));
+  if (!upwind) {
+    Ez[t] = avg;
+    return;
+  }
+  const double thx = 1e-9 * __longlong_as_double(dg[0].absF_D_max);
+  const double thy = 1e-9 * __longlong_as_double(dg[1].absF_D_max);
+  auto sgn = [](double F, double th) { return (fabs(F) > th) ? tsign(F) : 0.0; };
+  const double Sx = sgn(FxD[xf], thx), Sxm = sgn(FxD[xfm], thx);
+  const double Sy = sgn(FyD[yf], thy), Sym = sgn(FyD[yfm], thy);
+  const double dN = (1.0 - Sx) * (NE - eyf) + (1.0 + Sx) * (NW - eyfm);
+  const double dS = (1.0 - Sxm) * (eyf - SE) + (1.0 + Sxm) * (eyfm - SW);
+  const double dEdy = 0.125 * (dS - dN);
+  const double dE = (1.0 - Sy) * (NE - exf) + (1.0 + Sy) * (SE - exfm);
+  const double dW = (1.0 - Sym) * (exf - NW) + (1.0 + Sym) * (exfm - SW);
+  const double dEdx = 0.125 * (dW - dE);
+  Ez[t] = avg + (dEdy + dEdx);
+}
+
+// out = a*U0 + b*U + (c*dt)*L(U) with the flux divergence (sweep.py:122-135),
+// ct_rhs (ct.py:180-185), state.combine (state.py:77-103) and the face then
+// cell BCs of sync_state (grid.py:229-255), each ghost taking its source.
+__global__ __launch_bounds__(BLK) void k_update(G2 g, St2 U0, St2 U, const Flx Fx, const Flx Fy,
+                                                const double* Ez, double a, double b, double c,
+                                                const double* dt_dev, St2Out out) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const int nc = g.nyt + 1;
+  if (t >= (int64_t)(g.nxt + 1) * nc) return;
+  const int i = (int)(t / nc), j = (int)(t - (int64_t)i * nc);
+  const double w = c * (*dt_dev);
+  if (i < g.nxt && j < g.nyt) {
+    const int si = src_cell(i, g.nx, g.ng, g.bcx), sj = src_cell(j, g.ny, g.ng, g.bcy);
+    const int64_t cs = (int64_t)si * g.nyt + sj;
+    const int64_t xlo = (int64_t)si * g.nyt + sj, xhi = (int64_t)(si + 1) * g.nyt + sj;
+    const int64_t ylo = (int64_t)si * (g.nyt + 1) + sj, yhi = ylo + 1;
+    const int64_t cd = (int64_t)i * g.nyt + j;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double rx = -(Fx.p[k][xhi] - Fx.p[k][xlo]) / g.dx;
+      const double ry = -(Fy.p[k][yhi] - Fy.p[k][ylo]) / g.dy;
+      out.p[k][cd] = a * U0.p[k][cs] + b * U.p[k][cs] + w * (rx + ry);
+    }
+  }
+  if (j < g.nyt) {   // x-face (i, j), i in [0, nxt]
+    const int fi = src_face(i, g.nx, g.ng, g.bcx), sj = src_cell(j, g.ny, g.ng, g.bcy);
+    const int64_t s = (int64_t)fi * g.nyt + sj;
+    const double dB = (Ez[(int64_t)fi * nc + sj] - Ez[(int64_t)fi * nc + sj + 1]) / g.dy;
+    out.p[6][(int64_t)i * g.nyt + j] = a * U0.p[6][s] + b * U.p[6][s] + w * dB;
+  }
+  if (i < g.nxt) {   // y-face (i, j), j in [0, nyt]
+    const int si = src_cell(i, g.nx, g.ng, g.bcx), fj = src_face(j, g.ny, g.ng, g.bcy);
+    const int64_t s = (int64_t)si * (g.nyt + 1) + fj;
+    const double dB = (Ez[(int64_t)(si + 1) * nc + fj] - Ez[(int64_t)si * nc + fj]) / g.dx;
+    out.p[7][(int64_t)i * (g.nyt + 1) + j] = a * U0.p[7][s] + b * U.p[7][s] + w * dB;
+  }
+}
+
+// ghost faces and cells of a state in place (the sync path)
+__global__ __launch_bounds__(BLK) void k_bc_state(G2 g, St2Out U) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const int nc = g.nyt + 1;
+  if (t >= (int64_t)(g.nxt + 1) * nc) return;
+  const int i = (int)(t / nc), j = (int)(t - (int64_t)i * nc);
+  if (i < g.nxt && j < g.nyt) {
+    const int si = src_cell(i, g.nx, g.ng, g.bcx), sj = src_cell(j, g.ny, g.ng, g.bcy);
+    if (si != i || sj != j) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        U.p[k][(int64_t)i * g.nyt + j] = U.p[k][(int64_t)si * g.nyt + sj];
+    }
+  }
+  if (j < g.nyt) {
+    const int fi = src_face(i, g.nx, g.ng, g.bcx), sj = src_cell(j, g.ny, g.ng, g.bcy);
+    if (fi != i || sj != j) U.p[6][(int64_t)i * g.nyt + j] = U.p[6][(int64_t)fi * g.nyt + sj];
+  }
+  if (i < g.nxt) {
+    const int si = src_cell(i, g.nx, g.ng, g.bcx), fj = src_face(j, g.ny, g.ng, g.bcy);
+    if (si != i || fj != j)
+      U.p[7][(int64_t)i * (g.nyt + 1) + j] = U.p[7][(int64_t)si * (g.nyt + 1) + fj];
+  }
+}
+
+// con2prim input of cell (i, j): cons with B from the faces (driver2d.py:208-212)
+__device__ inline void cell_cons(const G2& g, const St2& S, int i, int j, double u[8]) {
+  const int64_t c = (int64_t)i * g.nyt + j;
+  u[CD] = S.p[0][c];
+  u[CS] = S.p[1][c];
+  u[CS + 1] = S.p[2][c];
+  u[CS + 2] = S.p[3][c];
+  u[CT] = S.p[4][c];
+  u[CB] = bcx(g, S.p[6], i, j);
+  u[CB + 1] = bcy(g, S.p[7], i, j);
+  u[CB + 2] = S.p[5][c];
+}
+
+__global__ __launch_bounds__(BLK) void k_c2p2d_a(G2 g, St2 S, srmhd_eos e, C2PRec* rec,
+                                                 unsigned long long* mask) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  uint64_t m = 0;
+  if (t < (int64_t)g.nxt * g.nyt) {
+    const int i = (int)(t / g.nyt), j = (int)(t - (int64_t)i * g.nyt);
+    double u[8];
+    cell_cons(g, S, i, j, u);
+    Kastaun k;
+    k.init(u, e);
+    m = c2p_pass_a(k, rec[t]);
+  }
+  m = wave_or(m);
+  if (lane0() && m) atomicOr(mask, (unsigned long long)m);
+}
+
+// pass B on the physical cells only: ghost primitives are overwritten by the
+// primitive BCs (driver2d.py:216)
+__global__ __launch_bounds__(BLK) void k_c2p2d_b(G2 g, St2 S, srmhd_eos e, const C2PRec* rec,
+                                                 const unsigned long long* mask, double atmo_rho,
+                                                 double atmo_eps, Pr2Out P) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (t >= (int64_t)g.nx * g.ny) return;
+  const int pi = (int)(t / g.ny), pj = (int)(t - (int64_t)pi * g.ny);
+  const int i = pi + g.ng, j = pj + g.ng;
+  double u[8];
+  cell_cons(g, S, i, j, u);
+  Kastaun k;
+  k.init(u, e);
+  Prim q;
+  const int64_t c = (int64_t)i * g.nyt + j;
+  c2p_pass_b(k, rec[c], c2p_break_iter(*mask), atmo_rho, atmo_eps, q);
+  P.p[0][c] = q.rho;
+  P.p[1][c] = q.v[0];
+  P.p[2][c] = q.v[1];
+  P.p[3][c] = q.v[2];
+  P.p[4][c] = q.p;
+  P.p[5][c] = q.eps;
+}
+
+__global__ __launch_bounds__(BLK) void k_prim_bc(G2 g, Pr2Out P) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (t >= (int64_t)g.nxt * g.nyt) return;
+  const int i = (int)(t / g.nyt), j = (int)(t - (int64_t)i * g.nyt);
+  const int si = src_cell(i, g.nx, g.ng, g.bcx), sj = src_cell(j, g.ny, g.ng, g.bcy);
+  if (si == i && sj == j) return;
+  const int64_t s = (int64_t)si * g.nyt + sj;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) P.p[k][t] = P.p[k][s];
+}
+
+// compute_dt_2d (driver2d.py:52-82): wave maxima of |v_x| + c_f, |v_y| + c_f
+// over the physical cells, as non-negative double bit patterns
+struct Pr9 {  // rho, vx, vy, vz, p, eps, Bx, By, Bz (Bx, By, Bz may be null)
+  const double* p[9];
+};
+
+__global__ __launch_bounds__(BLK) void k_dt_reduce(G2 g, St2 S, Pr9 P, srmhd_eos e,
+                                                   unsigned long long* mx) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  unsigned long long bx = 0, by = 0;
+  if (t < (int64_t)g.nx * g.ny) {
+    const int i = (int)(t / g.ny) + g.ng, j = (int)(t % g.ny) + g.ng;
+    const int64_t c = (int64_t)i * g.nyt + j;
+    const double rho = P.p[0][c], vx = P.p[1][c], vy = P.p[2][c], vz = P.p[3][c];
+    const double eps = P.p[5][c];
+    const double Bx = P.p[6] ? P.p[6][c] : bcx(g, S.p[6], i, j);
+    const double By = P.p[7] ? P.p[7][c] : bcy(g, S.p[7], i, j);
+    const double Bz = P.p[8] ? P.p[8][c] : S.p[5][c];
+    double p, cs2;
+    eos_press_cs2(e, eps, rho, p, cs2);
+    const double v2 = sq(vx) + sq(vy) + sq(vz);
+    const double W = 1.0 / csqrt(clo(1.0 - v2, 1e-10));
+    const double B2 = sq(Bx) + sq(By) + sq(Bz);
+    const double vB = vx * Bx + vy * By + vz * Bz;
+    const double b2 = (B2 + vB * vB) / (W * W);
+    const double h = 1.0 + eps + p / rho;
+    const double v02 = cs2 + (b2 / (rho * h + b2)) * (1.0 - cs2);
+    const double cf = csqrt(chi_(clo(v02, 0.0), 1.0));
+    bx = dbits(fabs(vx) + cf);
+    by = dbits(fabs(vy) + cf);
+    // fabs of a NaN keeps it above +inf in the unsigned order (torch.max propagates NaN)
+    bx &= 0x7fffffffffffffffull;
+    by &= 0x7fffffffffffffffull;
+  }
+  bx = wave_max(bx);
+  by = wave_max(by);
+  if (lane0()) {
+    atomicMax(&mx[0], bx);
+    atomicMax(&mx[1], by);
+  }
+}
+
+__global__ void k_dt_final(const unsigned long long* mx, double cfl, double dx, double dy,
+                           double* dt) {
+  if (threadIdx.x || blockIdx.x) return;
+  const double sx = clo(__longlong_as_double(mx[0]), 1e-14);
+  const double sy = clo(__longlong_as_double(mx[1]), 1e-14);
+  // torch evaluates `cfl / tensor` as reciprocal(tensor) * cfl: two roundings
+  *dt = (1.0 / (sx / dx + sy / dy)) * cfl;
+}
+
+__global__ void k_copy_diag(const srmhd_flux_diag* src, srmhd_flux_diag* dst, int n) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// ── host side ───────────────────────────────────────────────────────────────
+
+inline int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
+
+struct Carver {
+  char* base;
+  int64_t off = 0;
+  template <typename T>
+  T* take(int64_t count) {
+    T* p = (T*)(base + off);
+    off += align256(count * (int64_t)sizeof(T));
+    return p;
+  }
+};
+
+bool eos_ok(const srmhd_eos* e) { return e && e->gamma > 1.0; }
+
+template <int N>
+int flux_dispatch(int solver, int64_t n, const PrimPtrs& L, const PrimPtrs& R, const srmhd_eos& e,
+                  C2PRec* rec, srmhd_flux_diag* d, const ConsPtrs& F, const ConsPtrs& U,
+                  int want_u, double* pstar, uint8_t* status, hipStream_t s) {
+  const int nb = blocks_for(n);
+  switch (solver) {
+    case SOLVER_HLLE:
+      k_flux_b<N, SOLVER_HLLE><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d, F, U, want_u, pstar, status);
+      break;
+    case SOLVER_HLLC:
+      k_flux_b<N, SOLVER_HLLC><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d, F, U, want_u, pstar, status);
+      break;
+    default:
+      k_flux_a<N><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d);
+      k_flux_b<N, SOLVER_HLLD><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d, F, U, want_u, pstar, status);
+  }
+  return launch_status();
+}
+
+G2 make_g2(const srmhd_grid2d* g) {
+  G2 r;
+  r.nx = g->nx;
+  r.ny = g->ny;
+  r.ng = g->ng;
+  r.nxt = g->nx + 2 * g->ng;
+  r.nyt = g->ny + 2 * g->ng;
+  r.bcx = g->bc_x;
+  r.bcy = g->bc_y;
+  r.dx = g->dx;
+  r.dy = g->dy;
+  return r;
+}
+
+bool grid_ok(const srmhd_grid2d* g, int limiter) {
+  if (!g || g->nx < 1 || g->ny < 1 || g->ng < 1) return false;
+  const int need = limiter == SRMHD_PCM ? 0 : (limiter <= SRMHD_MINMOD ? 1 : (limiter == SRMHD_MP7 ? 3 : 2));
+  if (g->ng < (need + 1 > 2 ? need + 1 : 2) && limiter != SRMHD_PCM) return false;
+  for (int bc : {g->bc_x, g->bc_y})
+    if (bc != SRMHD_BC_OUTFLOW && bc != SRMHD_BC_PERIODIC) return false;
+  if (g->bc_x == SRMHD_BC_PERIODIC && g->nx <= g->ng) return false;
+  if (g->bc_y == SRMHD_BC_PERIODIC && g->ny <= g->ng) return false;
+  return g->dx > 0.0 && g->dy > 0.0;
+}
+
+struct Ws2 {
+  Flx Fx, Fy;
+  double* Ez;
+  C2PRec *recx, *recy, *recc;
+  srmhd_flux_diag* diag;        // [0] x sweep, [1] y sweep
+  unsigned long long* words;    // [0] cell c2p mask, [1..2] dt maxima
+  int64_t bytes;
+};
+
+Ws2 carve2d(const G2& g, void* work) {
+  Carver c{(char*)work};
+  Ws2 w;
+  const int64_t nfx = (int64_t)(g.nxt + 1) * g.nyt, nfy = (int64_t)g.nxt * (g.nyt + 1);
+  const int64_t ncell = (int64_t)g.nxt * g.nyt, ncor = (int64_t)(g.nxt + 1) * (g.nyt + 1);
+  w.diag = c.take<srmhd_flux_diag>(2);
+  w.words = c.take<unsigned long long>(4);
+  for (int k = 0; k < 7; ++k) w.Fx.p[k] = c.take<double>(nfx);
+  for (int k = 0; k < 7; ++k) w.Fy.p[k] = c.take<double>(nfy);
+  w.Ez = c.take<double>(ncor);
+  w.recx = c.take<C2PRec>(nfx);
+  w.recy = c.take<C2PRec>(nfy);
+  w.recc = c.take<C2PRec>(ncell);
+  w.bytes = c.off;
+  return w;
+}
+
+constexpr int N_WORDS = (int)(2 * sizeof(srmhd_flux_diag) / 8);
+
+// the two sweeps of ct_stage for one (limiter, solver)
+template <int LIM, int SOL>
+void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const Ws2& w,
+            hipStream_t s) {
+  const int bx = blocks_for((int64_t)(g.nxt + 1) * g.nyt);
+  const int by = blocks_for((int64_t)g.nxt * (g.nyt + 1));
+  if (SOL == SOLVER_HLLD) k_sweep_a<0, LIM><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0);
+  k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, w.Fx);
+  if (SOL == SOLVER_HLLD) k_sweep_a<1, LIM><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1);
+  k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, w.Fy);
+}
+
+template <int LIM>
+void sweeps_solver(int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
+                   const Ws2& w, hipStream_t s) {
+  switch (sol) {
+    case SOLVER_HLLE: sweeps<LIM, SOLVER_HLLE>(g, S, P, e, w, s); break;
+    case SOLVER_HLLC: sweeps<LIM, SOLVER_HLLC>(g, S, P, e, w, s); break;
+    default: sweeps<LIM, SOLVER_HLLD>(g, S, P, e, w, s);
+  }
+}
+
+void sweeps_dispatch(int lim, int sol, const G2& g, const St2& S, const Pr2& P,
+                     const srmhd_eos& e, const Ws2& w, hipStream_t s) {
+  switch (lim) {
+    case LIM_PCM: sweeps_solver<LIM_PCM>(sol, g, S, P, e, w, s); break;
+    case LIM_MC: sweeps_solver<LIM_MC>(sol, g, S, P, e, w, s); break;
+    case LIM_MINMOD: sweeps_solver<LIM_MINMOD>(sol, g, S, P, e, w, s); break;
+    case LIM_WENO5Z: sweeps_solver<LIM_WENO5Z>(sol, g, S, P, e, w, s); break;
+    case LIM_MP5: sweeps_solver<LIM_MP5>(sol, g, S, P, e, w, s); break;
+    default: sweeps_solver<LIM_MP7>(sol, g, S, P, e, w, s);
+  }
+}
+
+// con2prim of every cell of state S as one batch, then primitive BCs
+void c2p_cells(const G2& g, const St2& S, const srmhd_eos& e, double atmo_rho, double atmo_eps,
+               const Ws2& w, const Pr2Out& P, hipStream_t s) {
+  const int64_t ncell = (int64_t)g.nxt * g.nyt;
+  k_c2p2d_a<<<blocks_for(ncell), BLK, 0, s>>>(g, S, e, w.recc, w.words + 0);
+  k_c2p2d_b<<<blocks_for((int64_t)g.nx * g.ny), BLK, 0, s>>>(g, S, e, w.recc, w.words + 0,
+                                                             atmo_rho, atmo_eps, P);
+  k_prim_bc<<<blocks_for(ncell), BLK, 0, s>>>(g, P);
+}
+
+}  // namespace
+
+// ── C ABI ───────────────────────────────────────────────────────────────────
+
+extern "C" {
+
+int64_t srmhd_flux_workspace_bytes(int solver, int64_t n) {
+  if (n < 0) return -1;
+  return align256(sizeof(srmhd_flux_diag)) +
+         (solver == SRMHD_HLLD ? align256(n * (int64_t)sizeof(C2PRec)) : 0);
+}
+
+int srmhd_flux(int solver, int idir, int64_t n, const double* const* primL,
+               const double* const* primR, const srmhd_eos* eos, double* const* F,
+               double* const* U, double* pstar, uint8_t* status, srmhd_flux_diag* diag,
+               void* work, int64_t work_bytes, void* stream) {
+  if (solver < 0 || solver > 2 || idir < 0 || idir > 2 || n < 0 || !primL || !primR || !F ||
+      !eos_ok(eos) || !work)
+    return SRMHD_EARG;
+  if (work_bytes < srmhd_flux_workspace_bytes(solver, n)) return SRMHD_EARG;
+  hipStream_t s = (hipStream_t)stream;
+  Carver c{(char*)work};
+  srmhd_flux_diag* d = c.take<srmhd_flux_diag>(1);
+  C2PRec* rec = solver == SRMHD_HLLD ? c.take<C2PRec>(n) : nullptr;
+  k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)d, (int)(sizeof(srmhd_flux_diag) / 8));
+  if (n > 0) {
+    PrimPtrs L, R;
+    for (int k = 0; k < 9; ++k) {
+      L.p[k] = primL[k];
+      R.p[k] = primR[k];
+      if (!L.p[k] || !R.p[k]) return SRMHD_EARG;
+    }
+    ConsPtrs Fp, Up;
+    for (int k = 0; k < 8; ++k) {
+      Fp.p[k] = F[k];
+      Up.p[k] = U ? U[k] : nullptr;
+      if (!Fp.p[k] || (U && !Up.p[k])) return SRMHD_EARG;
+    }
+    const int want_u = U ? 1 : 0;
+    int rc;
+    if (idir == 0)
+      rc = flux_dispatch<0>(solver, n, L, R, *eos, rec, d, Fp, Up, want_u, pstar, status, s);
+    else if (idir == 1)
+      rc = flux_dispatch<1>(solver, n, L, R, *eos, rec, d, Fp, Up, want_u, pstar, status, s);
+    else
+      rc = flux_dispatch<2>(solver, n, L, R, *eos, rec, d, Fp, Up, want_u, pstar, status, s);
+    if (rc) return rc;
+  }
+  if (diag) k_copy_diag<<<1, 64, 0, s>>>(d, diag, 1);
+  return launch_status();
+}
+
+int64_t srmhd_c2p_workspace_bytes(int64_t n) {
+  if (n < 0) return -1;
+  return align256(8 * 8) + align256(n * (int64_t)sizeof(C2PRec));
+}
+
+int srmhd_c2p(int64_t n, const double* const* cons, const srmhd_eos* eos, double atmo_rho,
+              double atmo_eps, double* const* prim, uint8_t* status, void* work,
+              int64_t work_bytes, void* stream) {
+  if (n < 0 || !cons || !prim || !eos_ok(eos) || !work) return SRMHD_EARG;
+  if (work_bytes < srmhd_c2p_workspace_bytes(n)) return SRMHD_EARG;
+  hipStream_t s = (hipStream_t)stream;
+  Carver c{(char*)work};
+  unsigned long long* mask = c.take<unsigned long long>(8);
+  C2PRec* rec = c.take<C2PRec>(n);
+  if (n == 0) return SRMHD_OK;
+  CConsPtrs C;
+  OutPrimPtrs P;
+  for (int k = 0; k < 8; ++k)
+    if (!(C.p[k] = cons[k])) return SRMHD_EARG;
+  for (int k = 0; k < 9; ++k)
+    if (!(P.p[k] = prim[k])) return SRMHD_EARG;
+  k_zero_u64<<<1, 64, 0, s>>>(mask, 1);
+  k_c2p_a<<<blocks_for(n), BLK, 0, s>>>(n, C, *eos, rec, mask);
+  k_c2p_b<<<blocks_for(n), BLK, 0, s>>>(n, C, *eos, rec, mask, atmo_rho, atmo_eps, P, status);
+  return launch_status();
+}
+
+int srmhd_p2c(int64_t n, const double* const* prim, double* const* cons, void* stream) {
+  if (n < 0 || !prim || !cons) return SRMHD_EARG;
+  if (n == 0) return SRMHD_OK;
+  PrimPtrs P;
+  ConsPtrs C;
+  for (int k = 0; k < 9; ++k)
+    if (!(P.p[k] = prim[k])) return SRMHD_EARG;
+  for (int k = 0; k < 8; ++k)
+    if (!(C.p[k] = cons[k])) return SRMHD_EARG;
+  k_p2c<<<blocks_for(n), BLK, 0, (hipStream_t)stream>>>(n, P, C);
+  return launch_status();
+}
+
+int64_t srmhd2d_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s) {
+  if (!g || !s || !grid_ok(g, s->limiter)) return -1;
+  return carve2d(make_g2(g), nullptr).bytes;
+}
+
+static bool scheme_ok(const srmhd_scheme2d* s) {
+  return s && s->solver >= 0 && s->solver <= 2 && s->limiter >= 0 && s->limiter <= 5;
+}
+
+int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_eos* eos,
+                  double a, double b, double c, const double* dt_dev,
+                  const double* const* U0, const double* const* U, const double* const* P,
+                  double* const* out, double* const* Pout, srmhd_flux_diag* diag_xy,
+                  void* work, int64_t work_bytes, void* stream) {
+  if (!scheme_ok(sp) || !grid_ok(gp, sp->limiter) || !eos_ok(eos) || !dt_dev || !U0 || !U ||
+      !P || !out || !Pout || !work)
+    return SRMHD_EARG;
+  const G2 g = make_g2(gp);
+  const Ws2 w = carve2d(g, work);
+  if (work_bytes < w.bytes) return SRMHD_EARG;
+  St2 s0, s1, so;
+  St2Out o;
+  Pr2 pr, pro;
+  Pr2Out po;
+  for (int k = 0; k < 8; ++k) {
+    s0.p[k] = U0[k];
+    s1.p[k] = U[k];
+    o.p[k] = out[k];
+    so.p[k] = out[k];
+    if (!s0.p[k] || !s1.p[k] || !o.p[k]) return SRMHD_EARG;
+    if (o.p[k] == s0.p[k] || o.p[k] == s1.p[k]) return SRMHD_EARG;
+  }
+  for (int k = 0; k < 6; ++k) {
+    pr.p[k] = P[k];
+    po.p[k] = Pout[k];
+    pro.p[k] = Pout[k];
+    if (!pr.p[k] || !po.p[k] || po.p[k] == pr.p[k]) return SRMHD_EARG;
+  }
+  (void)pro;
+  hipStream_t st = (hipStream_t)stream;
+  k_zero_u64<<<1, 64, 0, st>>>((unsigned long long*)w.diag, N_WORDS);
+  k_zero_u64<<<1, 64, 0, st>>>(w.words, 4);
+  sweeps_dispatch(sp->limiter, sp->solver, g, s1, pr, *eos, w, st);
+  const int64_t ncor = (int64_t)(g.nxt + 1) * (g.nyt + 1);
+  k_corner<<<blocks_for(ncor), BLK, 0, st>>>(g, s1, pr, w.Fx.p[0], w.Fx.p[6], w.Fy.p[0],
+                                             w.Fy.p[6], w.diag, sp->upwind, w.Ez);
+  k_update<<<blocks_for(ncor), BLK, 0, st>>>(g, s0, s1, w.Fx, w.Fy, w.Ez, a, b, c, dt_dev, o);
+  c2p_cells(g, so, *eos, sp->atmo_rho, sp->atmo_eps, w, po, st);
+  if (diag_xy) k_copy_diag<<<1, 64, 0, st>>>(w.diag, diag_xy, 2);
+  return launch_status();
+}
+
+int srmhd2d_sync(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_eos* eos,
+                 double* const* U, double* const* P, void* work, int64_t work_bytes,
+                 void* stream) {
+  if (!scheme_ok(sp) || !grid_ok(gp, sp->limiter) || !eos_ok(eos) || !U || !P || !work)
+    return SRMHD_EARG;
+  const G2 g = make_g2(gp);
+  const Ws2 w = carve2d(g, work);
+  if (work_bytes < w.bytes) return SRMHD_EARG;
+  St2 s;
+  St2Out so;
+  Pr2Out po;
+  for (int k = 0; k < 8; ++k) {
+    s.p[k] = U[k];
+    so.p[k] = U[k];
+    if (!U[k]) return SRMHD_EARG;
+  }
+  for (int k = 0; k < 6; ++k)
+    if (!(po.p[k] = P[k])) return SRMHD_EARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t ncor = (int64_t)(g.nxt + 1) * (g.nyt + 1);
+  k_zero_u64<<<1, 64, 0, st>>>(w.words, 4);
+  k_bc_state<<<blocks_for(ncor), BLK, 0, st>>>(g, so);
+  c2p_cells(g, s, *eos, sp->atmo_rho, sp->atmo_eps, w, po, st);
+  return launch_status();
+}
+
+int srmhd2d_dt(const srmhd_grid2d* gp, const srmhd_eos* eos, double cfl, const double* const* U,
+               const double* const* P, double* dt_dev, void* work, int64_t work_bytes,
+               void* stream) {
+  if (!gp || gp->nx < 1 || gp->ny < 1 || !eos_ok(eos) || !P || !dt_dev || !work)
+    return SRMHD_EARG;
+  const G2 g = make_g2(gp);
+  const Ws2 w = carve2d(g, work);
+  if (work_bytes < w.bytes) return SRMHD_EARG;
+  St2 s;
+  Pr9 pr;
+  for (int k = 0; k < 9; ++k) pr.p[k] = P[k];
+  for (int k = 0; k < 6; ++k)
+    if (!pr.p[k]) return SRMHD_EARG;
+  const bool need_u = !pr.p[6] || !pr.p[7] || !pr.p[8];
+  for (int k = 0; k < 8; ++k) {
+    s.p[k] = U ? U[k] : nullptr;
+    if (need_u && !s.p[k]) return SRMHD_EARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  k_zero_u64<<<1, 64, 0, st>>>(w.words + 1, 2);
+  k_dt_reduce<<<blocks_for((int64_t)g.nx * g.ny), BLK, 0, st>>>(g, s, pr, *eos, w.words + 1);
+  k_dt_final<<<1, 64, 0, st>>>(w.words + 1, cfl, g.dx, g.dy, dt_dev);
+  return launch_status();
+}
+
+}  // extern "C"

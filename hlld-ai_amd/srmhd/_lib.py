@@ -1,0 +1,135 @@
+"""ctypes binding of the C ABI in include/srmhd.h.
+
+The library is loaded from ``_lib/libsrmhd_hip.so`` next to this file.  There
+is no fallback: if the library or a HIP device is missing, every entry point
+raises ``RuntimeError``.
+"""
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libsrmhd_hip.so")
+
+SOLVER_IDS = {"hlle": 0, "hllc": 1, "hlld": 2}
+LIMITER_IDS = {"pcm": 0, "mc": 1, "minmod": 2, "weno5z": 3, "mp5": 4, "mp7": 5}
+BC_IDS = {"outflow": 0, "periodic": 1}
+
+EARG = 1
+EHIP = 1000
+
+
+class Eos(ctypes.Structure):
+    _fields_ = [("K", ctypes.c_double), ("gamma", ctypes.c_double), ("gamma_th", ctypes.c_double),
+                ("is_cold", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class FluxDiag(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_ulonglong) for n in (
+        "n_interfaces", "n_not_converged", "n_wave_order_bad", "n_bn_zero", "n_hlle_fallback",
+        "sum_iters_converged", "n_converged", "max_iters", "c2p_mask", "absF_D_max")]
+
+
+class Grid2D(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("ng", ctypes.c_int32),
+                ("bc_x", ctypes.c_int32), ("bc_y", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double)]
+
+
+class Scheme2D(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int32), ("limiter", ctypes.c_int32), ("upwind", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("atmo_rho", ctypes.c_double),
+                ("atmo_eps", ctypes.c_double)]
+
+
+DIAG_WORDS = ctypes.sizeof(FluxDiag) // 8
+
+_lib = None
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_PP = ctypes.POINTER(ctypes.c_void_p)
+
+_SIGS = {
+    "srmhd_flux_workspace_bytes": (_I64, [ctypes.c_int, _I64]),
+    "srmhd_flux": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _I64, _PP, _PP, ctypes.POINTER(Eos),
+                                  _PP, _PP, _P, _P, _P, _P, _I64, _P]),
+    "srmhd_c2p_workspace_bytes": (_I64, [_I64]),
+    "srmhd_c2p": (ctypes.c_int, [_I64, _PP, ctypes.POINTER(Eos), _D, _D, _PP, _P, _P, _I64, _P]),
+    "srmhd_p2c": (ctypes.c_int, [_I64, _PP, _PP, _P]),
+    "srmhd2d_workspace_bytes": (_I64, [ctypes.POINTER(Grid2D), ctypes.POINTER(Scheme2D)]),
+    "srmhd2d_stage": (ctypes.c_int, [ctypes.POINTER(Grid2D), ctypes.POINTER(Scheme2D),
+                                     ctypes.POINTER(Eos), _D, _D, _D, _P, _PP, _PP, _PP, _PP, _PP,
+                                     _P, _P, _I64, _P]),
+    "srmhd2d_sync": (ctypes.c_int, [ctypes.POINTER(Grid2D), ctypes.POINTER(Scheme2D),
+                                    ctypes.POINTER(Eos), _PP, _PP, _P, _I64, _P]),
+    "srmhd2d_dt": (ctypes.c_int, [ctypes.POINTER(Grid2D), ctypes.POINTER(Eos), _D, _PP, _PP, _P,
+                                  _P, _I64, _P]),
+}
+EXPORTED = tuple(_SIGS)
+
+
+def load_library(path=LIB_PATH):
+    """Load the shared library and declare the prototypes (no GPU needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def lib():
+    """The library, for a process that is about to launch kernels."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("srmhd HIP path needs a HIP device (torch.cuda.is_available() is False)")
+    return load_library()
+
+
+def check(rc, what):
+    if rc == 0:
+        return
+    if rc == EARG:
+        raise ValueError(f"{what}: invalid argument")
+    raise RuntimeError(f"{what}: HIP error {rc - EHIP}")
+
+
+def ptrs(tensors):
+    """A C array of device pointers (kept alive by the returned object)."""
+    arr = (ctypes.c_void_p * len(tensors))(*[None if t is None else t.data_ptr() for t in tensors])
+    return ctypes.cast(arr, _PP), arr
+
+
+def stream_ptr(device=None):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def f64(t, device):
+    """A contiguous float64 tensor on ``device`` (no copy when already so)."""
+    if not torch.is_tensor(t):
+        t = torch.as_tensor(t, dtype=torch.float64)
+    return t.to(device=device, dtype=torch.float64).contiguous()
+
+
+def workspace(nbytes, device):
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+
+
+def diag_to_dict(words, solver, idir):
+    """FluxDiag words (a host int64 tensor) -> the reference's LAST_DIAG keys
+    (hlld.py:1516-1529)."""
+    w = [int(x) & 0xFFFFFFFFFFFFFFFF for x in words.tolist()]
+    n, nnc, nwo, nbz, nfb, si, ncv, mx = w[:8]
+    return dict(solver=solver, idir=idir, n_interfaces=n, n_not_converged=nnc,
+                n_wave_order_bad=nwo, n_bn_zero=nbz, n_hlle_fallback=nfb,
+                frac_hlle_fallback=nfb / max(n, 1),
+                mean_iters=(si / ncv) if ncv else float("nan"),
+                pstar_method="secant", pstar_cost=None, max_iters=mx)

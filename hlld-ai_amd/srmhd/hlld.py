@@ -1,0 +1,181 @@
+"""The flux_fn plugins on the HIP path: drop-in replacements of the
+reference's ``hlle_flux`` / ``hllc_flux`` / ``hlld_flux``
+(src/physics/hlld.py:1005, 1549, 1202) with the same contract
+
+    flux_fn(sL, sR, eos, idir=0) -> (F, U, p_star)
+
+sL, sR: dicts rho, vx, vy, vz, p, eps, Bx, By, Bz of equal-shape float64
+device tensors; F, U: dicts D, tau, Sx, Sy, Sz, Bx, By, Bz; p_star < 0 marks
+an HLLE fallback.  ``LAST_DIAG`` is refreshed by every hlld_flux call like
+the reference's module global (hlld.py:1516-1529); its values are read from
+the device on first access, so a call does not synchronise the stream.
+"""
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .eos import as_c
+
+PRIM_KEYS = ("rho", "vx", "vy", "vz", "p", "eps", "Bx", "By", "Bz")
+CONS_KEYS = ("D", "tau", "Sx", "Sy", "Sz", "Bx", "By", "Bz")
+
+
+class _LazyDiag(dict):
+    """LAST_DIAG whose values come from device counters when first read."""
+
+    def __init__(self):
+        super().__init__()
+        self._pending = None
+
+    def _set_pending(self, words, solver, idir):
+        self._pending = (words, solver, idir)
+
+    def _fill(self):
+        if self._pending is not None:
+            words, solver, idir = self._pending
+            self._pending = None
+            dict.clear(self)
+            dict.update(self, _lib.diag_to_dict(words.cpu(), solver, idir))
+
+    def __getitem__(self, k):
+        self._fill()
+        return dict.__getitem__(self, k)
+
+    def get(self, k, default=None):
+        self._fill()
+        return dict.get(self, k, default)
+
+    def __iter__(self):
+        self._fill()
+        return dict.__iter__(self)
+
+    def __len__(self):
+        self._fill()
+        return dict.__len__(self)
+
+    def __contains__(self, k):
+        self._fill()
+        return dict.__contains__(self, k)
+
+    def keys(self):
+        self._fill()
+        return dict.keys(self)
+
+    def items(self):
+        self._fill()
+        return dict.items(self)
+
+    def values(self):
+        self._fill()
+        return dict.values(self)
+
+    def copy(self):
+        self._fill()
+        return dict(dict.items(self))
+
+    def __repr__(self):
+        self._fill()
+        return dict.__repr__(self)
+
+    def update(self, *a, **kw):
+        self._fill()
+        dict.update(self, *a, **kw)
+
+
+LAST_DIAG = _LazyDiag()
+
+
+def _flux(solver, sL, sR, eos, idir, want_status=False):
+    if idir not in (0, 1, 2):
+        raise ValueError(f"idir must be 0, 1 or 2, got {idir!r}")
+    L = sL["rho"]
+    dev = L.device
+    if dev.type != "cuda":
+        raise RuntimeError("srmhd flux_fn needs device tensors (the HIP path has no CPU fallback)")
+    shape = L.shape
+    n = L.numel()
+    lib = _lib.lib()
+    pl = [_lib.f64(sL[k], dev).reshape(-1) for k in PRIM_KEYS]
+    pr = [_lib.f64(sR[k], dev).reshape(-1) for k in PRIM_KEYS]
+    for t in pl + pr:
+        if t.numel() != n:
+            raise ValueError("all state tensors must have the same number of elements")
+    F = [torch.empty(n, dtype=torch.float64, device=dev) for _ in CONS_KEYS]
+    U = [torch.empty(n, dtype=torch.float64, device=dev) for _ in CONS_KEYS]
+    ps = torch.empty(n, dtype=torch.float64, device=dev)
+    st = torch.empty(n, dtype=torch.uint8, device=dev)
+    diag = torch.zeros(_lib.DIAG_WORDS, dtype=torch.int64, device=dev)
+    sid = _lib.SOLVER_IDS[solver]
+    ws = _lib.workspace(lib.srmhd_flux_workspace_bytes(sid, n), dev)
+    a_l, keep_l = _lib.ptrs(pl)
+    a_r, keep_r = _lib.ptrs(pr)
+    a_f, keep_f = _lib.ptrs(F)
+    a_u, keep_u = _lib.ptrs(U)
+    e = as_c(eos)
+    with torch.cuda.device(dev):
+        rc = lib.srmhd_flux(sid, idir, n, a_l, a_r, ctypes.byref(e), a_f, a_u, ps.data_ptr(),
+                            st.data_ptr(), diag.data_ptr(), ws.data_ptr(), ws.numel(),
+                            _lib.stream_ptr(dev))
+    _lib.check(rc, f"srmhd_flux({solver})")
+    if solver == "hlld":
+        LAST_DIAG._set_pending(diag, "hlld", idir)
+    Fd = {k: F[i].reshape(shape) for i, k in enumerate(CONS_KEYS)}
+    Ud = {k: U[i].reshape(shape) for i, k in enumerate(CONS_KEYS)}
+    out = (Fd, Ud, ps.reshape(shape))
+    return out + (st.reshape(shape),) if want_status else out
+
+
+def hlle_flux(sL, sR, eos, idir=0):
+    """HLLE (hlld.py:1005-1044); p_star is the HLL total-pressure estimate."""
+    return _flux("hlle", sL, sR, eos, idir)
+
+
+def hllc_flux(sL, sR, eos, idir=0):
+    """HLLC (hlld.py:1549-1765)."""
+    return _flux("hllc", sL, sR, eos, idir)
+
+
+def hlld_flux(sL, sR, eos, idir=0, lo_fac=0.1, hi_fac=10):
+    """HLLD (hlld.py:1202-1543).  ``lo_fac``/``hi_fac`` are accepted and,
+    as in the reference, unused."""
+    return _flux("hlld", sL, sR, eos, idir)
+
+
+def flux_with_status(solver, sL, sR, eos, idir=0):
+    """(F, U, p_star, status) with per-face status bits: 1 p* not converged,
+    2 wave order, 4 B_n = 0, 8 HLLE fallback (HLLD only)."""
+    return _flux(solver, sL, sR, eos, idir, want_status=True)
+
+
+def primitive_to_conserved(prim):
+    """hlld.py:185-217 on the device."""
+    L = prim["rho"]
+    dev = L.device
+    if dev.type != "cuda":
+        raise RuntimeError("srmhd primitive_to_conserved needs device tensors")
+    shape, n = L.shape, L.numel()
+    p = [_lib.f64(prim[k], dev).reshape(-1) for k in PRIM_KEYS]
+    c = [torch.empty(n, dtype=torch.float64, device=dev) for _ in CONS_KEYS]
+    a_p, kp = _lib.ptrs(p)
+    a_c, kc = _lib.ptrs(c)
+    with torch.cuda.device(dev):
+        rc = _lib.lib().srmhd_p2c(n, a_p, a_c, _lib.stream_ptr(dev))
+    _lib.check(rc, "srmhd_p2c")
+    return {k: c[i].reshape(shape) for i, k in enumerate(CONS_KEYS)}
+
+
+SOLVERS = {"hlle": hlle_flux, "hllc": hllc_flux, "hlld": hlld_flux}
+
+
+def solver_name(flux_fn):
+    """Map a flux_fn (this module's, the reference's, or a name) to its name."""
+    if isinstance(flux_fn, str):
+        name = flux_fn
+    else:
+        name = getattr(flux_fn, "__name__", "")
+        name = {"hlle_flux": "hlle", "hllc_flux": "hllc", "hlld_flux": "hlld"}.get(name, name)
+    if name not in SOLVERS:
+        raise ValueError(f"unsupported flux_fn {flux_fn!r}: expected hlle, hllc or hlld")
+    return name

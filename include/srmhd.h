@@ -1,0 +1,137 @@
+/*
+ * srmhd.h — C ABI of the MI355X (gfx950) SRMHD finite-volume step.
+ *
+ * Every entry point takes plain device pointers (float64 structure-of-arrays,
+ * contiguous), sizes, and a HIP stream (hipStream_t passed as void*; NULL =
+ * the null stream).  The caller owns every buffer, including the workspace
+ * whose size the matching *_workspace_bytes() returns; nothing is allocated
+ * inside.  All calls are asynchronous on the stream.  Return value: 0 on
+ * success, SRMHD_EARG for a bad argument, otherwise SRMHD_EHIP + the
+ * hipError_t of the failing launch.  Calls that share a workspace must not
+ * overlap.
+ *
+ * Variable orders (the reference's dict keys):
+ *   primitives  rho, vx, vy, vz, p, eps, Bx, By, Bz            (9)
+ *   conserved   D, tau, Sx, Sy, Sz, Bx, By, Bz                 (8, tau form)
+ *   2D state    D, Sx, Sy, Sz, tau, Bz, Bxf, Byf               (8: the reference's
+ *               EVOLVED_KEYS, src/physics/state.py:41, then the staggered field)
+ *   2D prims    rho, vx, vy, vz, p, eps                         (6; B is derived
+ *               from the faces and from Bz)
+ *
+ * References are to kemi2805/HLLD-AI.
+ */
+#ifndef SRMHD_H
+#define SRMHD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SRMHD_OK 0
+#define SRMHD_EARG 1
+#define SRMHD_EHIP 1000
+
+/* hybrid_eos (src/physics/eos.py:15-20): p = K rho^g + (eps - eps_cold) rho (g_th - 1);
+ * is_cold = 1 when gamma_th is None or 1. */
+typedef struct srmhd_eos {
+  double K, gamma, gamma_th;
+  int32_t is_cold;
+  int32_t reserved;
+} srmhd_eos;
+
+/* Device-resident counters of one flux call (the reference's hlld.LAST_DIAG,
+ * src/physics/hlld.py:1516-1529).  Filled by srmhd_flux and per sweep by
+ * srmhd2d_stage; read with a device-to-host copy when wanted. */
+typedef struct srmhd_flux_diag {
+  unsigned long long n_interfaces;
+  unsigned long long n_not_converged;
+  unsigned long long n_wave_order_bad;
+  unsigned long long n_bn_zero;
+  unsigned long long n_hlle_fallback;
+  unsigned long long sum_iters_converged;
+  unsigned long long n_converged;
+  unsigned long long max_iters;
+  unsigned long long c2p_mask;      /* OR of the HLL-state con2prim masks */
+  unsigned long long absF_D_max;    /* bits of max |F_D| (upwind-EMF deadband scale) */
+} srmhd_flux_diag;
+
+enum { SRMHD_HLLE = 0, SRMHD_HLLC = 1, SRMHD_HLLD = 2 };
+enum { SRMHD_PCM = 0, SRMHD_MC = 1, SRMHD_MINMOD = 2, SRMHD_WENO5Z = 3, SRMHD_MP5 = 4,
+       SRMHD_MP7 = 5 };
+enum { SRMHD_BC_OUTFLOW = 0, SRMHD_BC_PERIODIC = 1 };
+
+/* ── point-wise flux: the flux_fn plugin ──────────────────────────────────────
+ * Replaces hlle_flux / hllc_flux / hlld_flux (src/physics/hlld.py:1005, 1549,
+ * 1202) behind the contract flux_fn(sL, sR, eos, idir) -> (F, U, p_star)
+ * (src/physics/sweep.py:96, driver.py:168).  n interfaces; primL/primR: 9
+ * pointers; F, U: 8 pointers (U may be NULL); pstar, status (uint8 per face:
+ * bit0 not converged, bit1 wave order, bit2 B_n = 0, bit3 HLLE fallback) may
+ * be NULL; diag (device) may be NULL.  HLLD runs the reference's
+ * batch-coupled con2prim of the HLL state over all n faces as one batch. */
+int64_t srmhd_flux_workspace_bytes(int solver, int64_t n);
+int srmhd_flux(int solver, int idir, int64_t n, const double* const* primL,
+               const double* const* primR, const srmhd_eos* eos, double* const* F,
+               double* const* U, double* pstar, uint8_t* status, srmhd_flux_diag* diag,
+               void* work, int64_t work_bytes, void* stream);
+
+/* ── con2prim ────────────────────────────────────────────────────────────────
+ * Replaces conservative_to_primitive (src/physics/c2p.py:313-363) over one
+ * batch of n cells: cons 8 pointers in, prim 9 pointers out, status (uint8:
+ * bit0 bracket_ok, bit1 converged, bit2 v_clamped, bit3 atmo_floored; may be
+ * NULL). */
+int64_t srmhd_c2p_workspace_bytes(int64_t n);
+int srmhd_c2p(int64_t n, const double* const* cons, const srmhd_eos* eos, double atmo_rho,
+              double atmo_eps, double* const* prim, uint8_t* status, void* work,
+              int64_t work_bytes, void* stream);
+
+/* primitive_to_conserved (src/physics/hlld.py:185-217): prim 9 -> cons 8. */
+int srmhd_p2c(int64_t n, const double* const* prim, double* const* cons, void* stream);
+
+/* ── 2D constrained-transport step ───────────────────────────────────────────
+ * Grid2D (src/physics/grid.py:146-206): cells (nxt, nyt) with nxt = nx+2ng,
+ * x-faces (nxt+1, nyt), y-faces (nxt, nyt+1), row-major with y contiguous. */
+typedef struct srmhd_grid2d {
+  int32_t nx, ny, ng, bc_x, bc_y, reserved;
+  double dx, dy;
+} srmhd_grid2d;
+
+typedef struct srmhd_scheme2d {
+  int32_t solver, limiter, upwind, reserved;
+  double atmo_rho, atmo_eps;
+} srmhd_scheme2d;
+
+int64_t srmhd2d_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s);
+
+/* One RK stage of rk_step_ct (src/physics/driver2d.py:241-247 = ct_stage +
+ * state.combine + sync_state): out = a*U0 + b*U + (c*dt)*L(U, P), then face
+ * and cell BCs, con2prim of every cell (one batch), primitive BCs.  dt is read
+ * from device memory (*dt_dev).  U0, U, out: 8 pointers (2D state order);
+ * P, Pout: 6 pointers.  out/Pout must not alias U0/U/P.  diag_xy (device, may
+ * be NULL) receives the x and y sweep counters (2 structs). */
+int srmhd2d_stage(const srmhd_grid2d* g, const srmhd_scheme2d* s, const srmhd_eos* eos,
+                  double a, double b, double c, const double* dt_dev,
+                  const double* const* U0, const double* const* U, const double* const* P,
+                  double* const* out, double* const* Pout, srmhd_flux_diag* diag_xy,
+                  void* work, int64_t work_bytes, void* stream);
+
+/* sync_state (src/physics/driver2d.py:196-218) in place on state U (face BCs,
+ * cell BCs) and into P: the initial-data and restart path. */
+int srmhd2d_sync(const srmhd_grid2d* g, const srmhd_scheme2d* s, const srmhd_eos* eos,
+                 double* const* U, double* const* P, void* work, int64_t work_bytes,
+                 void* stream);
+
+/* compute_dt_2d (src/physics/driver2d.py:52-82) into *dt_dev.  P: 9 pointers
+ * rho, vx, vy, vz, p, eps, Bx, By, Bz (cell-centred).  When P[6] and P[7] are
+ * NULL, Bx and By are the face averages of U's staggered field and Bz is U's
+ * Bz; U may be NULL otherwise. */
+int srmhd2d_dt(const srmhd_grid2d* g, const srmhd_eos* eos, double cfl,
+               const double* const* U, const double* const* P, double* dt_dev, void* work,
+               int64_t work_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SRMHD_H */

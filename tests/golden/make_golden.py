@@ -1,0 +1,281 @@
+"""Write the golden vectors that pin the oracle to the reference solver.
+
+Run in a container that holds the reference checkout (default
+/root/reference, or the path in $HLLD_AI_REFERENCE):
+
+    python tests/golden/make_golden.py
+
+It imports the reference's own functions for exactly the parts that no
+closed form or invariant pins (SURVEY of the reference: hlld_flux/hllc_flux
+branch decisions and values, the batch-coupled Kastaun con2prim,
+reconstruct_prims with its PCM fallback, rk_step_ct, driver.run on
+Balsara 1, run_tubes), feeds them inputs generated here from fixed seeds,
+and stores inputs and outputs as .npz data.  No reference source is copied;
+the fixtures are data only.  The GPU tests never need the reference: they
+read these files.
+"""
+
+import os
+import sys
+from pathlib import Path
+
+import numpy as np
+import torch
+
+HERE = Path(__file__).resolve().parent
+REF = os.environ.get("HLLD_AI_REFERENCE", "/root/reference")
+sys.path.insert(0, REF)
+
+from src.physics import hlld as R_hlld                               # noqa: E402
+from src.physics.c2p import conservative_to_primitive                # noqa: E402
+from src.physics.driver import run as R_run                          # noqa: E402
+from src.physics.driver2d import compute_dt_2d, prims_to_cons_2d, rk_step_ct, sync_state  # noqa: E402
+from src.physics.eos import hybrid_eos                               # noqa: E402
+from src.physics.grid import Grid2D                                  # noqa: E402
+from src.physics.initial_data2d import b_from_potential, magnetic_rotor  # noqa: E402
+from src.physics.reconstruction import ghosts_needed, reconstruct_prims  # noqa: E402
+from src.physics.state import EVOLVED_KEYS, State2D                  # noqa: E402
+from src.physics.tube_seed import run_tubes                          # noqa: E402
+
+PRIM9 = ("rho", "vx", "vy", "vz", "p", "eps", "Bx", "By", "Bz")
+CONS8 = ("D", "tau", "Sx", "Sy", "Sz", "Bx", "By", "Bz")
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def random_prims(n, eos, rng, max_W=5.0, bscale=3.0):
+    """Admissible random primitive states, sampled through the Lorentz factor."""
+    rho = 10.0 ** rng.uniform(-1.0, 1.0, n)
+    p = 10.0 ** rng.uniform(-1.5, 1.0, n)
+    W = 1.0 + 10.0 ** rng.uniform(-3.0, np.log10(max_W - 1.0), n)
+    sp = np.sqrt(1.0 - 1.0 / W ** 2)
+    th = rng.uniform(0.0, np.pi, n)
+    ph = rng.uniform(0.0, 2.0 * np.pi, n)
+    B = rng.uniform(-bscale, bscale, (3, n))
+    s = {"rho": rho, "p": p, "vx": sp * np.sin(th) * np.cos(ph),
+         "vy": sp * np.sin(th) * np.sin(ph), "vz": sp * np.cos(th),
+         "Bx": B[0], "By": B[1], "Bz": B[2]}
+    s = {k: torch.from_numpy(v.copy()) for k, v in s.items()}
+    s["eps"] = eos.eps__press_rho(s["p"], s["rho"])
+    return s
+
+
+def gen_flux():
+    eos = hybrid_eos(K=0.0, gamma=5.0 / 3.0, gamma_th=5.0 / 3.0)
+    rng = np.random.default_rng(20260824)
+    n = 384
+    sL = random_prims(n, eos, rng)
+    sR = random_prims(n, eos, rng)
+    # mild jumps: a second batch near-continuous, where HLLD rarely falls back
+    sR2 = {k: v.clone() for k, v in sL.items()}
+    for k in ("rho", "p"):
+        sR2[k] = sL[k] * torch.from_numpy(1.0 + 0.2 * rng.uniform(-1, 1, n))
+    for k in ("vx", "vy", "vz"):
+        sR2[k] = sL[k] * torch.from_numpy(1.0 + 0.1 * rng.uniform(-1, 1, n))
+    sR2["eps"] = eos.eps__press_rho(sR2["p"], sR2["rho"])
+    for k in ("Bx", "By", "Bz"):
+        sR[k] = sL[k]
+        sR2[k] = sL[k]
+    out = {}
+    for tag, right in (("a", sR), ("b", sR2)):
+        for k in PRIM9:
+            out[f"{tag}_L_{k}"] = _np(sL[k])
+            out[f"{tag}_R_{k}"] = _np(right[k])
+        for name, fn in (("hlle", R_hlld.hlle_flux), ("hllc", R_hlld.hllc_flux),
+                         ("hlld", R_hlld.hlld_flux)):
+            for idir in (0, 1, 2):
+                F, U, ps = fn(sL, right, eos, idir=idir)
+                for k in CONS8:
+                    out[f"{tag}_{name}{idir}_F_{k}"] = _np(F[k])
+                    out[f"{tag}_{name}{idir}_U_{k}"] = _np(U[k])
+                out[f"{tag}_{name}{idir}_pstar"] = _np(ps)
+                if name == "hlld":
+                    d = R_hlld.LAST_DIAG
+                    out[f"{tag}_{name}{idir}_diag"] = np.array(
+                        [d["n_not_converged"], d["n_wave_order_bad"], d["n_bn_zero"],
+                         d["n_hlle_fallback"], d["max_iters"]], dtype=np.int64)
+                    out[f"{tag}_{name}{idir}_mean_iters"] = np.array(d["mean_iters"])
+    np.savez_compressed(HERE / "flux_random.npz", **out)
+
+
+def gen_c2p():
+    eos = hybrid_eos(K=0.0, gamma=5.0 / 3.0, gamma_th=5.0 / 3.0)
+    rng = np.random.default_rng(7)
+    parts = [random_prims(96, eos, rng, max_W=2.0), random_prims(96, eos, rng, max_W=10.0),
+             random_prims(32, eos, rng, max_W=2.0, bscale=0.0)]
+    hot = {"rho": torch.full((16,), 0.1), "p": torch.full((16,), 1.0),
+           "vx": torch.full((16,), 0.99), "vy": torch.zeros(16), "vz": torch.zeros(16),
+           "Bx": torch.from_numpy(rng.uniform(-0.5, 0.5, 16)), "By": torch.zeros(16),
+           "Bz": torch.zeros(16)}
+    hot = {k: v.double() for k, v in hot.items()}
+    hot["eps"] = eos.eps__press_rho(hot["p"], hot["rho"])
+    parts.append(hot)
+    P = {k: torch.cat([q[k] for q in parts]) for k in PRIM9}
+    cons = R_hlld.primitive_to_conserved(P)
+    cons = {k: v.clone() for k, v in cons.items()}
+    # a few cells pushed off the admissible set (|S| > D + tau, tau < 0)
+    cons["Sx"][:4] = cons["Sx"][:4] * 3.0
+    cons["tau"][4:8] = -1e-3
+    out = {f"in_{k}": _np(v) for k, v in cons.items()}
+    prims, status = conservative_to_primitive({k: v.clone() for k, v in cons.items()}, eos,
+                                              return_status=True)
+    for k in PRIM9:
+        out[f"out_{k}"] = _np(prims[k])
+    for k, v in status.items():
+        out[f"st_{k}"] = _np(v)
+    # the hot batch alone: a batch where the secant leaves cells unconverged
+    sub = {k: v[-16:].clone() for k, v in cons.items()}
+    p2, s2 = conservative_to_primitive(sub, eos, return_status=True)
+    for k in PRIM9:
+        out[f"hot_{k}"] = _np(p2[k])
+    out["hot_converged"] = _np(s2["converged"])
+    np.savez_compressed(HERE / "c2p_batch.npz", **out)
+
+
+def gen_recon():
+    eos = hybrid_eos(K=0.0, gamma=5.0 / 3.0, gamma_th=5.0 / 3.0)
+    rng = np.random.default_rng(11)
+    shp = (13, 11)
+    P = {"rho": 10.0 ** rng.uniform(-1, 1, shp), "p": 10.0 ** rng.uniform(-1, 1, shp),
+         "vx": rng.uniform(-0.6, 0.6, shp), "vy": rng.uniform(-0.6, 0.6, shp),
+         "vz": rng.uniform(-0.3, 0.3, shp), "Bx": rng.uniform(-1, 1, shp),
+         "By": rng.uniform(-1, 1, shp), "Bz": rng.uniform(-1, 1, shp)}
+    P["rho"][6, 4] = 1e-13          # forces the PCM fallback around it
+    P = {k: torch.from_numpy(v.copy()) for k, v in P.items()}
+    P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
+    out = {f"in_{k}": _np(v) for k, v in P.items()}
+    for lim in ("pcm", "mc", "minmod", "weno5z", "mp5", "mp7"):
+        for axis in (0, 1):
+            L, Rr = reconstruct_prims(P, axis=axis, eos=eos, limiter=lim)
+            for k in PRIM9:
+                out[f"{lim}{axis}_L_{k}"] = _np(L[k])
+                out[f"{lim}{axis}_R_{k}"] = _np(Rr[k])
+    np.savez_compressed(HERE / "recon_block.npz", **out)
+
+
+def _tapered(g, eos):
+    X, Y = g.X, g.Y
+    r = torch.sqrt(X * X + Y * Y)
+    f = torch.where(r <= 0.1, torch.ones_like(r),
+                    torch.where(r < 0.115, (0.115 - r) / 0.015, torch.zeros_like(r)))
+    z = torch.zeros(g.shape, dtype=torch.float64)
+    om = 9.95 * f
+    P = {"rho": 1.0 + 9.0 * f, "p": torch.full_like(z, 1.0), "vx": -om * Y, "vy": om * X,
+         "vz": z.clone(), "Bx": torch.full_like(z, 1.0), "By": z.clone(), "Bz": z.clone()}
+    P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
+    YF = g.yf.unsqueeze(0).expand(g.nxt + 1, g.nyt + 1)
+    return P, 1.0 * YF
+
+
+def _ot2pi(g, eos, A=1.0):
+    X, Y = g.X, g.Y
+    amp = 0.99 / np.sqrt(2.0)
+    z = torch.zeros(g.shape, dtype=torch.float64)
+    P = {"rho": torch.full_like(z, 1.0), "p": torch.full_like(z, 1.0),
+         "vx": -amp * torch.sin(Y), "vy": amp * torch.sin(X), "vz": z.clone(),
+         "Bx": -A * torch.sin(Y), "By": A * torch.sin(2.0 * X), "Bz": z.clone()}
+    P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
+    XF = g.xf.unsqueeze(1).expand(g.nxt + 1, g.nyt + 1)
+    YF = g.yf.unsqueeze(0).expand(g.nxt + 1, g.nyt + 1)
+    return P, A * (torch.cos(YF) + 0.5 * torch.cos(2.0 * XF))
+
+
+def _state_arrays(prefix, st, out):
+    for k in EVOLVED_KEYS:
+        out[f"{prefix}cons_{k}"] = _np(st.cons[k])
+    out[f"{prefix}Bxf"] = _np(st.Bxf)
+    out[f"{prefix}Byf"] = _np(st.Byf)
+    for k in PRIM9:
+        out[f"{prefix}prim_{k}"] = _np(st.prims[k])
+
+
+def _run2d(name, problem, n, limiter, solver, scheme, steps, cfl, gamma, box, bc):
+    eos = hybrid_eos(K=0.0, gamma=gamma, gamma_th=gamma)
+    g = Grid2D(box[0], box[1], n, box[0], box[1], n, ng=ghosts_needed(limiter))
+    if problem == "rotor":
+        prims, Az = magnetic_rotor(g, eos)
+    elif problem == "tapered":
+        prims, Az = _tapered(g, eos)
+    else:
+        prims, Az = _ot2pi(g, eos)
+    Bxf, Byf = b_from_potential(Az, g.dx, g.dy)
+    c = prims_to_cons_2d(prims, g)
+    st = State2D(cons={k: c[k] for k in EVOLVED_KEYS}, Bxf=Bxf, Byf=Byf, prims=prims)
+    st = sync_state(st, g, eos, *bc)
+    out = {}
+    _state_arrays("init_", st, out)
+    flux = {"hlld": R_hlld.hlld_flux, "hllc": R_hlld.hllc_flux, "hlle": R_hlld.hlle_flux}[solver]
+    dts = []
+    for _ in range(steps):
+        dt = compute_dt_2d(st.prims, g, eos, cfl)
+        dts.append(dt)
+        st, _ = rk_step_ct(st, g, eos, dt, scheme=scheme, bc_x=bc[0], bc_y=bc[1],
+                           flux_fn=flux, limiter=limiter)
+    out["dts"] = np.array(dts)
+    _state_arrays("fin_", st, out)
+    np.savez_compressed(HERE / f"{name}.npz", **out)
+
+
+def gen_2d():
+    r = (-0.5, 0.5)
+    oo = ("outflow", "outflow")
+    _run2d("rotor_sharp_mc_hlld", "rotor", 24, "mc", "hlld", "rk3", 2, 0.4, 5.0 / 3.0, r, oo)
+    _run2d("rotor_tapered_weno5z_hlld", "tapered", 20, "weno5z", "hlld", "rk3", 1, 0.4,
+           5.0 / 3.0, r, oo)
+    _run2d("rotor_tapered_mp5_hllc", "tapered", 16, "mp5", "hllc", "rk3", 1, 0.4, 5.0 / 3.0, r, oo)
+    _run2d("rotor_sharp_mc_hlle", "rotor", 16, "mc", "hlle", "rk2", 1, 0.4, 5.0 / 3.0, r, oo)
+    _run2d("ot2pi_mc_hlld", "ot", 16, "mc", "hlld", "rk2", 2, 0.4, 4.0 / 3.0,
+           (0.0, 2.0 * np.pi), ("periodic", "periodic"))
+
+
+def gen_1d():
+    cfg = {"grid": {"xmin": 0.0, "xmax": 1.0, "ncells": 400, "ng": 2},
+           "eos": {"K": 0.0, "gamma": 2.0, "gamma_th": 2.0},
+           "run": {"problem": "generic", "t_end": 0.4, "cfl": 0.4, "atmo_rho": 1.0e-10,
+                   "solver": "hlld", "limiter": "mc", "device": "cpu",
+                   "primL": {"rho": 1.0, "p": 1.0, "vx": 0.0, "vy": 0.0, "vz": 0.0,
+                             "Bx": 0.5, "By": 1.0, "Bz": 0.0},
+                   "primR": {"rho": 0.125, "p": 0.1, "vx": 0.0, "vy": 0.0, "vz": 0.0,
+                             "Bx": 0.5, "By": -1.0, "Bz": 0.0},
+                   "x_interface": 0.5},
+           "output": {"dir": str(HERE / "_tmp_out"), "every_n_steps": 100000}}
+    out = {}
+    for solver, t_end in (("hlld", 0.4), ("hllc", 0.05), ("hlle", 0.05)):
+        cfg["run"]["solver"] = solver
+        cfg["run"]["t_end"] = t_end
+        t, grid, prims, cons = R_run(cfg)
+        out[f"{solver}_t"] = np.array(t)
+        for k in PRIM9:
+            out[f"{solver}_prim_{k}"] = _np(prims[k])
+        for k in ("D", "Sx", "Sy", "Sz", "tau"):
+            out[f"{solver}_cons_{k}"] = _np(cons[k])
+    np.savez_compressed(HERE / "balsara1.npz", **out)
+    import shutil
+    shutil.rmtree(HERE / "_tmp_out", ignore_errors=True)
+
+
+def gen_tubes():
+    eos = hybrid_eos(K=0.0, gamma=5.0 / 3.0, gamma_th=5.0 / 3.0)
+    BW_L, BW_R = [1.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0], [0.125, 0.6, 0.0, 0.0, 0.0, -1.0, 0.0]
+    QUIET = [1.0, 0.02, 0.0, 0.0, 0.0, 0.0, 0.0]
+    HOT_L, HOT_R = [1.0, 30.0, 0.5, 0.0, 0.0, 2.0, 0.0], [0.1, 0.1, -0.5, 0.0, 0.0, -2.0, 0.0]
+    UL = np.array([BW_L, QUIET, HOT_L])
+    UR = np.array([BW_R, QUIET, HOT_R])
+    Bn = np.array([0.5, 0.3, 0.5])
+    out = {"UL": UL, "UR": UR, "Bn": Bn}
+    for lim in ("mc", "mp5"):
+        prof, t = run_tubes(UL, UR, Bn, eos, ncells=48, tend=0.04, limiter=lim)
+        out[f"{lim}_t"] = np.array(t)
+        for k, v in prof.items():
+            out[f"{lim}_{k}"] = _np(v)
+    np.savez_compressed(HERE / "tubes_shared.npz", **out)
+
+
+if __name__ == "__main__":
+    torch.set_num_threads(4)
+    which = sys.argv[1:] or ["flux", "c2p", "recon", "2d", "1d", "tubes"]
+    for w in which:
+        print("generating", w, flush=True)
+        globals()[f"gen_{w}"]()

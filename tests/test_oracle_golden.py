@@ -1,0 +1,155 @@
+"""The oracle against the reference's own outputs (tests/golden/*.npz).
+
+The fixtures were written by tests/golden/make_golden.py from the reference
+solver.  Every comparison here is BITWISE (NaN == NaN): the oracle restates
+the reference with the same float64 operations in the same order, so any
+difference is a transcription error.
+"""
+
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import c2p as C, eos as E, recon as RC, riemann as R, step1d as S1, step2d as S2, tubes as T
+
+PRIM9 = ("rho", "vx", "vy", "vz", "p", "eps", "Bx", "By", "Bz")
+CONS8 = ("D", "tau", "Sx", "Sy", "Sz", "Bx", "By", "Bz")
+
+
+def _load(golden_dir, name):
+    path = os.path.join(golden_dir, name)
+    if not os.path.exists(path):
+        pytest.skip(f"{name} not generated")
+    return np.load(path)
+
+
+def _eq(a, b, what):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, what
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    assert same.all(), f"{what}: {int((~same).sum())} of {a.size} differ, max |d| = " \
+                       f"{np.nanmax(np.abs(a - b))}"
+
+
+@pytest.mark.parametrize("tag", ["a", "b"])
+@pytest.mark.parametrize("solver", ["hlle", "hllc", "hlld"])
+@pytest.mark.parametrize("idir", [0, 1, 2])
+def test_flux_bitwise(golden_dir, tag, solver, idir):
+    g = _load(golden_dir, "flux_random.npz")
+    eos = E.ideal(5.0 / 3.0)
+    sL = {k: torch.from_numpy(g[f"{tag}_L_{k}"]) for k in PRIM9}
+    sR = {k: torch.from_numpy(g[f"{tag}_R_{k}"]) for k in PRIM9}
+    F, U, ps = R.SOLVERS[solver](sL, sR, eos, idir=idir)
+    pre = f"{tag}_{solver}{idir}"
+    for k in CONS8:
+        _eq(F[k].numpy(), g[f"{pre}_F_{k}"], f"{pre} F[{k}]")
+        _eq(U[k].numpy(), g[f"{pre}_U_{k}"], f"{pre} U[{k}]")
+    _eq(ps.numpy(), g[f"{pre}_pstar"], f"{pre} p*")
+    if solver == "hlld":
+        d = R.LAST_DIAG
+        got = [d["n_not_converged"], d["n_wave_order_bad"], d["n_bn_zero"],
+               d["n_hlle_fallback"], d["max_iters"]]
+        assert got == list(g[f"{pre}_diag"])
+        _eq(d["mean_iters"], g[f"{pre}_mean_iters"], "mean_iters")
+
+
+def test_c2p_batch_bitwise(golden_dir):
+    g = _load(golden_dir, "c2p_batch.npz")
+    eos = E.ideal(5.0 / 3.0)
+    cons = {k: torch.from_numpy(g[f"in_{k}"].copy()) for k in CONS8}
+    P, S = C.cons_to_prim({k: v.clone() for k, v in cons.items()}, eos, return_status=True)
+    for k in PRIM9:
+        _eq(P[k].numpy(), g[f"out_{k}"], f"c2p {k}")
+    for k in ("bracket_ok", "converged", "v_clamped", "atmo_floored"):
+        _eq(S[k].numpy(), g[f"st_{k}"], f"status {k}")
+    sub = {k: v[-16:].clone() for k, v in cons.items()}
+    P2, S2_ = C.cons_to_prim(sub, eos, return_status=True)
+    for k in PRIM9:
+        _eq(P2[k].numpy(), g[f"hot_{k}"], f"hot {k}")
+    _eq(S2_["converged"].numpy(), g["hot_converged"], "hot converged")
+
+
+@pytest.mark.parametrize("lim", ["pcm", "mc", "minmod", "weno5z", "mp5", "mp7"])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_reconstruct_prims_bitwise(golden_dir, lim, axis):
+    g = _load(golden_dir, "recon_block.npz")
+    eos = E.ideal(5.0 / 3.0)
+    P = {k: torch.from_numpy(g[f"in_{k}"].copy()) for k in PRIM9}
+    L, R_ = RC.reconstruct_prims(P, axis, eos, lim)
+    for k in PRIM9:
+        _eq(L[k].numpy(), g[f"{lim}{axis}_L_{k}"], f"{lim} axis{axis} L[{k}]")
+        _eq(R_[k].numpy(), g[f"{lim}{axis}_R_{k}"], f"{lim} axis{axis} R[{k}]")
+
+
+def _state_from(g, prefix):
+    cons = {k: torch.from_numpy(g[f"{prefix}cons_{k}"].copy()) for k in S2.EVOLVED_KEYS}
+    prims = {k: torch.from_numpy(g[f"{prefix}prim_{k}"].copy()) for k in PRIM9}
+    return S2.State2D(cons, torch.from_numpy(g[f"{prefix}Bxf"].copy()),
+                      torch.from_numpy(g[f"{prefix}Byf"].copy()), prims)
+
+
+CASES_2D = [
+    # name, problem, n, limiter, solver, scheme, gamma, box, bc
+    ("rotor_sharp_mc_hlld", "rotor", 24, "mc", "hlld", "rk3", 5 / 3, (-0.5, 0.5), "outflow"),
+    ("rotor_tapered_weno5z_hlld", "tapered", 20, "weno5z", "hlld", "rk3", 5 / 3, (-0.5, 0.5), "outflow"),
+    ("rotor_tapered_mp5_hllc", "tapered", 16, "mp5", "hllc", "rk3", 5 / 3, (-0.5, 0.5), "outflow"),
+    ("rotor_sharp_mc_hlle", "rotor", 16, "mc", "hlle", "rk2", 5 / 3, (-0.5, 0.5), "outflow"),
+    ("ot2pi_mc_hlld", "ot", 16, "mc", "hlld", "rk2", 4 / 3, (0.0, 2 * np.pi), "periodic"),
+]
+
+
+@pytest.mark.parametrize("case", CASES_2D, ids=[c[0] for c in CASES_2D])
+def test_2d_steps_bitwise(golden_dir, case):
+    name, problem, n, lim, solver, scheme, gamma, box, bc = case
+    g = _load(golden_dir, f"{name}.npz")
+    eos = E.ideal(gamma)
+    grid = S2.make_grid(n, box, lim)
+    init = {"rotor": S2.rotor_sharp, "tapered": S2.rotor_tapered,
+            "ot": S2.orszag_tang_2pi}[problem]
+    P, Az = init(grid, eos)
+    st = S2.init_state(grid, eos, P, Az, (bc, bc))
+    ref0 = _state_from(g, "init_")
+    for k in S2.EVOLVED_KEYS:
+        _eq(st.cons[k].numpy(), ref0.cons[k].numpy(), f"init cons {k}")
+    for k in PRIM9:
+        _eq(st.prims[k].numpy(), ref0.prims[k].numpy(), f"init prim {k}")
+    for dt_ref in g["dts"]:
+        dt = S2.compute_dt_2d(st.prims, grid, eos, 0.4)
+        assert dt == dt_ref
+        st, _ = S2.rk_step_ct(st, grid, eos, dt, scheme=scheme, bc_x=bc, bc_y=bc,
+                              flux_fn=R.SOLVERS[solver], limiter=lim)
+    ref = _state_from(g, "fin_")
+    for k in S2.EVOLVED_KEYS:
+        _eq(st.cons[k].numpy(), ref.cons[k].numpy(), f"cons {k}")
+    _eq(st.Bxf.numpy(), ref.Bxf.numpy(), "Bxf")
+    _eq(st.Byf.numpy(), ref.Byf.numpy(), "Byf")
+    for k in PRIM9:
+        _eq(st.prims[k].numpy(), ref.prims[k].numpy(), f"prim {k}")
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("solver", ["hlld", "hllc", "hlle"])
+def test_balsara1_bitwise(golden_dir, solver):
+    g = _load(golden_dir, "balsara1.npz")
+    cfg = S1.balsara1_cfg(solver=solver)
+    cfg["run"]["t_end"] = float(g[f"{solver}_t"]) if solver != "hlld" else 0.4
+    if solver != "hlld":
+        cfg["run"]["t_end"] = 0.05
+    t, grid, P, cons, _ = S1.run(cfg)
+    assert t == float(g[f"{solver}_t"])
+    for k in PRIM9:
+        _eq(P[k].numpy(), g[f"{solver}_prim_{k}"], f"{solver} prim {k}")
+
+
+@pytest.mark.parametrize("lim", ["mc", "mp5"])
+def test_tubes_shared_bitwise(golden_dir, lim):
+    g = _load(golden_dir, "tubes_shared.npz")
+    eos = E.ideal(5.0 / 3.0)
+    prof, t = T.run_tubes_shared(g["UL"], g["UR"], g["Bn"], eos, ncells=48, tend=0.04,
+                                 limiter=lim)
+    assert t == float(g[f"{lim}_t"])
+    for k, v in prof.items():
+        _eq(v.numpy(), g[f"{lim}_{k}"], f"tubes {lim} {k}")
