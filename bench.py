@@ -1,0 +1,168 @@
+"""Throughput of the SRMHD step on one MI355X: zone-cycles/s.
+
+Workload (the headline configuration): the tapered relativistic magnetic
+rotor on [-0.5, 0.5]^2 at 2048^2, Gamma = 5/3, WENO5-Z + HLLD + upwind
+constrained transport + Kastaun con2prim + SSP-RK3, CFL 0.4, outflow BCs,
+float64 throughout.  A step is one full RK3 step including the CFL
+reduction; one zone-cycle is one physical cell advanced one step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints ONE JSON line.  ``roofline`` is for the dominant kernel (measured
+live with HIP events around its launches inside the timed region);
+``cpu_baseline`` times the CPU oracle (torch, float64) on a bounded sample of
+the same problem.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "hlld-ai_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(kernel, n, ng):
+    """Compulsory HBM bytes of one launch of ``kernel`` on an n^2 grid with
+    ng ghosts: every input array element read once, every output written
+    once (float64)."""
+    nt = n + 2 * ng
+    cells = nt * nt
+    faces = (nt + 1) * nt
+    if kernel in ("sweep_x_b", "sweep_y_b"):
+        # prims rho, p, vx, vy, vz + cons Bz + both face-B arrays; the
+        # pass-A record (3 doubles/face); 7 flux arrays out
+        return 8 * (6 * cells + 2 * faces) + 24 * faces + 56 * faces
+    if kernel in ("sweep_x_a", "sweep_y_a"):
+        return 8 * (6 * cells + 2 * faces) + 24 * faces
+    if kernel == "c2p_a":
+        # cons D, Sx, Sy, Sz, tau, Bz + both face arrays in; record out
+        return 8 * (6 * cells + 2 * faces) + 24 * cells
+    if kernel == "c2p_b":
+        phys = n * n
+        return phys * (8 * 8 + 24 + 6 * 8)
+    raise ValueError(kernel)
+
+
+def cpu_baseline(n_sample=256, limiter="weno5z"):
+    """The oracle (torch CPU float64) on the same problem at n_sample^2: one
+    RK3 step after the initial sync, timed."""
+    import torch
+    import oracle.eos as OE
+    import oracle.step2d as OS
+    threads = torch.get_num_threads()
+    eos = OE.ideal(5.0 / 3.0)
+    g = OS.make_grid(n_sample, (-0.5, 0.5), limiter)
+    P, Az = OS.rotor_tapered(g, eos)
+    st = OS.init_state(g, eos, P, Az)
+    dt = OS.compute_dt_2d(st.prims, g, eos, 0.4)
+    t0 = time.perf_counter()
+    OS.rk_step_ct(st, g, eos, dt, limiter=limiter)
+    wall = time.perf_counter() - t0
+    return {"value": n_sample * n_sample / wall, "unit": "zone-cycles/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"CPU oracle (torch {torch.__version__} float64, {threads} threads): one RK3 "
+                       f"step of the same tapered rotor, WENO5-Z/HLLD/CT, at {n_sample}^2 "
+                       f"({wall:.1f} s)")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--limiter", default="weno5z")
+    ap.add_argument("--solver", default="hlld")
+    ap.add_argument("--cfl", type=float, default=0.4)
+    ap.add_argument("--kernel", default="c2p_a",
+                    help="kernel class timed live for the roofline")
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the domain-decomposed design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes
+    import torch
+    from srmhd import _lib
+    from srmhd import driver2d as D2, eos as E, grid as G, initial_data2d as I
+    from srmhd.reconstruction import ghosts_needed
+
+    dev = torch.device("cuda:0")
+    eos = E.hybrid_eos(0.0, 5.0 / 3.0, 5.0 / 3.0)
+    ng = ghosts_needed(a.limiter)
+    grid = G.Grid2D(-0.5, 0.5, a.n, -0.5, 0.5, a.n, ng=ng, device=dev)
+    P, Az = I.rotor_tapered(grid, eos)
+    st = I.init_state(grid, eos, P, Az, limiter=a.limiter)
+    step = D2.Stepper2D(grid, eos, solver=a.solver, limiter=a.limiter)
+    step.load(st.state_list(), st.prim_list())
+    del st, P, Az
+
+    def one_step():
+        step.compute_dt(step.U, step.P, a.cfl)
+        step.step("rk3")
+
+    for _ in range(a.warmup):
+        one_step()
+    torch.cuda.synchronize()
+    print(f"warmup done ({a.warmup} steps)", file=sys.stderr, flush=True)
+
+    lib = step.lib
+    kcls = _lib.KERNEL_CLASSES[a.kernel]
+    _lib.check(lib.srmhd_timing_enable(kcls, 3 * a.steps), "srmhd_timing_enable")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        one_step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    _lib.check(lib.srmhd_timing_read(ctypes.byref(ms), ctypes.byref(cnt)), "srmhd_timing_read")
+    lib.srmhd_timing_enable(-1, 0)
+
+    zc = a.n * a.n * a.steps / wall
+    avg_ms = ms.value / max(cnt.value, 1)
+    kb = algorithmic_bytes(a.kernel, a.n, ng)
+    achieved = kb / (avg_ms * 1e-3) / 1e9
+    bytes_per_zc = 752.0  # RK3 2D algorithmic bytes per zone-cycle
+    rho = step.P[0][ng:ng + a.n, ng:ng + a.n]
+    finite = bool(torch.isfinite(rho).all().item())
+
+    out = {
+        "metric": "zone-cycles/s, 2D rotor 2048^2 (FP64 vs FP32-flux @ L1<=1e-6); HBM GB/s",
+        "value": zc, "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"tapered relativistic rotor {a.n}^2, {a.limiter.upper()} + "
+                               f"{a.solver.upper()} + CT + Kastaun C2P + SSP-RK3, CFL {a.cfl}, "
+                               "float64 (plain form)",
+                   "n": a.n, "ng": ng, "limiter": a.limiter, "solver": a.solver, "scheme": "rk3"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": a.kernel, "avg_launch_ms": avg_ms, "launches": cnt.value,
+                     "algorithmic_bytes_per_launch": kb},
+        "step_hbm": {"bytes_per_zone_cycle": bytes_per_zc,
+                     "achieved_GBs": zc * bytes_per_zc / 1e9,
+                     "frac": zc * bytes_per_zc / 1e9 / HBM_PEAK_GBS},
+        "state_finite": finite,
+    }
+    if not a.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.limiter)
+        except Exception as exc:  # the baseline never hides the GPU result
+            out["cpu_baseline"] = {"error": repr(exc)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

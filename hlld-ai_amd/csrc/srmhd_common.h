@@ -13,7 +13,7 @@
 
 #include "../../include/srmhd.h"
 
-#define SRMHD_DEV __device__ __forceinline__
+#define SRMHD_DEV __host__ __device__ __forceinline__
 
 namespace srmhd {
 
@@ -38,9 +38,9 @@ SRMHD_DEV bool finite_pos_or_zero(double x) { return x >= 0.0 && x <= DBL_MAX; }
 SRMHD_DEV double csqrt(double x) {
   double y = __builtin_sqrt(x);
   if (!(x >= 0x1p-900 && x <= DBL_MAX)) return y;
-  const long long b = __double_as_longlong(y);
-  const double up = __longlong_as_double(b + 1);
-  const double dn = __longlong_as_double(b - 1);
+  const long long b = __builtin_bit_cast(long long, y);
+  const double up = __builtin_bit_cast(double, b + 1);
+  const double dn = __builtin_bit_cast(double, b - 1);
   if (__builtin_fma(y, up, -x) < 0.0) {
     y = up;
   } else if (__builtin_fma(y, dn, -x) >= 0.0) {

@@ -660,6 +660,25 @@ __global__ void k_copy_diag(const srmhd_flux_diag* src, srmhd_flux_diag* dst, in
 
 // ── host side ───────────────────────────────────────────────────────────────
 
+// Optional live timing: HIP event pairs around every launch of one kernel
+// class, on the launch stream (srmhd_timing_enable / srmhd_timing_read).
+struct Timing {
+  int cls = -1;
+  int cap = 0, used = 0;
+  hipEvent_t* ev = nullptr;   // 2*cap events
+} g_timing;
+
+inline void tbegin(int cls, hipStream_t s) {
+  if (cls == g_timing.cls && g_timing.used < g_timing.cap)
+    hipEventRecord(g_timing.ev[2 * g_timing.used], s);
+}
+inline void tend(int cls, hipStream_t s) {
+  if (cls == g_timing.cls && g_timing.used < g_timing.cap) {
+    hipEventRecord(g_timing.ev[2 * g_timing.used + 1], s);
+    ++g_timing.used;
+  }
+}
+
 inline int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
 
 struct Carver {
@@ -753,10 +772,22 @@ void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const W
             hipStream_t s) {
   const int bx = blocks_for((int64_t)(g.nxt + 1) * g.nyt);
   const int by = blocks_for((int64_t)g.nxt * (g.nyt + 1));
-  if (SOL == SOLVER_HLLD) k_sweep_a<0, LIM><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0);
+  if (SOL == SOLVER_HLLD) {
+    tbegin(SRMHD_K_SWEEP_X_A, s);
+    k_sweep_a<0, LIM><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0);
+    tend(SRMHD_K_SWEEP_X_A, s);
+  }
+  tbegin(SRMHD_K_SWEEP_X_B, s);
   k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, w.Fx);
-  if (SOL == SOLVER_HLLD) k_sweep_a<1, LIM><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1);
+  tend(SRMHD_K_SWEEP_X_B, s);
+  if (SOL == SOLVER_HLLD) {
+    tbegin(SRMHD_K_SWEEP_Y_A, s);
+    k_sweep_a<1, LIM><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1);
+    tend(SRMHD_K_SWEEP_Y_A, s);
+  }
+  tbegin(SRMHD_K_SWEEP_Y_B, s);
   k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, w.Fy);
+  tend(SRMHD_K_SWEEP_Y_B, s);
 }
 
 template <int LIM>
@@ -785,9 +816,13 @@ void sweeps_dispatch(int lim, int sol, const G2& g, const St2& S, const Pr2& P,
 void c2p_cells(const G2& g, const St2& S, const srmhd_eos& e, double atmo_rho, double atmo_eps,
                const Ws2& w, const Pr2Out& P, hipStream_t s) {
   const int64_t ncell = (int64_t)g.nxt * g.nyt;
+  tbegin(SRMHD_K_C2P_A, s);
   k_c2p2d_a<<<blocks_for(ncell), BLK, 0, s>>>(g, S, e, w.recc, w.words + 0);
+  tend(SRMHD_K_C2P_A, s);
+  tbegin(SRMHD_K_C2P_B, s);
   k_c2p2d_b<<<blocks_for((int64_t)g.nx * g.ny), BLK, 0, s>>>(g, S, e, w.recc, w.words + 0,
                                                              atmo_rho, atmo_eps, P);
+  tend(SRMHD_K_C2P_B, s);
   k_prim_bc<<<blocks_for(ncell), BLK, 0, s>>>(g, P);
 }
 
@@ -927,9 +962,13 @@ int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_
   k_zero_u64<<<1, 64, 0, st>>>(w.words, 4);
   sweeps_dispatch(sp->limiter, sp->solver, g, s1, pr, *eos, w, st);
   const int64_t ncor = (int64_t)(g.nxt + 1) * (g.nyt + 1);
+  tbegin(SRMHD_K_CORNER, st);
   k_corner<<<blocks_for(ncor), BLK, 0, st>>>(g, s1, pr, w.Fx.p[0], w.Fx.p[6], w.Fy.p[0],
                                              w.Fy.p[6], w.diag, sp->upwind, w.Ez);
+  tend(SRMHD_K_CORNER, st);
+  tbegin(SRMHD_K_UPDATE, st);
   k_update<<<blocks_for(ncor), BLK, 0, st>>>(g, s0, s1, w.Fx, w.Fy, w.Ez, a, b, c, dt_dev, o);
+  tend(SRMHD_K_UPDATE, st);
   c2p_cells(g, so, *eos, sp->atmo_rho, sp->atmo_eps, w, po, st);
   if (diag_xy) k_copy_diag<<<1, 64, 0, st>>>(w.diag, diag_xy, 2);
   return launch_status();
@@ -984,6 +1023,43 @@ int srmhd2d_dt(const srmhd_grid2d* gp, const srmhd_eos* eos, double cfl, const d
   k_dt_reduce<<<blocks_for((int64_t)g.nx * g.ny), BLK, 0, st>>>(g, s, pr, *eos, w.words + 1);
   k_dt_final<<<1, 64, 0, st>>>(w.words + 1, cfl, g.dx, g.dy, dt_dev);
   return launch_status();
+}
+
+int srmhd_timing_enable(int kernel_class, int max_launches) {
+  if (max_launches < 0 || max_launches > (1 << 20)) return SRMHD_EARG;
+  if (g_timing.ev) {
+    for (int i = 0; i < 2 * g_timing.cap; ++i) hipEventDestroy(g_timing.ev[i]);
+    delete[] g_timing.ev;
+    g_timing.ev = nullptr;
+  }
+  g_timing.cls = -1;
+  g_timing.cap = g_timing.used = 0;
+  if (kernel_class < 0 || max_launches == 0) return SRMHD_OK;
+  g_timing.ev = new hipEvent_t[2 * max_launches];
+  for (int i = 0; i < 2 * max_launches; ++i) {
+    const hipError_t e = hipEventCreate(&g_timing.ev[i]);
+    if (e != hipSuccess) return hip_status(e);
+  }
+  g_timing.cap = max_launches;
+  g_timing.cls = kernel_class;
+  return SRMHD_OK;
+}
+
+int srmhd_timing_read(double* total_ms, int64_t* launches) {
+  if (!total_ms || !launches) return SRMHD_EARG;
+  double sum = 0.0;
+  for (int i = 0; i < g_timing.used; ++i) {
+    hipError_t e = hipEventSynchronize(g_timing.ev[2 * i + 1]);
+    if (e != hipSuccess) return hip_status(e);
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, g_timing.ev[2 * i], g_timing.ev[2 * i + 1]);
+    if (e != hipSuccess) return hip_status(e);
+    sum += ms;
+  }
+  *total_ms = sum;
+  *launches = g_timing.used;
+  g_timing.used = 0;
+  return SRMHD_OK;
 }
 
 }  // extern "C"

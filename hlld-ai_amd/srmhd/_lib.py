@@ -67,7 +67,12 @@ _SIGS = {
                                     ctypes.POINTER(Eos), _PP, _PP, _P, _I64, _P]),
     "srmhd2d_dt": (ctypes.c_int, [ctypes.POINTER(Grid2D), ctypes.POINTER(Eos), _D, _PP, _PP, _P,
                                   _P, _I64, _P]),
+    "srmhd_timing_enable": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "srmhd_timing_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_int64)]),
 }
+KERNEL_CLASSES = {"sweep_x_a": 0, "sweep_x_b": 1, "sweep_y_a": 2, "sweep_y_b": 3, "corner": 4,
+                  "update": 5, "c2p_a": 6, "c2p_b": 7}
 EXPORTED = tuple(_SIGS)
 
 

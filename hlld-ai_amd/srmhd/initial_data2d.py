@@ -6,6 +6,7 @@ field is its discrete curl, so div B = 0 at t = 0 (initial_data2d.py:24-29).
 
 import math
 
+import numpy as np
 import torch
 
 from .driver2d import prims_to_cons_2d, sync_state
@@ -41,7 +42,7 @@ def rotor_tapered(grid, eos, r0=0.1, r1=0.115, rho_in=10.0, rho_out=1.0, omega=9
     f = (r1 - r)/(r1 - r0) of (rho - 1) and omega over r0 < r < r1, rho = 1
     at rest outside; p = 1 everywhere, B = (B0, 0, 0) from Az = B0 y."""
     X, Y = grid.X, grid.Y
-    r = torch.sqrt(X * X + Y * Y)
+    r = torch.from_numpy(np.sqrt((X * X + Y * Y).numpy()))   # correctly rounded
     f = torch.where(r <= r0, torch.ones_like(r),
                     torch.where(r < r1, (r1 - r) / (r1 - r0), torch.zeros_like(r)))
     om = omega * f

@@ -130,6 +130,19 @@ int srmhd2d_dt(const srmhd_grid2d* g, const srmhd_eos* eos, double cfl,
                const double* const* U, const double* const* P, double* dt_dev, void* work,
                int64_t work_bytes, void* stream);
 
+/* ── live kernel timing (measurement only; no counterpart in the reference) ──
+ * srmhd_timing_enable(cls, n): from now on srmhd2d_stage records a HIP event
+ * pair around each launch of kernel class cls, on its stream, for up to n
+ * launches (cls < 0 or n = 0 disables).  srmhd_timing_read() waits for the
+ * recorded events and returns their summed duration and count, then resets.
+ * Not thread-safe: one timing session per process. */
+enum {
+  SRMHD_K_SWEEP_X_A = 0, SRMHD_K_SWEEP_X_B = 1, SRMHD_K_SWEEP_Y_A = 2, SRMHD_K_SWEEP_Y_B = 3,
+  SRMHD_K_CORNER = 4, SRMHD_K_UPDATE = 5, SRMHD_K_C2P_A = 6, SRMHD_K_C2P_B = 7
+};
+int srmhd_timing_enable(int kernel_class, int max_launches);
+int srmhd_timing_read(double* total_ms, int64_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,7 @@ runs each cell as a batch of one -- used only to bound the coupling effect.
 
 import torch
 
-from .physics import TINY, sdiv
+from .physics import TINY, sdiv, sqrt
 
 
 class Kastaun:
@@ -21,7 +21,7 @@ class Kastaun:
         tau = torch.clamp(cons["tau"], min=0.0)
         Bx, By, Bz = cons["Bx"], cons["By"], cons["Bz"]
         S2 = Sx ** 2 + Sy ** 2 + Sz ** 2
-        Sn = torch.sqrt(S2)
+        Sn = sqrt(S2)
         Smax = D + tau
         fac = torch.where(Sn > Smax, 0.9999 * Smax / (Sn + TINY), torch.ones_like(Sn))
         Sx, Sy, Sz = Sx * fac, Sy * fac, Sz * fac
@@ -33,7 +33,7 @@ class Kastaun:
         self.q = tau / (D + TINY)
         self.r = tuple(s / (D + TINY) for s in (Sx, Sy, Sz))
         self.r2 = S2 / (D ** 2 + TINY)
-        sD = torch.sqrt(D + TINY)
+        sD = sqrt(D + TINY)
         self.bt = tuple(b / sD for b in (Bx, By, Bz))
         self.bt2 = B2 / (D + TINY)
         self.br = BS / (D * sD + TINY)
@@ -53,14 +53,14 @@ class Kastaun:
 
     def fa(self, mu):
         """mu sqrt(h_min^2 + rbar^2) - 1   c2p.py:124-127."""
-        return mu * torch.sqrt(1.0 + self.rbar2(mu)) - 1.0
+        return mu * sqrt(1.0 + self.rbar2(mu)) - 1.0
 
     def master(self, mu):
         """c2p.py:129-158 -> (f, rho, eps, W, v^2)."""
         qb = self.qbar(mu)
         rb2 = self.rbar2(mu)
         vh2 = torch.clamp(mu ** 2 * rb2, max=self.v02)
-        Wh = 1.0 / torch.sqrt(torch.clamp(1.0 - vh2, min=1e-10))
+        Wh = 1.0 / sqrt(torch.clamp(1.0 - vh2, min=1e-10))
         rho = torch.clamp(self.D / Wh, min=torch.full_like(Wh, 1e-12),
                           max=torch.full_like(Wh, 1e15))
         eps0 = Wh * (qb - mu * rb2) + vh2 * Wh ** 2 / (1.0 + Wh)
@@ -133,7 +133,7 @@ class Kastaun:
         v = [mc * (self.r[i] + (mu * self.br) * self.bt[i]) for i in range(3)]
         v2 = torch.stack(v, dim=1).pow(2).sum(dim=1)
         lim = 1.0 - 1e-10
-        norm = torch.where(v2 > lim, torch.sqrt(torch.clamp(v2, max=lim) / (v2 + TINY)),
+        norm = torch.where(v2 > lim, sqrt(torch.clamp(v2, max=lim) / (v2 + TINY)),
                            torch.ones_like(v2))
         v = [c * norm for c in v]
         p = self.eos.press__eps_rho(eps, rho)

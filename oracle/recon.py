@@ -8,6 +8,8 @@ own edge value.
 
 import torch
 
+from .physics import sqrt
+
 STENCIL = {"pcm": 0, "mc": 1, "minmod": 1, "weno5z": 2, "mp5": 2, "mp7": 3}
 HIGH_ORDER = ("weno5z", "mp5", "mp7")
 RECON_KEYS = ("rho", "p", "vx", "vy", "vz", "Bx", "By", "Bz")   # reconstruction.py:269
@@ -162,7 +164,7 @@ def reconstruct_prims(prims, axis, eos, limiter="mc", floor_rho=1e-12, floor_p=1
     EOS-consistent eps."""
     work = {k: prims[k] for k in RECON_KEYS if k in prims}
     v2 = prims["vx"] ** 2 + prims["vy"] ** 2 + prims["vz"] ** 2
-    W = 1.0 / torch.sqrt(torch.clamp(1.0 - v2, min=1e-14))
+    W = 1.0 / sqrt(torch.clamp(1.0 - v2, min=1e-14))
     for c in ("vx", "vy", "vz"):
         work[c] = W * prims[c]
     L, R = {}, {}
@@ -178,7 +180,7 @@ def reconstruct_prims(prims, axis, eos, limiter="mc", floor_rho=1e-12, floor_p=1
                 R[k] = torch.where(bad, cr, R[k])
     for side in (L, R):
         z2 = side["vx"] ** 2 + side["vy"] ** 2 + side["vz"] ** 2
-        fac = 1.0 / torch.sqrt(1.0 + z2)
+        fac = 1.0 / sqrt(1.0 + z2)
         for c in ("vx", "vy", "vz"):
             side[c] = side[c] * fac
         side["rho"] = torch.clamp(side["rho"], min=floor_rho)

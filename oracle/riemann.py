@@ -10,7 +10,7 @@ import torch
 
 from .c2p import cons_to_prim
 from .physics import (BK, CONS_KEYS, SK, VEL, TINY, b_squared, energy_form,
-                      hll_flux, hll_state, lorentz, sdiv, srmhd_fluxes)
+                      hll_flux, hll_state, lorentz, sdiv, sqrt, srmhd_fluxes)
 
 LAST_DIAG: dict = {}
 PSTAR_TOL = 1.0e-10       # hlld.py:1080
@@ -61,7 +61,7 @@ def hllc_flux(sL, sR, eos, idir=0):
     qa = fE - cross
     qb = -fH[SK[n]] - E + btan2 + fbtan2
     qc = uH[SK[n]] - cross
-    disc = torch.sqrt(torch.clamp(qb ** 2 - 4.0 * qa * qc, min=0.0))
+    disc = sqrt(torch.clamp(qb ** 2 - 4.0 * qa * qc, min=0.0))
     vq = sdiv(-0.5 * (qb + disc), qa)
     vlin = sdiv(-qc, qb)
     vc = torch.where(qa.abs() < 1e-15,
@@ -69,7 +69,7 @@ def hllc_flux(sL, sR, eos, idir=0):
     vt1 = torch.where(hasB, sdiv(uH[BK[t1]] * vc - fH[BK[t1]], Bn), torch.zeros_like(vc))
     vt2 = torch.where(hasB, sdiv(uH[BK[t2]] * vc - fH[BK[t2]], Bn), torch.zeros_like(vc))
     v2 = vc ** 2 + vt1 ** 2 + vt2 ** 2
-    g = 1.0 / torch.sqrt(1.0 - torch.clamp(v2, max=1.0 - 1e-10))
+    g = 1.0 / sqrt(1.0 - torch.clamp(v2, max=1.0 - 1e-10))
     vB = torch.where(hasB, vc * uH[BK[n]] + vt1 * uH[BK[t1]] + vt2 * uH[BK[t2]],
                      torch.zeros_like(vc))
     pB = fH[SK[n]] + (Bn / g) ** 2 - vc * (fH["tau"] + fH["D"] - Bn * vB)
@@ -189,8 +189,8 @@ class StarStates:
         S2L = (TaL + DaL + p) * v2L - vBL * B2L
         S2R = (TaR + DaR + p) * v2R - vBR * B2R
 
-        sqL = torch.sqrt(torch.clamp(waL, min=0.0))
-        sqR = torch.sqrt(torch.clamp(waR, min=0.0))
+        sqL = sqrt(torch.clamp(waL, min=0.0))
+        sqR = sqrt(torch.clamp(waR, min=0.0))
         SgL, SgR = self.SgL, self.SgR
         dL = (-cm) * p + RTL - BnL * SgL * sqL
         dR = cx * p + RTR + BnR * SgR * sqR
@@ -284,7 +284,7 @@ def hlld_flux(sL, sR, eos, idir=0):
     big = a.abs() > 1e-14
     sa = torch.where(big, a, torch.ones_like(a))
     sb = torch.where(big, b, torch.ones_like(b))
-    vs = torch.where(big, (-b - torch.sqrt(disc)) / (2.0 * sa), -c / sb)
+    vs = torch.where(big, (-b - sqrt(disc)) / (2.0 * sa), -c / sb)
     vs = torch.clamp(vs, min=-(1.0 - 1e-10), max=(1.0 - 1e-10))
     p0 = torch.clamp(fH[SK[n]] - (fH["tau"] + fH["D"]) * vs, min=1e-30)
 

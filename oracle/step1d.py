@@ -6,7 +6,7 @@ import torch
 from .c2p import cons_to_prim
 from .eos import HybridEOS
 from .grid import Grid1D
-from .physics import prim_to_cons
+from .physics import prim_to_cons, sqrt
 from .recon import interface_states_1d
 from .riemann import SOLVERS, hlld_flux
 
@@ -21,13 +21,13 @@ def compute_dt(prims, eos, dx, cfl, ng, ncells, speed_of_light=False):
     Bx, By, Bz = prims["Bx"][ph], prims["By"][ph], prims["Bz"][ph]
     p, cs2 = eos.press_and_cs2(eps, rho)
     v2 = vx ** 2 + vy ** 2 + vz ** 2
-    W = 1.0 / torch.sqrt(torch.clamp(1.0 - v2, min=1e-10))
+    W = 1.0 / sqrt(torch.clamp(1.0 - v2, min=1e-10))
     B2 = Bx ** 2 + By ** 2 + Bz ** 2
     vB = vx * Bx + vy * By + vz * Bz
     b2 = (B2 + vB ** 2) / W ** 2
     h = 1.0 + eps + p / rho
     v02 = cs2 + (b2 / (rho * h + b2)) * (1.0 - cs2)
-    vf = torch.sqrt(torch.clamp(v02, min=0.0, max=1.0))
+    vf = sqrt(torch.clamp(v02, min=0.0, max=1.0))
     smax = torch.clamp((vx.abs() + vf).max(), min=1e-14)
     if speed_of_light:
         smax = torch.ones_like(smax)

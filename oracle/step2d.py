@@ -8,7 +8,7 @@ import torch
 
 from .c2p import cons_to_prim
 from .grid import Grid2D
-from .physics import prim_to_cons
+from .physics import prim_to_cons, sqrt
 from .recon import reconstruct_prims
 from .riemann import LAST_DIAG, hlld_flux
 
@@ -133,13 +133,13 @@ def compute_dt_2d(prims, grid, eos, cfl, speed_of_light=False):
     Bx, By, Bz = prims["Bx"][ph], prims["By"][ph], prims["Bz"][ph]
     p, cs2 = eos.press_and_cs2(eps, rho)
     v2 = vx ** 2 + vy ** 2 + vz ** 2
-    W = 1.0 / torch.sqrt(torch.clamp(1.0 - v2, min=1e-10))
+    W = 1.0 / sqrt(torch.clamp(1.0 - v2, min=1e-10))
     B2 = Bx ** 2 + By ** 2 + Bz ** 2
     vB = vx * Bx + vy * By + vz * Bz
     b2 = (B2 + vB ** 2) / W ** 2
     h = 1.0 + eps + p / rho
     v02 = cs2 + (b2 / (rho * h + b2)) * (1.0 - cs2)
-    cf = torch.sqrt(torch.clamp(v02, min=0.0, max=1.0))
+    cf = sqrt(torch.clamp(v02, min=0.0, max=1.0))
     if speed_of_light:
         sx = sy = torch.ones((), dtype=rho.dtype)
     else:
@@ -241,7 +241,7 @@ def rotor_tapered(grid, eos, r0=0.1, r1=0.115, rho_in=10.0, rho_out=1.0, omega=9
     linear taper f = (r1 - r)/(r1 - r0) of (rho - 1) and omega on
     r0 < r < r1, rho = 1 at rest outside; p = 1, B = (B0, 0, 0) from Az = B0 y."""
     X, Y = grid.X, grid.Y
-    r = torch.sqrt(X * X + Y * Y)
+    r = sqrt(X * X + Y * Y)
     f = torch.where(r <= r0, torch.ones_like(r),
                     torch.where(r < r1, (r1 - r) / (r1 - r0), torch.zeros_like(r)))
     rho = rho_out + (rho_in - rho_out) * f

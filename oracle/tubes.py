@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from .c2p import cons_to_prim
-from .physics import prim_to_cons
+from .physics import prim_to_cons, sqrt
 from .recon import ghosts_needed, reconstruct_prims
 from .riemann import SOLVERS
 from .step1d import compute_dt
@@ -93,7 +93,7 @@ def run_tubes_shared(UL, UR, Bn, eos, ncells=192, tend=0.18, cfl=0.3, limiter="m
     Bx = P["Bx"]
     t, step = 0.0, 0
     while t < tend - 1e-14 and step < max_steps:
-        cs = torch.sqrt(torch.clamp(
+        cs = sqrt(torch.clamp(
             eos.gamma * P["p"] / torch.clamp(
                 P["rho"] + eos.gamma * P["p"] / (eos.gamma - 1.0), min=1e-30), min=0.0))
         smax = float((P["vx"].abs() + torch.clamp(cs + 0.5, max=1.0)).max())
@@ -167,7 +167,7 @@ def draw_tubes(n_tubes, seed=0, lmin=128, lmax=2048):
         ct = u(-1.0, 1.0)
         ph = u(0.0, 2.0 * math.pi)
         sp = u(0.0, 0.9)
-        st = torch.sqrt(1.0 - ct * ct)
+        st = sqrt(1.0 - ct * ct)
         sides.append({"rho": rho, "p": p, "vx": sp * st * torch.cos(ph),
                       "vy": sp * st * torch.sin(ph), "vz": sp * ct,
                       "By": u(-2.0, 2.0), "Bz": u(-2.0, 2.0)})

@@ -273,9 +273,17 @@ def gen_tubes():
     np.savez_compressed(HERE / "tubes_shared.npz", **out)
 
 
+def gen_host():
+    """torch.sqrt of fixed inputs on the host that wrote the fixtures: torch's
+    CPU float64 sqrt is not correctly rounded on every host, so bitwise
+    comparisons against these fixtures only hold where it matches."""
+    x = np.random.default_rng(5).uniform(0.0, 4.0, 8192)
+    np.savez_compressed(HERE / "host_sqrt.npz", x=x, torch_sqrt=_np(torch.sqrt(torch.from_numpy(x))))
+
+
 if __name__ == "__main__":
     torch.set_num_threads(4)
-    which = sys.argv[1:] or ["flux", "c2p", "recon", "2d", "1d", "tubes"]
+    which = sys.argv[1:] or ["flux", "c2p", "recon", "2d", "1d", "tubes", "host"]
     for w in which:
         print("generating", w, flush=True)
         globals()[f"gen_{w}"]()

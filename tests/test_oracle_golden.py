@@ -12,7 +12,25 @@ import numpy as np
 import pytest
 import torch
 
-from oracle import c2p as C, eos as E, recon as RC, riemann as R, step1d as S1, step2d as S2, tubes as T
+from oracle import c2p as C, eos as E, physics as PH, recon as RC, riemann as R, step1d as S1, \
+    step2d as S2, tubes as T
+
+
+@pytest.fixture(autouse=True)
+def _reference_sqrt(golden_dir):
+    """Bitwise pinning uses the reference's own torch.sqrt, which is only
+    meaningful on a host whose torch.sqrt rounds like the one that wrote the
+    fixtures (torch's CPU float64 sqrt is not correctly rounded everywhere)."""
+    path = os.path.join(golden_dir, "host_sqrt.npz")
+    if not os.path.exists(path):
+        pytest.skip("host_sqrt.npz not generated")
+    h = np.load(path)
+    here = torch.sqrt(torch.from_numpy(h["x"])).numpy()
+    if not (here == h["torch_sqrt"]).all():
+        pytest.skip("this host's torch.sqrt rounds differently from the fixtures' host")
+    PH.set_sqrt_mode("torch")
+    yield
+    PH.set_sqrt_mode("ieee")
 
 PRIM9 = ("rho", "vx", "vy", "vz", "p", "eps", "Bx", "By", "Bz")
 CONS8 = ("D", "tau", "Sx", "Sy", "Sz", "Bx", "By", "Bz")
