@@ -26,6 +26,13 @@ for _p in (ROOT, os.path.join(ROOT, "hlld-ai_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+# FP64 VALU work per element of the dominant kernels, from rocprofv3 --pmc
+# SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 on the 1024^2 rotor (profiles/
+# sweep_x_a_rotor_counters.md): flops = 64 lanes x (add + mul + 2 fma + trans)
+# per wave, divided by the elements of the launch.
+FP64_FLOPS_PER_ELEMENT = {"sweep_x_a": 18878.0, "sweep_y_a": 18878.0, "c2p_a": 15487.0}
 
 
 def algorithmic_bytes(kernel, n, ng):
@@ -81,10 +88,13 @@ def main():
     ap.add_argument("--limiter", default="weno5z")
     ap.add_argument("--solver", default="hlld")
     ap.add_argument("--cfl", type=float, default=0.4)
-    ap.add_argument("--kernel", default="c2p_a",
+    ap.add_argument("--kernel", default="sweep_x_a",
                     help="kernel class timed live for the roofline")
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed region, copy a 512 MiB float64 buffer three times "
+                         "(the plain copy kernel that scales the FETCH/WRITE counters)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -130,9 +140,19 @@ def main():
     _lib.check(lib.srmhd_timing_read(ctypes.byref(ms), ctypes.byref(cnt)), "srmhd_timing_read")
     lib.srmhd_timing_enable(-1, 0)
 
+    if a.copy_calibration:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     zc = a.n * a.n * a.steps / wall
     avg_ms = ms.value / max(cnt.value, 1)
     kb = algorithmic_bytes(a.kernel, a.n, ng)
+    nt = a.n + 2 * ng
+    kelems = nt * nt if a.kernel.startswith("c2p") else (nt + 1) * nt
     achieved = kb / (avg_ms * 1e-3) / 1e9
     bytes_per_zc = 752.0  # RK3 2D algorithmic bytes per zone-cycle
     rho = step.P[0][ng:ng + a.n, ng:ng + a.n]
@@ -151,6 +171,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": a.kernel, "avg_launch_ms": avg_ms, "launches": cnt.value,
                      "algorithmic_bytes_per_launch": kb},
+        "fp64_valu": ({"kernel": a.kernel,
+                       "achieved": FP64_FLOPS_PER_ELEMENT[a.kernel] * kelems / (avg_ms * 1e-3) / 1e12,
+                       "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": FP64_FLOPS_PER_ELEMENT[a.kernel] * kelems / (avg_ms * 1e-3) / 1e12
+                       / FP64_PEAK_TFLOPS}
+                      if a.kernel in FP64_FLOPS_PER_ELEMENT else None),
         "step_hbm": {"bytes_per_zone_cycle": bytes_per_zc,
                      "achieved_GBs": zc * bytes_per_zc / 1e9,
                      "frac": zc * bytes_per_zc / 1e9 / HBM_PEAK_GBS},

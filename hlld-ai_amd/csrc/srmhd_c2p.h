@@ -203,7 +203,7 @@ SRMHD_DEV int c2p_break_iter(uint64_t ormask) {
 }
 
 // status bits written by the con2prim entry points
-enum { C2P_BRACKET_OK = 1, C2P_CONVERGED = 2, C2P_V_CLAMPED = 4, C2P_ATMO = 8 };
+enum { C2P_BRACKET_OK = 1, C2P_CONVERGED = 2, C2P_V_CLAMPED = 4, C2P_ATMO = 8, C2P_REDONE = 16 };
 
 // Pass B: full conservative_to_primitive of one element given its pass-A
 // record and the batch break iteration K.   c2p.py:313-363
@@ -222,7 +222,8 @@ SRMHD_DEV int c2p_pass_b(const Kastaun& k, const C2PRec& rec, int K, double atmo
   }
   bool conv = fabs(f1) < C2P_TOL;
   bool clamped;
-  if (!conv && good) {
+  const bool redo = !conv && good;
+  if (redo) {
     double fmu;
     const double mu = k.bisection(mup, fmu);
     conv = fabs(fmu) < C2P_TOL;
@@ -240,7 +241,7 @@ SRMHD_DEV int c2p_pass_b(const Kastaun& k, const C2PRec& rec, int K, double atmo
   }
   P.p = eos_press(k.e, P.eps, P.rho);
   return (good ? C2P_BRACKET_OK : 0) | (conv ? C2P_CONVERGED : 0) | (clamped ? C2P_V_CLAMPED : 0) |
-         (atmo ? C2P_ATMO : 0);
+         (atmo ? C2P_ATMO : 0) | (redo ? C2P_REDONE : 0);
 }
 
 }  // namespace srmhd

@@ -41,26 +41,85 @@ __device__ inline bool lane0() { return (threadIdx.x & 63) == 0; }
 
 __device__ inline unsigned long long dbits(double x) { return __double_as_longlong(x); }
 
-// Per-face counters of one flux call (hlld.py:1516-1529), wave-aggregated.
-__device__ void diag_accumulate(srmhd_flux_diag* d, bool active, int st, int iters) {
-  const unsigned long long nc = __popcll(__ballot(active && (st & FS_NOT_CONVERGED)));
-  const unsigned long long nw = __popcll(__ballot(active && (st & FS_WAVE_ORDER)));
-  const unsigned long long nb = __popcll(__ballot(active && (st & FS_BN_ZERO)));
-  const unsigned long long nf = __popcll(__ballot(active && (st & FS_FALLBACK)));
-  const unsigned long long na = __popcll(__ballot(active));
-  const bool conv = active && !(st & FS_NOT_CONVERGED);
-  const unsigned long long ncv = __popcll(__ballot(conv));
-  const unsigned long long si = wave_sum(conv ? (unsigned long long)iters : 0ull);
-  const unsigned long long mx = wave_max(active ? (unsigned long long)iters : 0ull);
+// Block-level combination of per-wave partials: lane 0 of every wave parks
+// its values in LDS, thread 0 folds them and issues one atomic per value.
+// Every thread of the block must call these (no early return before).
+__device__ inline void block_or_atomic(unsigned long long* dst, unsigned long long v) {
+  __shared__ unsigned long long sh[BLK / 64];
+  v = wave_or(v);
+  if (lane0()) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+#pragma unroll
+    for (int w = 0; w < BLK / 64; ++w) t |= sh[w];
+    if (t) atomicOr(dst, t);
+  }
+}
+
+__device__ inline void block_max_atomic2(unsigned long long* dst0, unsigned long long v0,
+                                         unsigned long long* dst1, unsigned long long v1) {
+  __shared__ unsigned long long sh[2][BLK / 64];
+  v0 = wave_max(v0);
+  v1 = wave_max(v1);
   if (lane0()) {
-    if (na) atomicAdd(&d->n_interfaces, na);
-    if (nc) atomicAdd(&d->n_not_converged, nc);
-    if (nw) atomicAdd(&d->n_wave_order_bad, nw);
-    if (nb) atomicAdd(&d->n_bn_zero, nb);
-    if (nf) atomicAdd(&d->n_hlle_fallback, nf);
-    if (ncv) atomicAdd(&d->n_converged, ncv);
-    if (si) atomicAdd(&d->sum_iters_converged, si);
-    if (mx) atomicMax(&d->max_iters, mx);
+    sh[0][threadIdx.x >> 6] = v0;
+    sh[1][threadIdx.x >> 6] = v1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, b = 0;
+#pragma unroll
+    for (int w = 0; w < BLK / 64; ++w) {
+      a = sh[0][w] > a ? sh[0][w] : a;
+      b = sh[1][w] > b ? sh[1][w] : b;
+    }
+    if (a) atomicMax(dst0, a);
+    if (b) atomicMax(dst1, b);
+  }
+}
+
+// Per-face counters of one flux call (hlld.py:1516-1529) plus the max |F_D|
+// of the sweep, block-aggregated.
+__device__ void diag_accumulate(srmhd_flux_diag* d, bool active, int st, int iters,
+                                unsigned long long absFD_bits) {
+  constexpr int NS = 8;
+  unsigned long long v[NS + 2];
+  v[0] = __popcll(__ballot(active));
+  v[1] = __popcll(__ballot(active && (st & FS_NOT_CONVERGED)));
+  v[2] = __popcll(__ballot(active && (st & FS_WAVE_ORDER)));
+  v[3] = __popcll(__ballot(active && (st & FS_BN_ZERO)));
+  v[4] = __popcll(__ballot(active && (st & FS_FALLBACK)));
+  const bool conv = active && !(st & FS_NOT_CONVERGED);
+  v[5] = __popcll(__ballot(conv));
+  v[6] = wave_sum(conv ? (unsigned long long)iters : 0ull);
+  v[7] = __popcll(__ballot(active && (st & FS_C2P_REDONE)));
+  v[NS] = wave_max(active ? (unsigned long long)iters : 0ull);
+  v[NS + 1] = wave_max(active ? absFD_bits : 0ull);
+  __shared__ unsigned long long sh[NS + 2][BLK / 64];
+  if (lane0()) {
+#pragma unroll
+    for (int k = 0; k < NS + 2; ++k) sh[k][threadIdx.x >> 6] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < NS + 2) {
+    const int k = threadIdx.x;
+    unsigned long long t = 0;
+    for (int w = 0; w < BLK / 64; ++w) t = (k < NS) ? t + sh[k][w] : (sh[k][w] > t ? sh[k][w] : t);
+    if (t) {
+      switch (k) {
+        case 0: atomicAdd(&d->n_interfaces, t); break;
+        case 1: atomicAdd(&d->n_not_converged, t); break;
+        case 2: atomicAdd(&d->n_wave_order_bad, t); break;
+        case 3: atomicAdd(&d->n_bn_zero, t); break;
+        case 4: atomicAdd(&d->n_hlle_fallback, t); break;
+        case 5: atomicAdd(&d->n_converged, t); break;
+        case 6: atomicAdd(&d->sum_iters_converged, t); break;
+        case 7: atomicAdd(&d->n_c2p_redo, t); break;
+        case 8: atomicMax(&d->max_iters, t); break;
+        default: atomicMax(&d->absF_D_max, t); break;
+      }
+    }
   }
 }
 
@@ -105,8 +164,7 @@ __global__ __launch_bounds__(BLK) void k_flux_a(int64_t n, PrimPtrs L, PrimPtrs 
     k.init(h.uH, e);
     m = c2p_pass_a(k, rec[i]);
   }
-  m = wave_or(m);
-  if (lane0() && m) atomicOr(&d->c2p_mask, (unsigned long long)m);
+  block_or_atomic(&d->c2p_mask, m);
 }
 
 template <int N, int S>
@@ -132,8 +190,9 @@ __global__ __launch_bounds__(BLK) void k_flux_b(int64_t n, PrimPtrs L, PrimPtrs 
       Kastaun k;
       k.init(h.uH, e);
       Prim ph;
-      c2p_pass_b(k, rec[i], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
+      const int cst = c2p_pass_b(k, rec[i], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
       st = hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, u, ps, iters);
+      if (cst & C2P_REDONE) st |= FS_C2P_REDONE;
     }
 #pragma unroll
     for (int k = 0; k < 8; ++k) F.p[k][i] = f[k];
@@ -144,7 +203,7 @@ __global__ __launch_bounds__(BLK) void k_flux_b(int64_t n, PrimPtrs L, PrimPtrs 
     if (pstar) pstar[i] = ps;
     if (status) status[i] = (uint8_t)st;
   }
-  if (S == SOLVER_HLLD) diag_accumulate(d, active, st, iters);
+  if (S == SOLVER_HLLD) diag_accumulate(d, active, st, iters, 0ull);
 }
 
 // ── flat con2prim / prim2con ─────────────────────────────────────────────────
@@ -169,8 +228,7 @@ __global__ __launch_bounds__(BLK) void k_c2p_a(int64_t n, CConsPtrs C, srmhd_eos
     k.init(u, e);
     m = c2p_pass_a(k, rec[i]);
   }
-  m = wave_or(m);
-  if (lane0() && m) atomicOr(mask, (unsigned long long)m);
+  block_or_atomic(mask, m);
 }
 
 struct OutPrimPtrs {
@@ -388,8 +446,7 @@ __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e
     k.init(h.uH, e);
     m = c2p_pass_a(k, rec[t]);
   }
-  m = wave_or(m);
-  if (lane0() && m) atomicOr(&d->c2p_mask, (unsigned long long)m);
+  block_or_atomic(&d->c2p_mask, m);
 }
 
 template <int N, int LIM, int SOL>
@@ -415,8 +472,9 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
       Kastaun k;
       k.init(h.uH, e);
       Prim ph;
-      c2p_pass_b(k, rec[t], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
+      const int cst = c2p_pass_b(k, rec[t], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
       st = hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, nullptr, ps, iters);
+      if (cst & C2P_REDONE) st |= FS_C2P_REDONE;
     }
     F.p[0][t] = f[CD];
     F.p[1][t] = f[CS];
@@ -427,9 +485,8 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
     F.p[6][t] = N == 0 ? f[CB + 1] : f[CB + 0];   // F^x(By) / F^y(Bx): the face EMF
     fD = f[CD];
   }
-  if (SOL == SOLVER_HLLD) diag_accumulate(d, active, st, iters);
-  const unsigned long long mb = wave_max(active ? dbits(fabs(fD)) : 0ull);
-  if (lane0() && mb) atomicMax(&d->absF_D_max, mb);
+  // the deadband scale of the upwind EMF is needed for every solver
+  diag_accumulate(d, active, st, iters, dbits(fabs(fD)));
 }
 
 // Gardiner-Stone corner EMF (ct.py:119-177) with the ring of zeros
@@ -563,15 +620,15 @@ __global__ __launch_bounds__(BLK) void k_c2p2d_a(G2 g, St2 S, srmhd_eos e, C2PRe
     k.init(u, e);
     m = c2p_pass_a(k, rec[t]);
   }
-  m = wave_or(m);
-  if (lane0() && m) atomicOr(mask, (unsigned long long)m);
+  block_or_atomic(mask, m);
 }
 
 // pass B on the physical cells only: ghost primitives are overwritten by the
 // primitive BCs (driver2d.py:216)
 __global__ __launch_bounds__(BLK) void k_c2p2d_b(G2 g, St2 S, srmhd_eos e, const C2PRec* rec,
                                                  const unsigned long long* mask, double atmo_rho,
-                                                 double atmo_eps, Pr2Out P) {
+                                                 double atmo_eps, Pr2Out P,
+                                                 unsigned long long* redo_count) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (t >= (int64_t)g.nx * g.ny) return;
   const int pi = (int)(t / g.ny), pj = (int)(t - (int64_t)pi * g.ny);
@@ -582,7 +639,8 @@ __global__ __launch_bounds__(BLK) void k_c2p2d_b(G2 g, St2 S, srmhd_eos e, const
   k.init(u, e);
   Prim q;
   const int64_t c = (int64_t)i * g.nyt + j;
-  c2p_pass_b(k, rec[c], c2p_break_iter(*mask), atmo_rho, atmo_eps, q);
+  const int st = c2p_pass_b(k, rec[c], c2p_break_iter(*mask), atmo_rho, atmo_eps, q);
+  if (st & C2P_REDONE) atomicAdd(redo_count, 1ull);
   P.p[0][c] = q.rho;
   P.p[1][c] = q.v[0];
   P.p[2][c] = q.v[1];
@@ -610,9 +668,9 @@ struct Pr9 {  // rho, vx, vy, vz, p, eps, Bx, By, Bz (Bx, By, Bz may be null)
 
 __global__ __launch_bounds__(BLK) void k_dt_reduce(G2 g, St2 S, Pr9 P, srmhd_eos e,
                                                    unsigned long long* mx) {
-  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   unsigned long long bx = 0, by = 0;
-  if (t < (int64_t)g.nx * g.ny) {
+  const int64_t n = (int64_t)g.nx * g.ny;
+  for (int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x; t < n; t += (int64_t)gridDim.x * BLK) {
     const int i = (int)(t / g.ny) + g.ng, j = (int)(t % g.ny) + g.ng;
     const int64_t c = (int64_t)i * g.nyt + j;
     const double rho = P.p[0][c], vx = P.p[1][c], vy = P.p[2][c], vz = P.p[3][c];
@@ -630,18 +688,13 @@ __global__ __launch_bounds__(BLK) void k_dt_reduce(G2 g, St2 S, Pr9 P, srmhd_eos
     const double h = 1.0 + eps + p / rho;
     const double v02 = cs2 + (b2 / (rho * h + b2)) * (1.0 - cs2);
     const double cf = csqrt(chi_(clo(v02, 0.0), 1.0));
-    bx = dbits(fabs(vx) + cf);
-    by = dbits(fabs(vy) + cf);
-    // fabs of a NaN keeps it above +inf in the unsigned order (torch.max propagates NaN)
-    bx &= 0x7fffffffffffffffull;
-    by &= 0x7fffffffffffffffull;
+    // |.| keeps a NaN above +inf in the unsigned order (torch.max propagates NaN)
+    const unsigned long long ax = dbits(fabs(vx) + cf) & 0x7fffffffffffffffull;
+    const unsigned long long ay = dbits(fabs(vy) + cf) & 0x7fffffffffffffffull;
+    bx = ax > bx ? ax : bx;
+    by = ay > by ? ay : by;
   }
-  bx = wave_max(bx);
-  by = wave_max(by);
-  if (lane0()) {
-    atomicMax(&mx[0], bx);
-    atomicMax(&mx[1], by);
-  }
+  block_max_atomic2(&mx[0], bx, &mx[1], by);
 }
 
 __global__ void k_dt_final(const unsigned long long* mx, double cfl, double dx, double dy,
@@ -821,7 +874,7 @@ void c2p_cells(const G2& g, const St2& S, const srmhd_eos& e, double atmo_rho, d
   tend(SRMHD_K_C2P_A, s);
   tbegin(SRMHD_K_C2P_B, s);
   k_c2p2d_b<<<blocks_for((int64_t)g.nx * g.ny), BLK, 0, s>>>(g, S, e, w.recc, w.words + 0,
-                                                             atmo_rho, atmo_eps, P);
+                                                             atmo_rho, atmo_eps, P, w.words + 3);
   tend(SRMHD_K_C2P_B, s);
   k_prim_bc<<<blocks_for(ncell), BLK, 0, s>>>(g, P);
 }
@@ -1020,7 +1073,8 @@ int srmhd2d_dt(const srmhd_grid2d* gp, const srmhd_eos* eos, double cfl, const d
   }
   hipStream_t st = (hipStream_t)stream;
   k_zero_u64<<<1, 64, 0, st>>>(w.words + 1, 2);
-  k_dt_reduce<<<blocks_for((int64_t)g.nx * g.ny), BLK, 0, st>>>(g, s, pr, *eos, w.words + 1);
+  const int nb = blocks_for((int64_t)g.nx * g.ny);
+  k_dt_reduce<<<nb < 2048 ? nb : 2048, BLK, 0, st>>>(g, s, pr, *eos, w.words + 1);
   k_dt_final<<<1, 64, 0, st>>>(w.words + 1, cfl, g.dx, g.dy, dt_dev);
   return launch_status();
 }

@@ -8,7 +8,10 @@
 namespace srmhd {
 
 enum Solver { SOLVER_HLLE = 0, SOLVER_HLLC = 1, SOLVER_HLLD = 2 };
-enum FaceStatus { FS_NOT_CONVERGED = 1, FS_WAVE_ORDER = 2, FS_BN_ZERO = 4, FS_FALLBACK = 8 };
+enum FaceStatus {
+  FS_NOT_CONVERGED = 1, FS_WAVE_ORDER = 2, FS_BN_ZERO = 4, FS_FALLBACK = 8,
+  FS_C2P_REDONE = 16   // the HLL-state con2prim needed the bisection continuation
+};
 
 constexpr double PSTAR_TOL = 1.0e-10;   // hlld.py:1080
 constexpr int PSTAR_MAXIT = 30;         // hlld.py:1327

@@ -29,7 +29,8 @@ class Eos(ctypes.Structure):
 class FluxDiag(ctypes.Structure):
     _fields_ = [(n, ctypes.c_ulonglong) for n in (
         "n_interfaces", "n_not_converged", "n_wave_order_bad", "n_bn_zero", "n_hlle_fallback",
-        "sum_iters_converged", "n_converged", "max_iters", "c2p_mask", "absF_D_max")]
+        "sum_iters_converged", "n_converged", "max_iters", "c2p_mask", "absF_D_max",
+        "n_c2p_redo")]
 
 
 class Grid2D(ctypes.Structure):
@@ -137,4 +138,5 @@ def diag_to_dict(words, solver, idir):
                 n_wave_order_bad=nwo, n_bn_zero=nbz, n_hlle_fallback=nfb,
                 frac_hlle_fallback=nfb / max(n, 1),
                 mean_iters=(si / ncv) if ncv else float("nan"),
-                pstar_method="secant", pstar_cost=None, max_iters=mx)
+                pstar_method="secant", pstar_cost=None, max_iters=mx,
+                n_c2p_redo=w[10] if len(w) > 10 else 0)

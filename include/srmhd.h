@@ -55,6 +55,7 @@ typedef struct srmhd_flux_diag {
   unsigned long long max_iters;
   unsigned long long c2p_mask;      /* OR of the HLL-state con2prim masks */
   unsigned long long absF_D_max;    /* bits of max |F_D| (upwind-EMF deadband scale) */
+  unsigned long long n_c2p_redo;    /* HLL-state con2prims finished by the bisection redo */
 } srmhd_flux_diag;
 
 enum { SRMHD_HLLE = 0, SRMHD_HLLC = 1, SRMHD_HLLD = 2 };
@@ -67,7 +68,8 @@ enum { SRMHD_BC_OUTFLOW = 0, SRMHD_BC_PERIODIC = 1 };
  * 1202) behind the contract flux_fn(sL, sR, eos, idir) -> (F, U, p_star)
  * (src/physics/sweep.py:96, driver.py:168).  n interfaces; primL/primR: 9
  * pointers; F, U: 8 pointers (U may be NULL); pstar, status (uint8 per face:
- * bit0 not converged, bit1 wave order, bit2 B_n = 0, bit3 HLLE fallback) may
+ * bit0 not converged, bit1 wave order, bit2 B_n = 0, bit3 HLLE fallback, bit4
+ * HLL-state con2prim redone by bisection) may
  * be NULL; diag (device) may be NULL.  HLLD runs the reference's
  * batch-coupled con2prim of the HLL state over all n faces as one batch. */
 int64_t srmhd_flux_workspace_bytes(int solver, int64_t n);
@@ -79,8 +81,8 @@ int srmhd_flux(int solver, int idir, int64_t n, const double* const* primL,
 /* ── con2prim ────────────────────────────────────────────────────────────────
  * Replaces conservative_to_primitive (src/physics/c2p.py:313-363) over one
  * batch of n cells: cons 8 pointers in, prim 9 pointers out, status (uint8:
- * bit0 bracket_ok, bit1 converged, bit2 v_clamped, bit3 atmo_floored; may be
- * NULL). */
+ * bit0 bracket_ok, bit1 converged, bit2 v_clamped, bit3 atmo_floored, bit4 finished
+ * by the bisection redo; may be NULL). */
 int64_t srmhd_c2p_workspace_bytes(int64_t n);
 int srmhd_c2p(int64_t n, const double* const* cons, const srmhd_eos* eos, double atmo_rho,
               double atmo_eps, double* const* prim, uint8_t* status, void* work,
