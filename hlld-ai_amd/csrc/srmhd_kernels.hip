@@ -879,6 +879,215 @@ void c2p_cells(const G2& g, const St2& S, const srmhd_eos& e, double atmo_rho, d
   k_prim_bc<<<blocks_for(ncell), BLK, 0, s>>>(g, P);
 }
 
+// ── 1D shock tube (driver.py:135-236) ────────────────────────────────────────
+
+struct G1 {
+  int n, ng, nt, bc;   // bc: 0 outflow, 1 periodic, 2 constant
+  double dx;
+};
+
+// source of a 1D ghost cell for outflow / periodic (grid.py:49-71)
+__device__ inline int src1(int i, const G1& g) {
+  if (i < g.ng) return g.bc == SRMHD_BC_PERIODIC ? i + g.n : g.ng;
+  if (i >= g.ng + g.n) return g.bc == SRMHD_BC_PERIODIC ? i - g.n : g.ng + g.n - 1;
+  return i;
+}
+
+// get_interface_states (reconstruction.py:354-373): every primitive key is
+// reconstructed directly (no W v, no floors, eps too) on the physical faces
+// ng .. ng+n of the whole array.
+template <int LIM>
+__global__ __launch_bounds__(BLK) void k_recon1d(G1 g, PrimPtrs P, OutPrimPtrs L, OutPrimPtrs R) {
+  const int j = blockIdx.x * BLK + threadIdx.x;
+  if (j > g.n) return;
+  const int f = g.ng + j;
+  constexpr int w = stencil_halfwidth(LIM);
+  double q[8];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+#pragma unroll
+    for (int m = 3 - w; m <= 4 + w; ++m) q[m] = P.p[k][clampi(f + m - 4, 0, g.nt - 1)];
+    double ql, qr;
+    face_states<LIM>(q, f, g.nt, ql, qr);
+    L.p[k][j] = ql;
+    R.p[k][j] = qr;
+  }
+}
+
+struct CC8 {
+  const double* p[8];
+};
+
+// Heun stage (driver.py:219-232): stage 1 U1 = U + dt k1, stage 2
+// U' = 0.5 (U + U1 + dt k2), with k = (F[j] - F[j+1])/dx on physical cells
+// (0 on ghosts), F[Bx] := 0, then the BC of every ghost and Bx := U's Bx.
+// Cons order: D, tau, Sx, Sy, Sz, Bx, By, Bz.
+__global__ __launch_bounds__(BLK) void k_heun1d(G1 g, int stage, const double* dt_dev, CC8 U,
+                                                CC8 U1, CC8 F, ConsPtrs out) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= g.nt) return;
+  const double dt = *dt_dev;
+  const bool ghost = i < g.ng || i >= g.ng + g.n;
+  const int s = (g.bc == 2) ? i : src1(i, g);
+  const bool sphys = s >= g.ng && s < g.ng + g.n;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    double v;
+    if (k == CB) {
+      v = U.p[k][i];
+    } else if (ghost && g.bc == 2) {
+      v = U.p[k][i];
+    } else {
+      double rhs = 0.0;
+      if (sphys) {
+        const int j = s - g.ng;
+        rhs = (F.p[k][j] - F.p[k][j + 1]) / g.dx;
+      }
+      v = stage == 1 ? U.p[k][s] + dt * rhs : 0.5 * (U.p[k][s] + U1.p[k][s] + dt * rhs);
+    }
+    out.p[k][i] = v;
+  }
+}
+
+// primitive BCs of the 1D driver; constant takes the old primitives
+__global__ __launch_bounds__(BLK) void k_primbc1d(G1 g, OutPrimPtrs P, PrimPtrs Pold) {
+  const int i = blockIdx.x * BLK + threadIdx.x;
+  if (i >= g.nt) return;
+  if (i >= g.ng && i < g.ng + g.n) return;
+  if (g.bc == 2) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) P.p[k][i] = Pold.p[k][i];
+    return;
+  }
+  const int s = src1(i, g);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) P.p[k][i] = P.p[k][s];
+}
+
+// compute_dt (driver.py:77-127): (cfl dx) / max(|vx| + v_fast), torch's
+// float / tensor being reciprocal(tensor) * float
+__global__ __launch_bounds__(BLK) void k_dt1d(G1 g, PrimPtrs P, srmhd_eos e, double cfl,
+                                              int speed_of_light, double* dt) {
+  unsigned long long bx = 0;
+  for (int t = threadIdx.x; t < g.n; t += BLK) {
+    const int c = t + g.ng;
+    const double rho = P.p[0][c], vx = P.p[1][c], vy = P.p[2][c], vz = P.p[3][c];
+    const double eps = P.p[5][c], Bx = P.p[6][c], By = P.p[7][c], Bz = P.p[8][c];
+    double p, cs2;
+    eos_press_cs2(e, eps, rho, p, cs2);
+    const double v2 = sq(vx) + sq(vy) + sq(vz);
+    const double W = 1.0 / csqrt(clo(1.0 - v2, 1e-10));
+    const double B2 = sq(Bx) + sq(By) + sq(Bz);
+    const double vB = vx * Bx + vy * By + vz * Bz;
+    const double b2 = (B2 + vB * vB) / (W * W);
+    const double h = 1.0 + eps + p / rho;
+    const double v02 = cs2 + (b2 / (rho * h + b2)) * (1.0 - cs2);
+    const double vf = csqrt(chi_(clo(v02, 0.0), 1.0));
+    const unsigned long long a = dbits(fabs(vx) + vf) & 0x7fffffffffffffffull;
+    bx = a > bx ? a : bx;
+  }
+  __shared__ unsigned long long sh[BLK / 64];
+  bx = wave_max(bx);
+  if (lane0()) sh[threadIdx.x >> 6] = bx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = 0;
+    for (int w = 0; w < BLK / 64; ++w) m = sh[w] > m ? sh[w] : m;
+    double smax = clo(__longlong_as_double(m), 1e-14);
+    if (speed_of_light) smax = 1.0;
+    *dt = (1.0 / smax) * (cfl * g.dx);
+  }
+}
+
+struct Ws1 {
+  OutPrimPtrs L, R, P1;
+  ConsPtrs F, U1;
+  C2PRec *recf, *recc;
+  srmhd_flux_diag* diag;    // [0] stage-1 flux, [1] stage-2 flux
+  unsigned long long* mask;  // cell con2prim batch mask, one per stage
+  int64_t bytes;
+};
+
+Ws1 carve1d(const G1& g, void* work) {
+  Carver c{(char*)work};
+  Ws1 w;
+  const int64_t nf = g.n + 1, nt = g.nt;
+  w.diag = c.take<srmhd_flux_diag>(2);
+  w.mask = c.take<unsigned long long>(2);
+  for (int k = 0; k < 9; ++k) w.L.p[k] = c.take<double>(nf);
+  for (int k = 0; k < 9; ++k) w.R.p[k] = c.take<double>(nf);
+  for (int k = 0; k < 9; ++k) w.P1.p[k] = c.take<double>(nt);
+  for (int k = 0; k < 8; ++k) w.F.p[k] = c.take<double>(nf);
+  for (int k = 0; k < 8; ++k) w.U1.p[k] = c.take<double>(nt);
+  w.recf = c.take<C2PRec>(nf);
+  w.recc = c.take<C2PRec>(nt);
+  w.bytes = c.off;
+  return w;
+}
+
+template <int LIM>
+void recon1d(const G1& g, const PrimPtrs& P, const Ws1& w, hipStream_t s) {
+  k_recon1d<LIM><<<blocks_for(g.n + 1), BLK, 0, s>>>(g, P, w.L, w.R);
+}
+
+void recon1d_dispatch(int lim, const G1& g, const PrimPtrs& P, const Ws1& w, hipStream_t s) {
+  switch (lim) {
+    case LIM_PCM: recon1d<LIM_PCM>(g, P, w, s); break;
+    case LIM_MC: recon1d<LIM_MC>(g, P, w, s); break;
+    case LIM_MINMOD: recon1d<LIM_MINMOD>(g, P, w, s); break;
+    case LIM_WENO5Z: recon1d<LIM_WENO5Z>(g, P, w, s); break;
+    case LIM_MP5: recon1d<LIM_MP5>(g, P, w, s); break;
+    default: recon1d<LIM_MP7>(g, P, w, s);
+  }
+}
+
+PrimPtrs as_in(const OutPrimPtrs& o) {
+  PrimPtrs p;
+  for (int k = 0; k < 9; ++k) p.p[k] = o.p[k];
+  return p;
+}
+
+// one stage: reconstruction, the flux on the n+1 physical faces, the Heun
+// combination with BCs, con2prim of every cell, primitive BCs
+int stage1d(const G1& g, int solver, int lim, const srmhd_eos& e, double atmo_rho, int stage,
+            const double* dt_dev, const CC8& U, const CC8& U1, const PrimPtrs& P,
+            const PrimPtrs& Pold, const ConsPtrs& out, const OutPrimPtrs& Pout, const Ws1& w,
+            hipStream_t s) {
+  recon1d_dispatch(lim, g, P, w, s);
+  ConsPtrs none;
+  for (int k = 0; k < 8; ++k) none.p[k] = nullptr;
+  srmhd_flux_diag* d = w.diag + (stage - 1);
+  const int rc = flux_dispatch<0>(solver, g.n + 1, as_in(w.L), as_in(w.R), e, w.recf, d, w.F,
+                                  none, 0, nullptr, nullptr, s);
+  if (rc) return rc;
+  CC8 F;
+  for (int k = 0; k < 8; ++k) F.p[k] = w.F.p[k];
+  k_heun1d<<<blocks_for(g.nt), BLK, 0, s>>>(g, stage, dt_dev, U, U1, F, out);
+  CConsPtrs C;
+  for (int k = 0; k < 8; ++k) C.p[k] = out.p[k];
+  unsigned long long* mask = w.mask + (stage - 1);
+  k_c2p_a<<<blocks_for(g.nt), BLK, 0, s>>>(g.nt, C, e, w.recc, mask);
+  k_c2p_b<<<blocks_for(g.nt), BLK, 0, s>>>(g.nt, C, e, w.recc, mask, atmo_rho, 1e-10, Pout,
+                                           nullptr);
+  k_primbc1d<<<blocks_for(g.nt), BLK, 0, s>>>(g, Pout, Pold);
+  return launch_status();
+}
+
+bool g1_ok(const srmhd_grid1d* g) {
+  return g && g->ncells >= 1 && g->ng >= 1 && g->bc >= 0 && g->bc <= 2 && g->dx > 0.0 &&
+         !(g->bc == SRMHD_BC_PERIODIC && g->ncells < g->ng);
+}
+
+G1 make_g1(const srmhd_grid1d* g) {
+  G1 r;
+  r.n = g->ncells;
+  r.ng = g->ng;
+  r.nt = g->ncells + 2 * g->ng;
+  r.bc = g->bc;
+  r.dx = g->dx;
+  return r;
+}
+
 }  // namespace
 
 // ── C ABI ───────────────────────────────────────────────────────────────────
@@ -1076,6 +1285,59 @@ int srmhd2d_dt(const srmhd_grid2d* gp, const srmhd_eos* eos, double cfl, const d
   const int nb = blocks_for((int64_t)g.nx * g.ny);
   k_dt_reduce<<<nb < 2048 ? nb : 2048, BLK, 0, st>>>(g, s, pr, *eos, w.words + 1);
   k_dt_final<<<1, 64, 0, st>>>(w.words + 1, cfl, g.dx, g.dy, dt_dev);
+  return launch_status();
+}
+
+int64_t srmhd1d_workspace_bytes(const srmhd_grid1d* g) {
+  if (!g1_ok(g)) return -1;
+  return carve1d(make_g1(g), nullptr).bytes;
+}
+
+int srmhd1d_rk2_step(const srmhd_grid1d* gp, int solver, int limiter, const srmhd_eos* eos,
+                     double atmo_rho, const double* dt_dev, const double* const* cons,
+                     const double* const* prim, double* const* cons_out, double* const* prim_out,
+                     srmhd_flux_diag* diag2, void* work, int64_t work_bytes, void* stream) {
+  if (!g1_ok(gp) || solver < 0 || solver > 2 || limiter < 0 || limiter > 5 || !eos_ok(eos) ||
+      !dt_dev || !cons || !prim || !cons_out || !prim_out || !work)
+    return SRMHD_EARG;
+  const G1 g = make_g1(gp);
+  const Ws1 w = carve1d(g, work);
+  if (work_bytes < w.bytes) return SRMHD_EARG;
+  CC8 U, U1;
+  ConsPtrs out;
+  PrimPtrs P;
+  OutPrimPtrs Pout;
+  for (int k = 0; k < 8; ++k) {
+    U.p[k] = cons[k];
+    U1.p[k] = w.U1.p[k];
+    out.p[k] = cons_out[k];
+    if (!U.p[k] || !out.p[k] || out.p[k] == U.p[k]) return SRMHD_EARG;
+  }
+  for (int k = 0; k < 9; ++k) {
+    P.p[k] = prim[k];
+    Pout.p[k] = prim_out[k];
+    if (!P.p[k] || !Pout.p[k] || Pout.p[k] == P.p[k]) return SRMHD_EARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)w.diag, N_WORDS);
+  k_zero_u64<<<1, 64, 0, s>>>(w.mask, 2);
+  int rc = stage1d(g, solver, limiter, *eos, atmo_rho, 1, dt_dev, U, U1, P, P, w.U1, w.P1, w, s);
+  if (rc) return rc;
+  rc = stage1d(g, solver, limiter, *eos, atmo_rho, 2, dt_dev, U, U1, as_in(w.P1), P, out, Pout,
+               w, s);
+  if (rc) return rc;
+  if (diag2) k_copy_diag<<<1, 64, 0, s>>>(w.diag, diag2, 2);
+  return launch_status();
+}
+
+int srmhd1d_dt(const srmhd_grid1d* gp, const srmhd_eos* eos, double cfl, int speed_of_light,
+               const double* const* prim, double* dt_dev, void* stream) {
+  if (!g1_ok(gp) || !eos_ok(eos) || !prim || !dt_dev) return SRMHD_EARG;
+  const G1 g = make_g1(gp);
+  PrimPtrs P;
+  for (int k = 0; k < 9; ++k)
+    if (!(P.p[k] = prim[k])) return SRMHD_EARG;
+  k_dt1d<<<1, BLK, 0, (hipStream_t)stream>>>(g, P, *eos, cfl, speed_of_light, dt_dev);
   return launch_status();
 }
 

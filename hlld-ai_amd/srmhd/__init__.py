@@ -7,6 +7,6 @@ Python host code mirroring the reference's plugin and driver interfaces
 include/srmhd.h.  There is no CPU fallback.
 """
 
-from . import _lib, eos, hlld, c2p, grid, state, driver2d, initial_data2d  # noqa: F401
+from . import _lib, eos, hlld, c2p, grid, state, driver, driver2d, initial_data2d  # noqa: F401
 
-__all__ = ["eos", "hlld", "c2p", "grid", "state", "driver2d", "initial_data2d"]
+__all__ = ["eos", "hlld", "c2p", "grid", "state", "driver", "driver2d", "initial_data2d"]

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(_HERE, "_lib", "libsrmhd_hip.so")
 
 SOLVER_IDS = {"hlle": 0, "hllc": 1, "hlld": 2}
 LIMITER_IDS = {"pcm": 0, "mc": 1, "minmod": 2, "weno5z": 3, "mp5": 4, "mp7": 5}
-BC_IDS = {"outflow": 0, "periodic": 1}
+BC_IDS = {"outflow": 0, "periodic": 1, "constant": 2}
 
 EARG = 1
 EHIP = 1000
@@ -37,6 +37,11 @@ class Grid2D(ctypes.Structure):
     _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("ng", ctypes.c_int32),
                 ("bc_x", ctypes.c_int32), ("bc_y", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("dx", ctypes.c_double), ("dy", ctypes.c_double)]
+
+
+class Grid1D(ctypes.Structure):
+    _fields_ = [("ncells", ctypes.c_int32), ("ng", ctypes.c_int32), ("bc", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("dx", ctypes.c_double)]
 
 
 class Scheme2D(ctypes.Structure):
@@ -68,6 +73,12 @@ _SIGS = {
                                     ctypes.POINTER(Eos), _PP, _PP, _P, _I64, _P]),
     "srmhd2d_dt": (ctypes.c_int, [ctypes.POINTER(Grid2D), ctypes.POINTER(Eos), _D, _PP, _PP, _P,
                                   _P, _I64, _P]),
+    "srmhd1d_workspace_bytes": (_I64, [ctypes.POINTER(Grid1D)]),
+    "srmhd1d_rk2_step": (ctypes.c_int, [ctypes.POINTER(Grid1D), ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(Eos), _D, _P, _PP, _PP, _PP, _PP, _P, _P,
+                                        _I64, _P]),
+    "srmhd1d_dt": (ctypes.c_int, [ctypes.POINTER(Grid1D), ctypes.POINTER(Eos), _D, ctypes.c_int,
+                                  _PP, _P, _P]),
     "srmhd_timing_enable": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "srmhd_timing_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_int64)]),

@@ -132,6 +132,34 @@ int srmhd2d_dt(const srmhd_grid2d* g, const srmhd_eos* eos, double cfl,
                const double* const* U, const double* const* P, double* dt_dev, void* work,
                int64_t work_bytes, void* stream);
 
+/* ── 1D shock tube (driver.run) ─────────────────────────────────────────────
+ * Grid1D (src/physics/grid.py:23-43): ncells physical cells, ng ghosts per
+ * side; bc 0 outflow, 1 periodic, 2 constant (ghosts keep the step-start
+ * values, grid.py:49-87). */
+typedef struct srmhd_grid1d {
+  int32_t ncells, ng, bc, reserved;
+  double dx;
+} srmhd_grid1d;
+
+int64_t srmhd1d_workspace_bytes(const srmhd_grid1d* g);
+
+/* One Heun SSP-RK2 step of driver.rk2_step (src/physics/driver.py:192-236):
+ * get_interface_states on the ncells+1 physical faces (every primitive key
+ * reconstructed directly, reconstruction.py:354-373), the flux (HLLD's
+ * HLL-state con2prim batch = those faces), F[Bx] = 0, the BCs, Bx restored,
+ * con2prim of every cell as one batch, primitive BCs.  cons/cons_out: 8
+ * conserved pointers (D, tau, Sx, Sy, Sz, Bx, By, Bz) of ncells+2ng; prim /
+ * prim_out: 9 primitive pointers.  dt from device memory.  diag2 (device,
+ * may be NULL) gets the two stages' flux counters. */
+int srmhd1d_rk2_step(const srmhd_grid1d* g, int solver, int limiter, const srmhd_eos* eos,
+                     double atmo_rho, const double* dt_dev, const double* const* cons,
+                     const double* const* prim, double* const* cons_out, double* const* prim_out,
+                     srmhd_flux_diag* diag2, void* work, int64_t work_bytes, void* stream);
+
+/* compute_dt (src/physics/driver.py:77-127) into *dt_dev. */
+int srmhd1d_dt(const srmhd_grid1d* g, const srmhd_eos* eos, double cfl, int speed_of_light,
+               const double* const* prim, double* dt_dev, void* stream);
+
 /* ── live kernel timing (measurement only; no counterpart in the reference) ──
  * srmhd_timing_enable(cls, n): from now on srmhd2d_stage records a HIP event
  * pair around each launch of kernel class cls, on its stream, for up to n
