@@ -205,24 +205,43 @@ SRMHD_DEV int c2p_break_iter(uint64_t ormask) {
 // status bits written by the con2prim entry points
 enum { C2P_BRACKET_OK = 1, C2P_CONVERGED = 2, C2P_V_CLAMPED = 4, C2P_ATMO = 8, C2P_REDONE = 16 };
 
-// Pass B: full conservative_to_primitive of one element given its pass-A
-// record and the batch break iteration K.   c2p.py:313-363
-SRMHD_DEV int c2p_pass_b(const Kastaun& k, const C2PRec& rec, int K, double atmo_rho,
-                         double atmo_eps, Prim& P) {
-  const bool good = rec.mup_signed > 0.0;
-  const double mup = fabs(rec.mup_signed);
-  double x1 = rec.x1, f1 = rec.f1;
+// the secant state after the batch's K steps (pass A kept the 60-step one)
+SRMHD_DEV void c2p_secant_at(const Kastaun& k, const C2PRec& rec, int K, double& x1,
+                             double& f1) {
+  x1 = rec.x1;
+  f1 = rec.f1;
   if (K < C2P_SECANT) {
     double x0 = 0.0;
-    x1 = mup * (1.0 - 1e-10);
+    x1 = fabs(rec.mup_signed) * (1.0 - 1e-10);
     double f0 = k.master(x0);
     f1 = k.master(x1);
 #pragma unroll 1
     for (int it = 0; it < K; ++it) k.secant_step(x0, f0, x1, f1);
   }
+}
+
+// whether the element goes to the bisection redo: not converged after the
+// batch's K secant steps but with a good bracket   c2p.py:338
+SRMHD_DEV bool c2p_needs_redo(const Kastaun& k, const C2PRec& rec, int K) {
+  double x1, f1;
+  c2p_secant_at(k, rec, K, x1, f1);
+  return !(fabs(f1) < C2P_TOL) && rec.mup_signed > 0.0;
+}
+
+// Pass B: full conservative_to_primitive of one element given its pass-A
+// record and the batch break iteration K.   c2p.py:313-363
+// force < 0: the element is redone by bisection when it needs it itself;
+// force >= 0: the caller decides (force != 0 redoes), for batches whose redo
+// set is wider than the elements that need it.
+SRMHD_DEV int c2p_pass_b(const Kastaun& k, const C2PRec& rec, int K, double atmo_rho,
+                         double atmo_eps, Prim& P, int force = -1) {
+  const bool good = rec.mup_signed > 0.0;
+  const double mup = fabs(rec.mup_signed);
+  double x1, f1;
+  c2p_secant_at(k, rec, K, x1, f1);
   bool conv = fabs(f1) < C2P_TOL;
   bool clamped;
-  const bool redo = !conv && good;
+  const bool redo = force < 0 ? (!conv && good) : (force != 0);
   if (redo) {
     double fmu;
     const double mu = k.bisection(mup, fmu);

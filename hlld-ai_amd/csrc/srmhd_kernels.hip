@@ -4,42 +4,13 @@
 // Reductions are wave-level (shuffles, ballots) followed by one atomic per
 // wave on 64-bit words; every reduced quantity is an integer, a bit mask or a
 // max, so results are deterministic.
+#include "srmhd_launch.h"
 #include "srmhd_recon.h"
 #include "srmhd_riemann.h"
 
 using namespace srmhd;
 
 namespace {
-
-constexpr int BLK = 256;
-
-inline int blocks_for(int64_t n) { return (int)((n + BLK - 1) / BLK); }
-
-inline int hip_status(hipError_t e) { return e == hipSuccess ? SRMHD_OK : SRMHD_EHIP + (int)e; }
-
-inline int launch_status() { return hip_status(hipGetLastError()); }
-
-__device__ inline unsigned long long wave_or(unsigned long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off);
-  return v;
-}
-__device__ inline unsigned long long wave_max(unsigned long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(v, off);
-    v = o > v ? o : v;
-  }
-  return v;
-}
-__device__ inline unsigned long long wave_sum(unsigned long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-__device__ inline bool lane0() { return (threadIdx.x & 63) == 0; }
-
-__device__ inline unsigned long long dbits(double x) { return __double_as_longlong(x); }
 
 // Block-level combination of per-wave partials: lane 0 of every wave parks
 // its values in LDS, thread 0 folds them and issues one atomic per value.
@@ -312,7 +283,6 @@ __device__ inline int src_face(int f, int n, int ng, int bc) {
   return f;
 }
 
-__device__ inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
 
 // cell-centred B from the staggered field (ct.py:44-54)
 __device__ inline double bcx(const G2& g, const double* Bxf, int i, int j) {
@@ -723,29 +693,15 @@ struct Timing {
 
 inline void tbegin(int cls, hipStream_t s) {
   if (cls == g_timing.cls && g_timing.used < g_timing.cap)
-    hipEventRecord(g_timing.ev[2 * g_timing.used], s);
+    (void)hipEventRecord(g_timing.ev[2 * g_timing.used], s);
 }
 inline void tend(int cls, hipStream_t s) {
   if (cls == g_timing.cls && g_timing.used < g_timing.cap) {
-    hipEventRecord(g_timing.ev[2 * g_timing.used + 1], s);
+    (void)hipEventRecord(g_timing.ev[2 * g_timing.used + 1], s);
     ++g_timing.used;
   }
 }
 
-inline int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
-
-struct Carver {
-  char* base;
-  int64_t off = 0;
-  template <typename T>
-  T* take(int64_t count) {
-    T* p = (T*)(base + off);
-    off += align256(count * (int64_t)sizeof(T));
-    return p;
-  }
-};
-
-bool eos_ok(const srmhd_eos* e) { return e && e->gamma > 1.0; }
 
 template <int N>
 int flux_dispatch(int solver, int64_t n, const PrimPtrs& L, const PrimPtrs& R, const srmhd_eos& e,
@@ -1344,7 +1300,7 @@ int srmhd1d_dt(const srmhd_grid1d* gp, const srmhd_eos* eos, double cfl, int spe
 int srmhd_timing_enable(int kernel_class, int max_launches) {
   if (max_launches < 0 || max_launches > (1 << 20)) return SRMHD_EARG;
   if (g_timing.ev) {
-    for (int i = 0; i < 2 * g_timing.cap; ++i) hipEventDestroy(g_timing.ev[i]);
+    for (int i = 0; i < 2 * g_timing.cap; ++i) (void)hipEventDestroy(g_timing.ev[i]);
     delete[] g_timing.ev;
     g_timing.ev = nullptr;
   }

@@ -1,0 +1,57 @@
+// Host/launch helpers shared by the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "srmhd_common.h"
+
+namespace srmhd {
+
+constexpr int BLK = 256;
+
+inline int blocks_for(int64_t n) { return (int)((n + BLK - 1) / BLK); }
+
+inline int hip_status(hipError_t e) { return e == hipSuccess ? SRMHD_OK : SRMHD_EHIP + (int)e; }
+
+inline int launch_status() { return hip_status(hipGetLastError()); }
+
+__device__ inline unsigned long long wave_or(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off);
+  return v;
+}
+__device__ inline unsigned long long wave_max(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ inline bool lane0() { return (threadIdx.x & 63) == 0; }
+
+__device__ inline unsigned long long dbits(double x) { return __double_as_longlong(x); }
+
+__device__ inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+inline int64_t align256(int64_t x) { return (x + 255) & ~(int64_t)255; }
+
+struct Carver {
+  char* base;
+  int64_t off = 0;
+  template <typename T>
+  T* take(int64_t count) {
+    T* p = (T*)(base + off);
+    off += align256(count * (int64_t)sizeof(T));
+    return p;
+  }
+};
+
+inline bool eos_ok(const srmhd_eos* e) { return e && e->gamma > 1.0; }
+
+}  // namespace srmhd

@@ -18,8 +18,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(ROOT, "csrc")
 OUT_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(OUT_DIR, "libsrmhd_hip.so")
-SOURCES = ["srmhd_kernels.hip"]
-HEADERS = ["srmhd_common.h", "srmhd_c2p.h", "srmhd_recon.h", "srmhd_riemann.h"]
+SOURCES = ["srmhd_kernels.hip", "srmhd_tubes.hip"]
+HEADERS = ["srmhd_common.h", "srmhd_c2p.h", "srmhd_recon.h", "srmhd_riemann.h", "srmhd_launch.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -30,18 +30,37 @@ FLAGS = [
 ]
 
 
-def build(verbose=False, extra=()):
-    os.makedirs(OUT_DIR, exist_ok=True)
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, *extra, "-I", CSRC, "-o", tmp] + [os.path.join(CSRC, s) for s in SOURCES]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError(f"hipcc failed ({r.returncode})")
-    if verbose and (r.stdout or r.stderr):
-        sys.stderr.write(r.stdout + r.stderr)
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+
+def build(verbose=False, extra=()):
+    """Compile every source to an object in parallel, then link the library."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    objs, procs = [], []
+    for src in SOURCES:
+        obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmd = [HIPCC, *FLAGS, *extra, "-c", "-I", CSRC, "-o", obj, os.path.join(CSRC, src)]
+        procs.append((src, _run([c for c in cmd if c != "-shared"], verbose)))
+    failed = []
+    for src, p in procs:
+        out = p.communicate()[0]
+        if p.returncode != 0:
+            failed.append(src)
+            sys.stderr.write(out)
+        elif verbose and out:
+            sys.stderr.write(out)
+    if failed:
+        raise RuntimeError(f"hipcc failed for {failed}")
+    tmp = LIB + ".tmp"
+    p = _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
+    out = p.communicate()[0]
+    if p.returncode != 0:
+        sys.stderr.write(out)
+        raise RuntimeError(f"link failed ({p.returncode})")
     os.replace(tmp, LIB)
     return LIB
 

@@ -50,6 +50,12 @@ class Scheme2D(ctypes.Structure):
                 ("atmo_eps", ctypes.c_double)]
 
 
+class TubeLayout(ctypes.Structure):
+    _fields_ = [("off_c", ctypes.c_void_p), ("ncells", ctypes.c_void_p), ("dx", ctypes.c_void_p),
+                ("Bn", ctypes.c_void_p), ("order", ctypes.c_void_p),
+                ("dx_shared", ctypes.c_double)]
+
+
 DIAG_WORDS = ctypes.sizeof(FluxDiag) // 8
 
 _lib = None
@@ -79,6 +85,16 @@ _SIGS = {
                                         _I64, _P]),
     "srmhd1d_dt": (ctypes.c_int, [ctypes.POINTER(Grid1D), ctypes.POINTER(Eos), _D, ctypes.c_int,
                                   _PP, _P, _P]),
+    "srmhd_tubes_workspace_bytes": (_I64, [ctypes.c_int, _I64]),
+    "srmhd_tubes_run": (ctypes.c_int, [ctypes.c_int, _I64, ctypes.POINTER(TubeLayout),
+                                       ctypes.POINTER(Eos), ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, _D, _D, ctypes.c_int, _PP, _PP,
+                                       _P, _P, _I64, _P]),
+    "srmhd_tubes_clock_bytes": (_I64, [_I64]),
+    "srmhd_tubes_shared_step": (ctypes.c_int, [ctypes.c_int, _I64, ctypes.POINTER(TubeLayout),
+                                               ctypes.POINTER(Eos), ctypes.c_int, ctypes.c_int,
+                                               _D, _D, _D, ctypes.c_int, _PP, _PP, _P, _P, _I64,
+                                               _P]),
     "srmhd_timing_enable": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
     "srmhd_timing_read": (ctypes.c_int, [ctypes.POINTER(ctypes.c_double),
                                          ctypes.POINTER(ctypes.c_int64)]),

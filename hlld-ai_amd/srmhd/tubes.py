@@ -1,0 +1,370 @@
+"""Batched 1D Riemann tubes on the HIP path.
+
+Two forms:
+
+* ``run_tubes(UL, UR, Bn, eos, *, ncells, tend, cfl, limiter, ng, max_steps,
+  flux)`` -- the reference's entry point (src/physics/tube_seed.py:93-144):
+  N equal tubes, ONE dt for all of them and one con2prim batch over every
+  tube, Heun SSP-RK2 to ``tend``.  Each loop iteration is one
+  ``srmhd_tubes_shared_step`` call (a short chain of grid-wide launches that
+  keeps t, dt and the loop test on the device), so ``max_steps`` iterations
+  are captured in one HIP graph and replayed with no host round trip.
+* ``TubeBatch`` -- thousands of tubes of uneven length in one ragged layout,
+  each with its own dt (the 1D driver's bound, src/physics/driver.py:77-127),
+  SSP-RK2/3, any number of steps in ONE persistent launch
+  (``srmhd_tubes_run``: a workgroup owns a tube; the workgroup size is picked
+  by ``autotune_block``).
+
+There is no CPU fallback: without the HIP library or a device every call
+raises.
+"""
+
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .eos import as_c
+from .hlld import solver_name
+from .reconstruction import ghosts_needed
+
+U_KEYS = ("D", "Sx", "Sy", "Sz", "tau", "By", "Bz")   # the C ABI's evolved order
+P_KEYS = ("rho", "vx", "vy", "vz", "p", "eps")
+SIDE_KEYS = ("rho", "p", "vx", "vy", "vz", "By", "Bz")
+
+
+TubeLayout = _lib.TubeLayout
+
+
+def draw_tubes(n_tubes, seed=0, lmin=128, lmax=2048):
+    """The batched-tube workload: lengths U{lmin..lmax}; per tube rho_L, rho_R
+    log-uniform [0.1, 10], p_L, p_R log-uniform [0.01, 10], velocity direction
+    uniform on the sphere with |v| ~ U[0, 0.9] per side, By, Bz ~ U[-2, 2] per
+    side, one B_n ~ U[-1, 1].  Drawn with a CPU torch.Generator seeded with
+    ``seed`` (so every device sees the same tubes).  Returns (lengths, left,
+    right, Bn) as CPU tensors."""
+    g = torch.Generator().manual_seed(seed)
+
+    def u(lo, hi):
+        return lo + (hi - lo) * torch.rand((n_tubes,), generator=g, dtype=torch.float64)
+
+    lengths = torch.randint(lmin, lmax + 1, (n_tubes,), generator=g)
+    sides = []
+    for _ in range(2):
+        rho = 10.0 ** u(-1.0, 1.0)
+        p = 10.0 ** u(-2.0, 1.0)
+        ct = u(-1.0, 1.0)
+        ph = u(0.0, 2.0 * math.pi)
+        sp = u(0.0, 0.9)
+        st = torch.sqrt(1.0 - ct * ct)
+        sides.append({"rho": rho, "p": p, "vx": sp * st * torch.cos(ph),
+                      "vy": sp * st * torch.sin(ph), "vz": sp * ct,
+                      "By": u(-2.0, 2.0), "Bz": u(-2.0, 2.0)})
+    return lengths, sides[0], sides[1], u(-1.0, 1.0)
+
+
+def _check_lib_rc(rc, what):
+    _lib.check(rc, what)
+
+
+class TubeBatch:
+    """Ragged tubes resident on the device.
+
+    ``lengths``: physical cells per tube; each tube spans x in [0, 1] with dx
+    = 1 / L and its interface at 0.5; ``left`` / ``right``: dicts of per-tube
+    values of rho, p, vx, vy, vz, By, Bz; ``Bn``: per-tube normal field.
+    """
+
+    def __init__(self, lengths, left, right, Bn, eos, *, limiter="mp5", solver="hlld", ng=None,
+                 device="cuda", atmo_rho=1e-10, order="longest_first"):
+        self.lib = _lib.lib()
+        self.device = torch.device(device)
+        self.eos = eos
+        self.ceos = as_c(eos)
+        self.solver = solver_name(solver)
+        if limiter not in _lib.LIMITER_IDS:
+            raise ValueError(f"unknown limiter {limiter!r}")
+        self.limiter = limiter
+        self.ng = ghosts_needed(limiter) if ng is None else int(ng)
+        self.atmo_rho = float(atmo_rho)
+        d = self.device
+        L = torch.as_tensor(lengths, dtype=torch.int64).cpu()
+        if L.ndim != 1 or L.numel() == 0 or int(L.min()) < 1:
+            raise ValueError("lengths must be a non-empty 1-D list of positive cell counts")
+        self.n_tubes = int(L.numel())
+        self.lengths = L
+        nt = L + 2 * self.ng
+        off = torch.zeros(self.n_tubes + 1, dtype=torch.int64)
+        off[1:] = torch.cumsum(nt, 0)
+        self.off_host = off
+        self.total_cells = int(off[-1])
+        self.off_c = off.to(d)
+        self.ncells = L.to(torch.int32).to(d)
+        self.dx = (1.0 / L.to(torch.float64)).to(d)
+        self.Bn = _lib.f64(Bn, d).reshape(-1)
+        if self.Bn.numel() != self.n_tubes:
+            raise ValueError("Bn must hold one value per tube")
+        if order == "longest_first":
+            self.order = torch.argsort(L, descending=True, stable=True).to(torch.int32).to(d)
+        elif order is None:
+            self.order = None
+        else:
+            self.order = torch.as_tensor(order, dtype=torch.int32).to(d)
+        self.layout = TubeLayout(self.off_c.data_ptr(), self.ncells.data_ptr(),
+                                 self.dx.data_ptr(), self.Bn.data_ptr(),
+                                 None if self.order is None else self.order.data_ptr(), 0.0)
+        nb = self.lib.srmhd_tubes_workspace_bytes(self.n_tubes, self.total_cells)
+        if nb < 0:
+            raise ValueError("layout rejected by srmhd_tubes_workspace_bytes")
+        self.ws = _lib.workspace(nb, d)
+        self.t = torch.zeros(self.n_tubes, dtype=torch.float64, device=d)
+        self.U = [torch.empty(self.total_cells, dtype=torch.float64, device=d) for _ in U_KEYS]
+        self.P = [torch.empty(self.total_cells, dtype=torch.float64, device=d) for _ in P_KEYS]
+        self._init_state(left, right)
+        self._uptr = _lib.ptrs(self.U)
+        self._pptr = _lib.ptrs(self.P)
+        self.block = 256
+
+    # ── initial data ──────────────────────────────────────────────────────
+    def _init_state(self, left, right):
+        """Cell-centred primitives of every tube (ghosts included) on x in
+        [0, 1] with the interface at 0.5, then the conserved state."""
+        d = self.device
+        tube = torch.repeat_interleave(torch.arange(self.n_tubes, device=d),
+                                       (self.lengths + 2 * self.ng).to(d))
+        local = torch.arange(self.total_cells, device=d) - self.off_c[tube]
+        x = (local.to(torch.float64) - self.ng + 0.5) * self.dx[tube]
+        m = x < 0.5
+        P = {}
+        for k in SIDE_KEYS:
+            lv = _lib.f64(left[k], d).reshape(-1)[tube]
+            rv = _lib.f64(right[k], d).reshape(-1)[tube]
+            P[k] = torch.where(m, lv, rv).contiguous()
+        P["Bx"] = self.Bn[tube].contiguous()
+        P["eps"] = self.eos.eps__press_rho(P["p"], P["rho"]).contiguous()
+        cons = [torch.empty(self.total_cells, dtype=torch.float64, device=d) for _ in range(8)]
+        pa, keep_p = _lib.ptrs([P[k] for k in ("rho", "vx", "vy", "vz", "p", "eps", "Bx", "By",
+                                                "Bz")])
+        ca, keep_c = _lib.ptrs(cons)
+        _check_lib_rc(self.lib.srmhd_p2c(self.total_cells, pa, ca, _lib.stream_ptr(d)),
+                      "srmhd_p2c")
+        # C order D, tau, Sx, Sy, Sz, Bx, By, Bz -> evolved D, Sx, Sy, Sz, tau, By, Bz
+        for dst, src in zip(self.U, (cons[0], cons[2], cons[3], cons[4], cons[1])):
+            dst.copy_(src)
+        self.U[5].copy_(P["By"])
+        self.U[6].copy_(P["Bz"])
+        for dst, k in zip(self.P, P_KEYS):
+            dst.copy_(P[k])
+
+    # ── stepping ──────────────────────────────────────────────────────────
+    def run(self, steps, *, cfl=0.4, scheme="rk3", block=None, stream=None):
+        """Advance every tube ``steps`` steps with its own dt in one launch."""
+        if scheme not in ("rk2", "rk3"):
+            raise ValueError("scheme must be 'rk2' or 'rk3'")
+        s = stream if stream is not None else _lib.stream_ptr(self.device)
+        rc = self.lib.srmhd_tubes_run(
+            self.n_tubes, self.total_cells, ctypes.byref(self.layout), ctypes.byref(self.ceos),
+            _lib.SOLVER_IDS[self.solver], _lib.LIMITER_IDS[self.limiter],
+            1 if scheme == "rk3" else 0, int(steps), float(cfl), self.atmo_rho,
+            int(block or self.block), self._uptr[0], self._pptr[0], self.t.data_ptr(),
+            self.ws.data_ptr(), self.ws.numel(), s)
+        _check_lib_rc(rc, "srmhd_tubes_run")
+
+    def capture(self, steps, *, cfl=0.4, scheme="rk3", block=None):
+        """A HIP graph of ``run(steps)`` on a side stream; replay with
+        ``graph.replay()``."""
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            g.capture_begin()
+            self.run(steps, cfl=cfl, scheme=scheme, block=block,
+                     stream=ctypes.c_void_p(side.cuda_stream))
+            g.capture_end()
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        return g
+
+    # ── read-back ─────────────────────────────────────────────────────────
+    def tube(self, i):
+        """Primitives of tube ``i`` (ghosts included) as the reference's dict:
+        rho, vx, vy, vz, p, eps, Bx, By, Bz."""
+        a, b = int(self.off_host[i]), int(self.off_host[i + 1])
+        out = {k: v[a:b] for k, v in zip(P_KEYS, self.P)}
+        out["Bx"] = self.Bn[i].expand(b - a)
+        out["By"] = self.U[5][a:b]
+        out["Bz"] = self.U[6][a:b]
+        return out
+
+    def state(self):
+        return {k: v for k, v in zip(U_KEYS, self.U)}
+
+    def snapshot(self):
+        return [u.clone() for u in self.U], [p.clone() for p in self.P], self.t.clone()
+
+    def restore(self, snap):
+        U, P, t = snap
+        for dst, src in zip(self.U, U):
+            dst.copy_(src)
+        for dst, src in zip(self.P, P):
+            dst.copy_(src)
+        self.t.copy_(t)
+
+
+# ── workgroup-size autotuner ───────────────────────────────────────────────
+
+_TUNE_CACHE = {}
+
+
+def _shape_key(batch, scheme):
+    L = batch.lengths
+    return (batch.n_tubes, batch.total_cells, int(L.max()), batch.limiter, batch.solver, scheme)
+
+
+def autotune_block(batch, *, steps=3, cfl=0.4, scheme="rk3", candidates=(64, 128, 256, 512),
+                   repeats=2):
+    """Time ``steps`` steps at each workgroup size on the batch itself (its
+    state is restored afterwards), keep the fastest, cache it per shape
+    (tube count, total cells, longest tube, limiter, solver, scheme).
+    Returns (best block, {block: ms})."""
+    key = _shape_key(batch, scheme)
+    if key in _TUNE_CACHE:
+        batch.block = _TUNE_CACHE[key][0]
+        return _TUNE_CACHE[key]
+    snap = batch.snapshot()
+    times = {}
+    for b in candidates:
+        best = float("inf")
+        for _ in range(repeats):
+            batch.restore(snap)
+            torch.cuda.synchronize(batch.device)
+            t0 = time.perf_counter()
+            batch.run(steps, cfl=cfl, scheme=scheme, block=b)
+            torch.cuda.synchronize(batch.device)
+            best = min(best, time.perf_counter() - t0)
+        times[b] = 1e3 * best
+    batch.restore(snap)
+    torch.cuda.synchronize(batch.device)
+    best_b = min(times, key=times.get)
+    _TUNE_CACHE[key] = (best_b, times)
+    batch.block = best_b
+    return best_b, times
+
+
+# ── the reference's run_tubes (shared dt) ──────────────────────────────────
+
+def _prims_from(UL, UR, Bn, eos, ncells, ng, device):
+    """tube_seed.py:44-69: two constant states split at the middle of every
+    tube, the solver-frame TOTAL pressure turned into gas pressure (that
+    per-tube algebra in numpy float64, as the reference does it), then the
+    columns on ``device`` and the two-cell-copy ghosts (tube_seed.py:72-75).
+    Returns the 9 primitives as (N, nt) tube-major tensors."""
+    d = device
+    UL = np.asarray(UL, dtype=np.float64)
+    UR = np.asarray(UR, dtype=np.float64)
+    Bn = np.asarray(Bn, dtype=np.float64).reshape(-1)
+    n = Bn.shape[0]
+    nt = ncells + 2 * ng
+    half = nt // 2
+    pg = []
+    for U in (UL, UR):
+        v2 = U[:, 2] ** 2 + U[:, 3] ** 2 + U[:, 4] ** 2
+        W2 = 1.0 / np.maximum(1.0 - v2, 1e-300)
+        eta = Bn * U[:, 2] + U[:, 5] * U[:, 3] + U[:, 6] * U[:, 4]
+        b2 = (Bn ** 2 + U[:, 5] ** 2 + U[:, 6] ** 2) / W2 + eta ** 2
+        pg.append(U[:, 1] - 0.5 * b2)
+    cols = {"rho": (UL[:, 0], UR[:, 0]), "p": (pg[0], pg[1]), "vx": (UL[:, 2], UR[:, 2]),
+            "vy": (UL[:, 3], UR[:, 3]), "vz": (UL[:, 4], UR[:, 4]), "By": (UL[:, 5], UR[:, 5]),
+            "Bz": (UL[:, 6], UR[:, 6])}
+    P = {}
+    for k, (a, b) in cols.items():
+        col = torch.empty((n, nt), dtype=torch.float64, device=d)
+        col[:, :half] = torch.as_tensor(a, device=d)[:, None]
+        col[:, half:] = torch.as_tensor(b, device=d)[:, None]
+        P[k] = col
+    P["Bx"] = torch.as_tensor(Bn, device=d)[:, None].expand(n, nt).contiguous()
+    P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
+    for v in P.values():
+        v[:, :ng] = v[:, ng:ng + 1].clone()
+        v[:, -ng:] = v[:, -ng - 1:-ng].clone()
+    return P
+
+
+def run_tubes(UL, UR, Bn, eos, *, ncells=192, tend=0.18, cfl=0.3, limiter="mc", ng=None,
+              max_steps=400, flux="hlld", device="cuda", block=256, graph=True):
+    """Evolve N Riemann problems to ``tend`` (tube_seed.py:93-144).
+
+    ``UL``/``UR`` are (N, 7) solver-frame states (rho, P_tot, vx, vy, vz, By,
+    Bz), ``Bn`` is (N,).  Returns the primitive profiles, each an
+    (ncells, N) tensor with the ghosts stripped, and t.  The loop runs on the
+    device: with ``graph`` the ``max_steps`` iterations are one HIP graph."""
+    lib = _lib.lib()
+    d = torch.device(device)
+    if ng is None:
+        ng = ghosts_needed(limiter)
+    dx = 1.0 / ncells
+    P = _prims_from(UL, UR, Bn, eos, ncells, ng, d)
+    n = P["rho"].shape[0]
+    nt = ncells + 2 * ng
+    total = n * nt
+    flat = {k: v.reshape(-1).contiguous() for k, v in P.items()}
+    cons = [torch.empty(total, dtype=torch.float64, device=d) for _ in range(8)]
+    pa, keep_p = _lib.ptrs([flat[k] for k in ("rho", "vx", "vy", "vz", "p", "eps", "Bx", "By",
+                                               "Bz")])
+    ca, keep_c = _lib.ptrs(cons)
+    _check_lib_rc(lib.srmhd_p2c(total, pa, ca, _lib.stream_ptr(d)), "srmhd_p2c")
+    U = [cons[0].clone(), cons[2].clone(), cons[3].clone(), cons[4].clone(), cons[1].clone(),
+         flat["By"].clone(), flat["Bz"].clone()]
+    Pv = [flat[k].clone() for k in P_KEYS]
+    off = (torch.arange(n + 1, dtype=torch.int64) * nt).to(d)
+    ncl = torch.full((n,), ncells, dtype=torch.int32, device=d)
+    dxs = torch.full((n,), dx, dtype=torch.float64, device=d)
+    Bn_d = flat["Bx"].reshape(n, nt)[:, 0].contiguous()
+    lay = TubeLayout(off.data_ptr(), ncl.data_ptr(), dxs.data_ptr(), Bn_d.data_ptr(), None,
+                     float(dx))
+    nb = lib.srmhd_tubes_workspace_bytes(n, total)
+    ws = _lib.workspace(nb, d)
+    cb = lib.srmhd_tubes_clock_bytes(nt)
+    if cb < 0:
+        raise ValueError("srmhd_tubes_clock_bytes rejected the tube length")
+    clock = torch.zeros((cb + 7) // 8, dtype=torch.float64, device=d)
+    ua, keep_u = _lib.ptrs(U)
+    pv, keep_pv = _lib.ptrs(Pv)
+    ceos = as_c(eos)
+    sid, lid = _lib.SOLVER_IDS[solver_name(flux)], _lib.LIMITER_IDS[limiter]
+
+    def one(stream):
+        rc = lib.srmhd_tubes_shared_step(n, total, ctypes.byref(lay), ctypes.byref(ceos), sid,
+                                         lid, float(cfl), float(tend), 1e-10, int(block), ua,
+                                         pv, clock.data_ptr(), ws.data_ptr(), ws.numel(), stream)
+        _check_lib_rc(rc, "srmhd_tubes_shared_step")
+
+    if graph and max_steps > 0:
+        g = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(d)
+        side.wait_stream(torch.cuda.current_stream(d))
+        with torch.cuda.stream(side):
+            g.capture_begin()
+            sp = ctypes.c_void_p(side.cuda_stream)
+            for _ in range(max_steps):
+                one(sp)
+            g.capture_end()
+        torch.cuda.current_stream(d).wait_stream(side)
+        g.replay()
+    else:
+        sp = _lib.stream_ptr(d)
+        for _ in range(max_steps):
+            one(sp)
+    torch.cuda.synchronize(d)
+    t = float(clock[0].item())
+    if int(clock[9:10].view(torch.int64).item()) != 0:
+        raise IndexError("con2prim redo: a tube index past the last row of the (rows, N) batch "
+                         "(the reference raises IndexError here, c2p.py:339-341)")
+    ph = slice(ng, ng + ncells)
+    out = {k: v.reshape(n, nt)[:, ph].T for k, v in zip(P_KEYS, Pv)}
+    out["Bx"] = Bn_d[None, :].expand(ncells, n)
+    out["By"] = U[5].reshape(n, nt)[:, ph].T
+    out["Bz"] = U[6].reshape(n, nt)[:, ph].T
+    return out, t

@@ -160,6 +160,65 @@ int srmhd1d_rk2_step(const srmhd_grid1d* g, int solver, int limiter, const srmhd
 int srmhd1d_dt(const srmhd_grid1d* g, const srmhd_eos* eos, double cfl, int speed_of_light,
                const double* const* prim, double* dt_dev, void* stream);
 
+/* ── batched 1D Riemann tubes (src/physics/tube_seed.py:72-144) ─────────────
+ * n_tubes tubes of possibly different lengths in one ragged layout.  Tube t
+ * owns cells [off_c[t], off_c[t+1]) = ncells[t] physical cells plus ng ghosts
+ * per side (ng = off_c[t+1] - off_c[t] - ncells[t] halves), and faces
+ * [off_c[t] + t, off_c[t+1] + t + 1).  All arrays below are device memory.
+ * U: 7 conserved pointers D, Sx, Sy, Sz, tau, By, Bz (B_n is constant per
+ * tube); P: 6 primitive pointers rho, vx, vy, vz, p, eps; both over
+ * total_cells = off_c[n_tubes] and updated in place.  order (may be NULL):
+ * workgroup -> tube, e.g. longest first.  dx_shared: the one dx of the
+ * shared-dt form (host value). */
+typedef struct srmhd_tube_layout {
+  const int64_t* off_c;   /* n_tubes + 1 */
+  const int32_t* ncells;  /* n_tubes */
+  const double* dx;       /* n_tubes */
+  const double* Bn;       /* n_tubes */
+  const int32_t* order;   /* n_tubes or NULL */
+  double dx_shared;
+} srmhd_tube_layout;
+
+int64_t srmhd_tubes_workspace_bytes(int n_tubes, int64_t total_cells);
+
+/* `steps` steps of every tube in ONE launch, each tube with its own dt:
+ * compute_dt of the 1D driver (src/physics/driver.py:77-127) over its
+ * physical cells; SSP-RK2 (rk3 = 0) or SSP-RK3 (rk3 = 1) with the weights of
+ * src/physics/driver2d.py:104-105 in combine form a U0 + b U + (c dt) L;
+ * each stage as tube_seed.py:81-109: reconstruct_prims along the tube with
+ * L[Bx] = R[Bx] = B_n, the flux on every face of the tube (HLLD's HLL-state
+ * con2prim batch = the tube's faces), L = -(F[j+1] - F[j]) / dx, con2prim of
+ * every cell of the tube as one batch, two-cell-copy ghosts (which also reach
+ * U's By, Bz, tube_seed.py:72-75,105-108).  t_out[t] += the tube's elapsed
+ * time.  block: threads per workgroup, 64, 128, 256 or 512. */
+int srmhd_tubes_run(int n_tubes, int64_t total_cells, const srmhd_tube_layout* layout,
+                    const srmhd_eos* eos, int solver, int limiter, int rk3, int steps, double cfl,
+                    double atmo_rho, int block, double* const* U, double* const* P, double* t_out,
+                    void* work, int64_t work_bytes, void* stream);
+
+/* The shared-dt form of run_tubes (tube_seed.py:93-144): n_tubes tubes of
+ * equal length (rows = total_cells / n_tubes cells each, ghosts included),
+ * one dt and one con2prim batch over every tube.  clock: device memory of
+ * srmhd_tubes_clock_bytes(rows) bytes, zeroed by the caller before the first
+ * step; it starts with doubles {t, steps, dt, active} and 64-bit words
+ * {mask x4, smax, index_errors}.  Each call is one iteration of the
+ * reference's loop: while t < tend - 1e-14, dt = min(cfl dx / max(smax,
+ * 1e-8), tend - t), smax the max over every cell of every tube of |vx| +
+ * min(c_s + 0.5, 1) (ideal-gas c_s from eos->gamma), then one Heun step (U1 =
+ * U + dt L, U' = 0.5 (U + U1 + dt L)) with the stage algebra of
+ * srmhd_tubes_run; after tend a call does nothing, so max_steps calls can be
+ * captured in one graph.  The reference hands its (rows, N) arrays to
+ * conservative_to_primitive unflattened, whose bisection redo then re-solves
+ * every row named by the row OR the column index of an element that needs it
+ * (c2p.py:339-346); that set is reproduced.  A column index past the last row
+ * (an IndexError in the reference) increments index_errors.
+ * layout->dx_shared is the dx of every tube. */
+int64_t srmhd_tubes_clock_bytes(int64_t rows);
+int srmhd_tubes_shared_step(int n_tubes, int64_t total_cells, const srmhd_tube_layout* layout,
+                            const srmhd_eos* eos, int solver, int limiter, double cfl, double tend,
+                            double atmo_rho, int block, double* const* U, double* const* P,
+                            void* clock, void* work, int64_t work_bytes, void* stream);
+
 /* ── live kernel timing (measurement only; no counterpart in the reference) ──
  * srmhd_timing_enable(cls, n): from now on srmhd2d_stage records a HIP event
  * pair around each launch of kernel class cls, on its stream, for up to n
