@@ -79,10 +79,102 @@ def cpu_baseline(n_sample=256, limiter="weno5z"):
                        f"({wall:.1f} s)")}
 
 
+def cpu_baseline_tubes(lengths, left, right, Bn, steps=2, limiter="mp5"):
+    """The oracle's per-tube loop (torch CPU float64) on the first tube of
+    the same workload for a few steps, timed."""
+    import torch
+    import oracle.eos as OE
+    import oracle.tubes as OT
+    threads = torch.get_num_threads()
+    eos = OE.ideal(5.0 / 3.0)
+    L = int(lengths[0])
+    side = lambda s: {k: float(v[0]) for k, v in s.items()}  # noqa: E731
+    P0 = OT.tube_initial(L, 3, eos, side(left), side(right), float(Bn[0]))
+    t0 = time.perf_counter()
+    OT.run_tube(P0, eos, L, 3, steps, cfl=0.4, limiter=limiter, scheme="rk3", flux="hlld")
+    wall = time.perf_counter() - t0
+    return {"value": L * steps / wall, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+            "sample": (f"CPU oracle (torch {torch.__version__} float64, {threads} threads): "
+                       f"{steps} MP5/HLLD/RK3 steps of tube 0 ({L} cells), the per-tube loop "
+                       f"form ({wall:.1f} s)")}
+
+
+def bench_tubes(a):
+    """4096 ragged Riemann tubes, MP5 + HLLD + SSP-RK3, per-tube dt, all
+    steps in one persistent graph-captured launch."""
+    import torch
+    from srmhd import eos as E, tubes as T
+    dev = torch.device("cuda:0")
+    eos = E.hybrid_eos(0.0, 5.0 / 3.0, 5.0 / 3.0)
+    lengths, left, right, Bn = T.draw_tubes(a.tubes, seed=0, lmin=128, lmax=2048)
+    batch = T.TubeBatch(lengths, left, right, Bn, eos, limiter=a.limiter, solver=a.solver,
+                        device=dev)
+    zones = int(lengths.sum())
+    best, tune = T.autotune_block(batch, steps=2, cfl=a.cfl)
+    print(f"autotune: block {best} ({tune})", file=sys.stderr, flush=True)
+    snap = batch.snapshot()
+    if a.warmup:
+        batch.run(a.warmup, cfl=a.cfl)
+        torch.cuda.synchronize()
+        batch.restore(snap)
+    graph = batch.capture(a.steps, cfl=a.cfl)
+    torch.cuda.synchronize()
+    print(f"warmup done ({a.warmup} steps), graph captured", file=sys.stderr, flush=True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    graph.replay()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1)
+    zc = zones * a.steps / wall
+    finite = bool(all(torch.isfinite(p).all().item() for p in batch.P))
+    # the per-tube loop on the GPU: the first tubes one launch each, same steps
+    n_loop = min(a.loop_tubes, a.tubes)
+    sub = [T.TubeBatch(lengths[i:i + 1], {k: v[i:i + 1] for k, v in left.items()},
+                       {k: v[i:i + 1] for k, v in right.items()}, Bn[i:i + 1], eos,
+                       limiter=a.limiter, solver=a.solver, device=dev) for i in range(n_loop)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b in sub:
+        b.run(a.steps, cfl=a.cfl, block=best)
+        torch.cuda.synchronize()
+    loop_wall = time.perf_counter() - t0
+    loop_zc = int(lengths[:n_loop].sum()) * a.steps / loop_wall
+    out = {
+        "metric": "zone-cycles/s, batched 1D Riemann tubes (4096 ragged tubes, MP5/HLLD/RK3)",
+        "value": zc, "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{a.tubes} Riemann tubes, lengths U[128, 2048] (seed 0, "
+                               f"{zones} cells), {a.limiter.upper()} + {a.solver.upper()} + "
+                               f"SSP-RK3, per-tube dt, CFL {a.cfl}, float64, one graph-captured "
+                               "persistent launch",
+                   "tubes": a.tubes, "cells": zones, "block": best},
+        "kernel_ms": kernel_ms,
+        "per_tube_loop": {"value": loop_zc, "unit": "zone-cycles/s", "tubes_timed": n_loop,
+                          "speedup_of_batch": zc / loop_zc},
+        "autotune_ms": tune,
+        "state_finite": finite,
+    }
+    if not a.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline_tubes(lengths, left, right, Bn, limiter=a.limiter)
+        except Exception as exc:  # the baseline never hides the GPU result
+            out["cpu_baseline"] = {"error": repr(exc)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--workload", default="rotor", choices=("rotor", "tubes"),
+                    help="rotor: the 2048^2 rotor (headline); tubes: 4096 batched tubes")
+    ap.add_argument("--tubes", type=int, default=4096)
+    ap.add_argument("--loop-tubes", type=int, default=16,
+                    help="tubes timed one launch each for the per-tube-loop comparison")
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--limiter", default="weno5z")
@@ -100,6 +192,15 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the domain-decomposed design is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "tubes":
+        if a.steps is None:
+            a.steps = 500
+        if a.limiter == "weno5z":
+            a.limiter = "mp5"
+        bench_tubes(a)
+        return
+    if a.steps is None:
+        a.steps = 10
 
     import ctypes
     import torch

@@ -4,95 +4,14 @@
 // Reductions are wave-level (shuffles, ballots) followed by one atomic per
 // wave on 64-bit words; every reduced quantity is an integer, a bit mask or a
 // max, so results are deterministic.
-#include "srmhd_launch.h"
+#include "srmhd_grid2d.h"
+#include "srmhd_mixed.h"
 #include "srmhd_recon.h"
 #include "srmhd_riemann.h"
 
 using namespace srmhd;
 
 namespace {
-
-// Block-level combination of per-wave partials: lane 0 of every wave parks
-// its values in LDS, thread 0 folds them and issues one atomic per value.
-// Every thread of the block must call these (no early return before).
-__device__ inline void block_or_atomic(unsigned long long* dst, unsigned long long v) {
-  __shared__ unsigned long long sh[BLK / 64];
-  v = wave_or(v);
-  if (lane0()) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-#pragma unroll
-    for (int w = 0; w < BLK / 64; ++w) t |= sh[w];
-    if (t) atomicOr(dst, t);
-  }
-}
-
-__device__ inline void block_max_atomic2(unsigned long long* dst0, unsigned long long v0,
-                                         unsigned long long* dst1, unsigned long long v1) {
-  __shared__ unsigned long long sh[2][BLK / 64];
-  v0 = wave_max(v0);
-  v1 = wave_max(v1);
-  if (lane0()) {
-    sh[0][threadIdx.x >> 6] = v0;
-    sh[1][threadIdx.x >> 6] = v1;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long a = 0, b = 0;
-#pragma unroll
-    for (int w = 0; w < BLK / 64; ++w) {
-      a = sh[0][w] > a ? sh[0][w] : a;
-      b = sh[1][w] > b ? sh[1][w] : b;
-    }
-    if (a) atomicMax(dst0, a);
-    if (b) atomicMax(dst1, b);
-  }
-}
-
-// Per-face counters of one flux call (hlld.py:1516-1529) plus the max |F_D|
-// of the sweep, block-aggregated.
-__device__ void diag_accumulate(srmhd_flux_diag* d, bool active, int st, int iters,
-                                unsigned long long absFD_bits) {
-  constexpr int NS = 8;
-  unsigned long long v[NS + 2];
-  v[0] = __popcll(__ballot(active));
-  v[1] = __popcll(__ballot(active && (st & FS_NOT_CONVERGED)));
-  v[2] = __popcll(__ballot(active && (st & FS_WAVE_ORDER)));
-  v[3] = __popcll(__ballot(active && (st & FS_BN_ZERO)));
-  v[4] = __popcll(__ballot(active && (st & FS_FALLBACK)));
-  const bool conv = active && !(st & FS_NOT_CONVERGED);
-  v[5] = __popcll(__ballot(conv));
-  v[6] = wave_sum(conv ? (unsigned long long)iters : 0ull);
-  v[7] = __popcll(__ballot(active && (st & FS_C2P_REDONE)));
-  v[NS] = wave_max(active ? (unsigned long long)iters : 0ull);
-  v[NS + 1] = wave_max(active ? absFD_bits : 0ull);
-  __shared__ unsigned long long sh[NS + 2][BLK / 64];
-  if (lane0()) {
-#pragma unroll
-    for (int k = 0; k < NS + 2; ++k) sh[k][threadIdx.x >> 6] = v[k];
-  }
-  __syncthreads();
-  if (threadIdx.x < NS + 2) {
-    const int k = threadIdx.x;
-    unsigned long long t = 0;
-    for (int w = 0; w < BLK / 64; ++w) t = (k < NS) ? t + sh[k][w] : (sh[k][w] > t ? sh[k][w] : t);
-    if (t) {
-      switch (k) {
-        case 0: atomicAdd(&d->n_interfaces, t); break;
-        case 1: atomicAdd(&d->n_not_converged, t); break;
-        case 2: atomicAdd(&d->n_wave_order_bad, t); break;
-        case 3: atomicAdd(&d->n_bn_zero, t); break;
-        case 4: atomicAdd(&d->n_hlle_fallback, t); break;
-        case 5: atomicAdd(&d->n_converged, t); break;
-        case 6: atomicAdd(&d->sum_iters_converged, t); break;
-        case 7: atomicAdd(&d->n_c2p_redo, t); break;
-        case 8: atomicMax(&d->max_iters, t); break;
-        default: atomicMax(&d->absF_D_max, t); break;
-      }
-    }
-  }
-}
 
 struct PrimPtrs {
   const double* p[9];
@@ -241,57 +160,6 @@ __global__ __launch_bounds__(BLK) void k_p2c(int64_t n, PrimPtrs P, ConsPtrs C) 
   for (int k = 0; k < 8; ++k) C.p[k][i] = c[k];
 }
 
-// ── 2D ──────────────────────────────────────────────────────────────────────
-
-struct G2 {
-  int nx, ny, ng, nxt, nyt, bcx, bcy;
-  double dx, dy;
-};
-
-struct St2 {  // D, Sx, Sy, Sz, tau, Bz, Bxf, Byf
-  const double* p[8];
-};
-struct St2Out {
-  double* p[8];
-};
-struct Pr2 {  // rho, vx, vy, vz, p, eps
-  const double* p[6];
-};
-struct Pr2Out {
-  double* p[6];
-};
-struct Flx {  // D, Sx, Sy, Sz, tau, Bz, E-component
-  double* p[7];
-};
-
-// BC source index of a cell-centred axis (grid.py:93-113)
-__device__ inline int src_cell(int i, int n, int ng, int bc) {
-  if (i < ng) return bc == SRMHD_BC_PERIODIC ? i + n : ng;
-  if (i >= ng + n) return bc == SRMHD_BC_PERIODIC ? i - n : ng + n - 1;
-  return i;
-}
-// BC source index of a face-staggered axis along its normal (grid.py:116-143)
-__device__ inline int src_face(int f, int n, int ng, int bc) {
-  if (bc == SRMHD_BC_PERIODIC) {
-    if (f < ng) return f + n;
-    if (f == ng + n) return ng;
-    if (f > ng + n) return f - n;
-    return f;
-  }
-  if (f < ng) return ng;
-  if (f > ng + n) return ng + n;
-  return f;
-}
-
-
-// cell-centred B from the staggered field (ct.py:44-54)
-__device__ inline double bcx(const G2& g, const double* Bxf, int i, int j) {
-  return 0.5 * (Bxf[(int64_t)i * g.nyt + j] + Bxf[(int64_t)(i + 1) * g.nyt + j]);
-}
-__device__ inline double bcy(const G2& g, const double* Byf, int i, int j) {
-  return 0.5 * (Byf[(int64_t)i * (g.nyt + 1) + j] + Byf[(int64_t)i * (g.nyt + 1) + j + 1]);
-}
-
 // Reconstructed face states of face (i, j) of the sweep along N, as
 // reconstruct_prims (reconstruction.py:272-338, vel_var "Wv") followed by
 // B_n := face value (sweep.py:82-89).
@@ -388,18 +256,6 @@ __device__ void face_prims2d(const G2& g, const St2& S, const Pr2& P, const srmh
   }
 }
 
-template <int N>
-__device__ inline void face_ij(const G2& g, int64_t t, int& i, int& j) {
-  const int ny = N == 0 ? g.nyt : g.nyt + 1;
-  i = (int)(t / ny);
-  j = (int)(t - (int64_t)i * ny);
-}
-
-template <int N>
-__device__ inline int64_t n_faces(const G2& g) {
-  return N == 0 ? (int64_t)(g.nxt + 1) * g.nyt : (int64_t)g.nxt * (g.nyt + 1);
-}
-
 template <int N, int LIM>
 __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e, C2PRec* rec,
                                                  srmhd_flux_diag* d) {
@@ -457,6 +313,40 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
   }
   // the deadband scale of the upwind EMF is needed for every solver
   diag_accumulate(d, active, st, iters, dbits(fabs(fD)));
+}
+
+// FP64 recomputation of the faces the mixed-precision sweep flagged (its
+// list: fix[0] = count, indices from fix[2]): the FP64 face states and HLLD,
+// with the HLL-state con2prim of the face as its own batch.
+template <int N, int LIM>
+__global__ __launch_bounds__(BLK) void k_fixup64(G2 g, St2 S, Pr2 P, srmhd_eos e,
+                                                 const unsigned* fix, Flx F) {
+  const unsigned cnt = fix[0];
+  for (unsigned q = blockIdx.x * BLK + threadIdx.x; q < cnt; q += gridDim.x * BLK) {
+    const int64_t t = fix[2 + q];
+    int i, j;
+    face_ij<N>(g, t, i, j);
+    Prim l, r;
+    face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
+    HlldPrep<N> h;
+    h.run(l, r, e);
+    Kastaun k;
+    k.init(h.uH, e);
+    C2PRec rec;
+    const uint64_t m = c2p_pass_a(k, rec);
+    Prim ph;
+    c2p_pass_b(k, rec, c2p_break_iter(m), 1e-10, 1e-10, ph);
+    double f[8], ps;
+    int it;
+    hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, nullptr, ps, it);
+    F.p[0][t] = f[CD];
+    F.p[1][t] = f[CS];
+    F.p[2][t] = f[CS + 1];
+    F.p[3][t] = f[CS + 2];
+    F.p[4][t] = f[CT];
+    F.p[5][t] = f[CB + 2];
+    F.p[6][t] = N == 0 ? f[CB + 1] : f[CB + 0];
+  }
 }
 
 // Gardiner-Stone corner EMF (ct.py:119-177) with the ring of zeros
@@ -809,6 +699,43 @@ void sweeps_solver(int sol, const G2& g, const St2& S, const Pr2& P, const srmhd
   }
 }
 
+// the mixed-precision sweep along dir and the FP64 fix-up of its flagged faces
+template <int N, int LIM>
+void fixup_lim(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const unsigned* fix,
+               const Flx& F, hipStream_t s) {
+  k_fixup64<N, LIM><<<64, BLK, 0, s>>>(g, S, P, e, fix, F);
+}
+
+template <int N>
+void fixup_dispatch(int lim, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
+                    const unsigned* fix, const Flx& F, hipStream_t s) {
+  switch (lim) {
+    case LIM_PCM: fixup_lim<N, LIM_PCM>(g, S, P, e, fix, F, s); break;
+    case LIM_MC: fixup_lim<N, LIM_MC>(g, S, P, e, fix, F, s); break;
+    case LIM_MINMOD: fixup_lim<N, LIM_MINMOD>(g, S, P, e, fix, F, s); break;
+    case LIM_WENO5Z: fixup_lim<N, LIM_WENO5Z>(g, S, P, e, fix, F, s); break;
+    case LIM_MP5: fixup_lim<N, LIM_MP5>(g, S, P, e, fix, F, s); break;
+    default: fixup_lim<N, LIM_MP7>(g, S, P, e, fix, F, s);
+  }
+}
+
+void sweeps_mixed(int lim, int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
+                  const Ws2& w, hipStream_t s) {
+  // the C2PRec areas are unused in this form: they hold the fix-up lists
+  unsigned* fx = (unsigned*)w.recx;
+  unsigned* fy = (unsigned*)w.recy;
+  k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)fx, 1);
+  k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)fy, 1);
+  tbegin(SRMHD_K_SWEEP_X_B, s);
+  sweep_mixed(0, lim, sol, g, S, P, e, w.diag + 0, w.Fx, fx, s);
+  tend(SRMHD_K_SWEEP_X_B, s);
+  if (sol == SOLVER_HLLD) fixup_dispatch<0>(lim, g, S, P, e, fx, w.Fx, s);
+  tbegin(SRMHD_K_SWEEP_Y_B, s);
+  sweep_mixed(1, lim, sol, g, S, P, e, w.diag + 1, w.Fy, fy, s);
+  tend(SRMHD_K_SWEEP_Y_B, s);
+  if (sol == SOLVER_HLLD) fixup_dispatch<1>(lim, g, S, P, e, fy, w.Fy, s);
+}
+
 void sweeps_dispatch(int lim, int sol, const G2& g, const St2& S, const Pr2& P,
                      const srmhd_eos& e, const Ws2& w, hipStream_t s) {
   switch (lim) {
@@ -1142,7 +1069,8 @@ int64_t srmhd2d_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s) 
 }
 
 static bool scheme_ok(const srmhd_scheme2d* s) {
-  return s && s->solver >= 0 && s->solver <= 2 && s->limiter >= 0 && s->limiter <= 5;
+  return s && s->solver >= 0 && s->solver <= 2 && s->limiter >= 0 && s->limiter <= 5 &&
+         (s->precision == SRMHD_PREC_FP64 || s->precision == SRMHD_PREC_MIXED);
 }
 
 int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_eos* eos,
@@ -1178,7 +1106,11 @@ int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_
   hipStream_t st = (hipStream_t)stream;
   k_zero_u64<<<1, 64, 0, st>>>((unsigned long long*)w.diag, N_WORDS);
   k_zero_u64<<<1, 64, 0, st>>>(w.words, 4);
-  sweeps_dispatch(sp->limiter, sp->solver, g, s1, pr, *eos, w, st);
+  if (sp->precision == SRMHD_PREC_MIXED) {
+    sweeps_mixed(sp->limiter, sp->solver, g, s1, pr, *eos, w, st);
+  } else {
+    sweeps_dispatch(sp->limiter, sp->solver, g, s1, pr, *eos, w, st);
+  }
   const int64_t ncor = (int64_t)(g.nxt + 1) * (g.nyt + 1);
   tbegin(SRMHD_K_CORNER, st);
   k_corner<<<blocks_for(ncor), BLK, 0, st>>>(g, s1, pr, w.Fx.p[0], w.Fx.p[6], w.Fy.p[0],

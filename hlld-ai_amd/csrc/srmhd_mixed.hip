@@ -1,0 +1,185 @@
+// Mixed-precision sweeps of the 2D stage: reconstruction and the Riemann
+// solver in FP32, everything else of the stage (corner EMF, CT face-B
+// update, RK accumulation, con2prim) in the FP64 kernels.
+//
+// One kernel per sweep (no pass A: the FP32 HLL-state con2prim is per
+// element).  It reads the FP64 state, forms the reconstructed variables
+// (rho, p, W v with W from FP64, B) in FP64, reconstructs their deviations
+// from the centre cell in FP32, and writes the same seven FP64 flux arrays as
+// the FP64 sweeps, so the downstream kernels are shared.  Built with FP32 contraction and
+// approximate FP32 division / sqrt (this translation unit only).
+#include "srmhd_mixed.h"
+
+#include "srmhd_f32.h"
+#include "srmhd_grid2d.h"
+#include "srmhd_recon.h"
+
+namespace srmhd {
+namespace {
+
+// face_prims2d in FP32 (reconstruction.py:272-338 with vel_var "Wv", then
+// B_n := face value, sweep.py:82-89).  Every limiter is translation
+// equivariant, so each variable is reconstructed in FP32 as the deviation
+// from the stencil's centre cell (formed in FP64) and the centre value is
+// added back in FP64: the FP32 arithmetic then only touches the small
+// differences, and the face value carries one FP32 rounding.
+template <int N, int LIM>
+__device__ void face_prims_f32(const G2& g, const St2& S, const Pr2& P, const f32::Eos& e, int i,
+                               int j, f32::Prim& L, f32::Prim& R) {
+  constexpr int w = stencil_halfwidth(LIM);
+  constexpr int k0 = 3 - w, k1 = 4 + w;
+  const int n = N == 0 ? g.nxt : g.nyt;
+  const int f = N == 0 ? i : j;
+  auto cidx = [&](int k) -> int64_t {
+    const int c = clampi(f + k - 4, 0, n - 1);
+    return N == 0 ? (int64_t)c * g.nyt + j : (int64_t)i * g.nyt + c;
+  };
+  double a[8];
+  float q[8];
+  double ql[7], qr[7];   // rho, p, zx, zy, zz, B_t1, B_t2
+  // deviation reconstruction of a[k0..k1] about a[3]
+  auto recon = [&](double& l, double& r) {
+    const double base = a[3];
+#pragma unroll
+    for (int k = k0; k <= k1; ++k) q[k] = (float)(a[k] - base);
+    float fl, fr;
+    f32::face_states<LIM>(q, f, n, fl, fr);
+    l = base + (double)fl;
+    r = base + (double)fr;
+  };
+#pragma unroll
+  for (int v = 0; v < 2; ++v) {
+    const double* src = P.p[v == 0 ? 0 : 4];
+#pragma unroll
+    for (int k = k0; k <= k1; ++k) a[k] = src[cidx(k)];
+    recon(ql[v], qr[v]);
+  }
+  const bool bad = (ql[0] <= 1e-12) || (qr[0] <= 1e-12) || (ql[1] <= 1e-14) || (qr[1] <= 1e-14);
+  if (bad) {
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const double* src = P.p[v == 0 ? 0 : 4];
+      ql[v] = src[cidx(f == 0 ? 4 : 3)];
+      qr[v] = src[cidx(f == n ? 3 : 4)];
+    }
+  }
+  double W[8];
+#pragma unroll
+  for (int k = k0; k <= k1; ++k) {
+    const int64_t c = cidx(k);
+    const double vx = P.p[1][c], vy = P.p[2][c], vz = P.p[3][c];
+    W[k] = 1.0 / __builtin_sqrt(fmax(1.0 - (vx * vx + vy * vy + vz * vz), 1e-14));
+  }
+#pragma unroll
+  for (int v = 2; v < 7; ++v) {
+#pragma unroll
+    for (int k = k0; k <= k1; ++k) {
+      const int64_t c = cidx(k);
+      if (v <= 4) {
+        a[k] = W[k] * P.p[v - 1][c];
+      } else {
+        const bool fromBz = (N == 0) ? (v == 6) : (v == 5);
+        const int ci = N == 0 ? clampi(f + k - 4, 0, n - 1) : i;
+        const int cj = N == 0 ? j : clampi(f + k - 4, 0, n - 1);
+        a[k] = fromBz ? S.p[5][c] : ((N == 0) ? bcy(g, S.p[7], ci, cj) : bcx(g, S.p[6], ci, cj));
+      }
+    }
+    if (bad) {
+      ql[v] = f == 0 ? a[4] : a[3];
+      qr[v] = f == n ? a[3] : a[4];
+    } else {
+      recon(ql[v], qr[v]);
+    }
+  }
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+  const float Bn = (float)(N == 0 ? S.p[6][(int64_t)i * g.nyt + j]
+                                  : S.p[7][(int64_t)i * (g.nyt + 1) + j]);
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const double* x = s ? qr : ql;
+    f32::Prim& Q = s ? R : L;
+    const double fac = 1.0 / __builtin_sqrt(1.0 + (x[2] * x[2] + x[3] * x[3] + x[4] * x[4]));
+    Q.v[0] = (float)(x[2] * fac);
+    Q.v[1] = (float)(x[3] * fac);
+    Q.v[2] = (float)(x[4] * fac);
+    Q.rho = fmaxf((float)x[0], 1e-12f);
+    Q.p = fmaxf((float)x[1], 1e-14f);
+    Q.eps = e.eps_of(Q.p, Q.rho);
+    Q.B[N] = Bn;
+    Q.B[T1] = (float)x[5];
+    Q.B[T2] = (float)x[6];
+  }
+}
+
+// fix: count + indices of the faces to recompute in FP64 (fix[0] = count,
+// indices from fix[2]); capacity >= the number of faces.
+template <int N, int LIM, int SOL>
+__global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, srmhd_eos e64,
+                                                   srmhd_flux_diag* d, Flx F, unsigned* fix) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const bool active = t < n_faces<N>(g);
+  int st = 0;
+  double fD = 0.0;
+  if (active) {
+    int i, j;
+    face_ij<N>(g, t, i, j);
+    const f32::Eos e(e64);
+    f32::Prim l, r;
+    face_prims_f32<N, LIM>(g, S, P, e, i, j, l, r);
+    float f[8];
+    if (SOL == SOLVER_HLLE) {
+      f32::hlle<N>(l, r, e, f);
+    } else if (SOL == SOLVER_HLLC) {
+      f32::hllc<N>(l, r, e, f);
+    } else {
+      st = f32::hlld<N>(l, r, e, f);
+      if (st & f32::FS_DOUBT) fix[2 + atomicAdd(fix, 1u)] = (unsigned)t;
+    }
+    F.p[0][t] = (double)f[CD];
+    F.p[1][t] = (double)f[CS];
+    F.p[2][t] = (double)f[CS + 1];
+    F.p[3][t] = (double)f[CS + 2];
+    F.p[4][t] = (double)f[CT];
+    F.p[5][t] = (double)f[CB + 2];
+    F.p[6][t] = (double)(N == 0 ? f[CB + 1] : f[CB + 0]);
+    fD = (double)f[CD];
+  }
+  diag_accumulate(d, active, st, 0, dbits(fabs(fD)));
+}
+
+template <int N, int LIM>
+void launch_lim(int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
+                srmhd_flux_diag* d, const Flx& F, unsigned* fix, hipStream_t s) {
+  const int nb = blocks_for(n_faces_host(g, N));
+  switch (sol) {
+    case SOLVER_HLLE: k_sweep_f32<N, LIM, SOLVER_HLLE><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix); break;
+    case SOLVER_HLLC: k_sweep_f32<N, LIM, SOLVER_HLLC><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix); break;
+    default: k_sweep_f32<N, LIM, SOLVER_HLLD><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix);
+  }
+}
+
+template <int N>
+void launch_dir(int lim, int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
+                srmhd_flux_diag* d, const Flx& F, unsigned* fix, hipStream_t s) {
+  switch (lim) {
+    case LIM_PCM: launch_lim<N, LIM_PCM>(sol, g, S, P, e, d, F, fix, s); break;
+    case LIM_MC: launch_lim<N, LIM_MC>(sol, g, S, P, e, d, F, fix, s); break;
+    case LIM_MINMOD: launch_lim<N, LIM_MINMOD>(sol, g, S, P, e, d, F, fix, s); break;
+    case LIM_WENO5Z: launch_lim<N, LIM_WENO5Z>(sol, g, S, P, e, d, F, fix, s); break;
+    case LIM_MP5: launch_lim<N, LIM_MP5>(sol, g, S, P, e, d, F, fix, s); break;
+    default: launch_lim<N, LIM_MP7>(sol, g, S, P, e, d, F, fix, s);
+  }
+}
+
+}  // namespace
+
+void sweep_mixed(int dir, int lim, int sol, const G2& g, const St2& S, const Pr2& P,
+                 const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,
+                 hipStream_t s) {
+  if (dir == 0)
+    launch_dir<0>(lim, sol, g, S, P, e, d, F, fix, s);
+  else
+    launch_dir<1>(lim, sol, g, S, P, e, d, F, fix, s);
+}
+
+}  // namespace srmhd

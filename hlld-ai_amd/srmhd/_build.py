@@ -18,8 +18,12 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(ROOT, "csrc")
 OUT_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(OUT_DIR, "libsrmhd_hip.so")
-SOURCES = ["srmhd_kernels.hip", "srmhd_tubes.hip"]
-HEADERS = ["srmhd_common.h", "srmhd_c2p.h", "srmhd_recon.h", "srmhd_riemann.h", "srmhd_launch.h"]
+SOURCES = ["srmhd_kernels.hip", "srmhd_tubes.hip", "srmhd_mixed.hip"]
+# the mixed-precision sweeps: FP32 contraction and approximate FP32 division
+# and sqrt (this file only; the FP64 sources keep -ffp-contract=off)
+EXTRA = {"srmhd_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
+HEADERS = ["srmhd_common.h", "srmhd_c2p.h", "srmhd_recon.h", "srmhd_riemann.h", "srmhd_launch.h",
+           "srmhd_grid2d.h", "srmhd_mixed.h", "srmhd_f32.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -43,7 +47,8 @@ def build(verbose=False, extra=()):
     for src in SOURCES:
         obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmd = [HIPCC, *FLAGS, *extra, "-c", "-I", CSRC, "-o", obj, os.path.join(CSRC, src)]
+        cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), *extra, "-c", "-I", CSRC, "-o", obj,
+               os.path.join(CSRC, src)]
         procs.append((src, _run([c for c in cmd if c != "-shared"], verbose)))
     failed = []
     for src, p in procs:

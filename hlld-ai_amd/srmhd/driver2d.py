@@ -34,7 +34,8 @@ class Stepper2D:
     """Device buffers, workspace and launch parameters for one grid + scheme."""
 
     def __init__(self, grid, eos, *, solver="hlld", limiter="mc", bc_x="outflow",
-                 bc_y="outflow", upwind=True, atmo_rho=1e-10, atmo_eps=1e-10, device=None):
+                 bc_y="outflow", upwind=True, atmo_rho=1e-10, atmo_eps=1e-10, device=None,
+                 precision="fp64"):
         self.grid = grid
         self.device = torch.device(device) if device is not None else grid.device
         if self.device.type != "cuda":
@@ -48,8 +49,12 @@ class Stepper2D:
             raise ValueError(f"unknown limiter {limiter!r}")
         self.bc = (bc_x, bc_y)
         self.cg = _c_grid(grid, bc_x, bc_y)
+        if precision not in _lib.PRECISION_IDS:
+            raise ValueError(f"precision must be one of {tuple(_lib.PRECISION_IDS)}")
+        self.precision = precision
         self.cs = _lib.Scheme2D(_lib.SOLVER_IDS[self.solver], _lib.LIMITER_IDS[limiter],
-                                1 if upwind else 0, 0, float(atmo_rho), float(atmo_eps))
+                                1 if upwind else 0, _lib.PRECISION_IDS[precision],
+                                float(atmo_rho), float(atmo_eps))
         nb = self.lib.srmhd2d_workspace_bytes(ctypes.byref(self.cg), ctypes.byref(self.cs))
         if nb < 0:
             raise ValueError("grid / scheme rejected by srmhd2d_workspace_bytes")
@@ -145,16 +150,17 @@ class Stepper2D:
 _STEPPERS = {}
 
 
-def _stepper(grid, eos, solver, limiter, bc_x, bc_y, upwind, atmo_rho, device):
+def _stepper(grid, eos, solver, limiter, bc_x, bc_y, upwind, atmo_rho, device,
+             precision="fp64"):
     key = (id(grid), grid.nx, grid.ny, grid.ng, float(grid.dx), float(grid.dy),
            float(eos.K), float(eos.gamma), getattr(eos, "gamma_th", None), solver, limiter,
-           bc_x, bc_y, bool(upwind), float(atmo_rho), str(device))
+           bc_x, bc_y, bool(upwind), float(atmo_rho), str(device), precision)
     st = _STEPPERS.get(key)
     if st is None:
         if len(_STEPPERS) > 8:
             _STEPPERS.clear()
         st = Stepper2D(grid, eos, solver=solver, limiter=limiter, bc_x=bc_x, bc_y=bc_y,
-                       upwind=upwind, atmo_rho=atmo_rho, device=device)
+                       upwind=upwind, atmo_rho=atmo_rho, device=device, precision=precision)
         _STEPPERS[key] = st
     return st
 
@@ -221,13 +227,15 @@ def compute_dt_2d(prims, grid, eos, cfl, speed_of_light=False):
 
 def rk_step_ct(state, grid, eos, dt, *, scheme="rk3", bc_x="outflow", bc_y="outflow",
                flux_fn="hlld", limiter="mc", vel_var="Wv", emf_mode="solver", upwind=True,
-               atmo_rho=1e-10, recorder=None):
+               atmo_rho=1e-10, recorder=None, precision="fp64"):
     """One SSP-RK step with constrained transport (driver2d.py:221-249).
     ``flux_fn`` is one of this package's flux plugins, the reference's, or a
-    name; ``dt`` a float or a device scalar tensor."""
+    name; ``dt`` a float or a device scalar tensor.  ``precision="mixed"``
+    (no counterpart in the reference) runs reconstruction and the Riemann
+    solver in FP32 and the rest of the stage in FP64."""
     _check_options(vel_var, emf_mode, recorder)
     st = _stepper(grid, eos, solver_name(flux_fn), limiter, bc_x, bc_y, upwind, atmo_rho,
-                  grid.device)
+                  grid.device, precision)
     st.set_dt(dt)
     U0 = [t.contiguous() for t in state.state_list()]
     U, P = U0, [state.prims[k].contiguous() for k in PRIM6]

@@ -99,8 +99,15 @@ typedef struct srmhd_grid2d {
   double dx, dy;
 } srmhd_grid2d;
 
+/* precision: SRMHD_PREC_FP64 (the reference's float64 arithmetic, bit-faithful)
+ * or SRMHD_PREC_MIXED (no counterpart in the reference: reconstruction and
+ * the Riemann solver in FP32, with the HLL-state con2prim that seeds HLLD's
+ * p* bracket solved per face in FP32; corner EMF, CT face-B update, RK
+ * accumulation and con2prim stay FP64, so div B is untouched). */
+enum { SRMHD_PREC_FP64 = 0, SRMHD_PREC_MIXED = 1 };
+
 typedef struct srmhd_scheme2d {
-  int32_t solver, limiter, upwind, reserved;
+  int32_t solver, limiter, upwind, precision;
   double atmo_rho, atmo_eps;
 } srmhd_scheme2d;
 
