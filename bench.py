@@ -32,16 +32,30 @@ FP64_PEAK_TFLOPS = 78.6
 # SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 on the 1024^2 rotor (profiles/
 # sweep_x_a_rotor_counters.md): flops = 64 lanes x (add + mul + 2 fma + trans)
 # per wave, divided by the elements of the launch.
-FP64_FLOPS_PER_ELEMENT = {"sweep_x_a": 18878.0, "sweep_y_a": 18878.0, "c2p_a": 15487.0}
+FP64_FLOPS_PER_ELEMENT = {("fp64", "sweep_x_a"): 18878.0, ("fp64", "sweep_y_a"): 18878.0,
+                          ("fp64", "c2p_a"): 15487.0, ("mixed", "c2p_a"): 15487.0}
+
+# the kernel class each form spends most time in (rocprofv3 --stats of the
+# 2048^2 rotor: profiles/rotor2048_weno5z_hlld_kernel_stats.md)
+DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "c2p_a"}
+
+# measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
+# scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
+# (form, kernel class, n)
+MEASURED_TRAFFIC = {}
 
 
-def algorithmic_bytes(kernel, n, ng):
+def algorithmic_bytes(kernel, n, ng, precision="fp64"):
     """Compulsory HBM bytes of one launch of ``kernel`` on an n^2 grid with
     ng ghosts: every input array element read once, every output written
     once (float64)."""
     nt = n + 2 * ng
     cells = nt * nt
     faces = (nt + 1) * nt
+    if precision == "mixed" and kernel in ("sweep_x_b", "sweep_y_b"):
+        # the one FP32 sweep kernel: prims rho, p, vx, vy, vz + cons Bz +
+        # both face-B arrays in, 7 FP64 flux arrays out
+        return 8 * (6 * cells + 2 * faces) + 56 * faces
     if kernel in ("sweep_x_b", "sweep_y_b"):
         # prims rho, p, vx, vy, vz + cons Bz + both face-B arrays; the
         # pass-A record (3 doubles/face); 7 flux arrays out
@@ -129,7 +143,14 @@ def bench_tubes(a):
     wall = time.perf_counter() - t0
     kernel_ms = ev0.elapsed_time(ev1)
     zc = zones * a.steps / wall
-    finite = bool(all(torch.isfinite(p).all().item() for p in batch.P))
+    fin = torch.ones(batch.total_cells, dtype=torch.bool, device=dev)
+    for p in batch.P:
+        fin &= torch.isfinite(p)
+    seg = torch.repeat_interleave(torch.arange(a.tubes, device=dev),
+                                  (batch.off_host[1:] - batch.off_host[:-1]).to(dev))
+    bad = torch.zeros(a.tubes, dtype=torch.int64, device=dev).index_add_(
+        0, seg, (~fin).to(torch.int64))
+    nonfinite = [int(t) for t in torch.nonzero(bad).reshape(-1).tolist()]
     # the per-tube loop on the GPU: the first tubes one launch each, same steps
     n_loop = min(a.loop_tubes, a.tubes)
     sub = [T.TubeBatch(lengths[i:i + 1], {k: v[i:i + 1] for k, v in left.items()},
@@ -156,7 +177,10 @@ def bench_tubes(a):
         "per_tube_loop": {"value": loop_zc, "unit": "zone-cycles/s", "tubes_timed": n_loop,
                           "speedup_of_batch": zc / loop_zc},
         "autotune_ms": tune,
-        "state_finite": finite,
+        # MP5 + HLLD blows up on a few of the random Riemann problems; the CPU
+        # oracle's per-tube loop does the same on them (e.g. tubes 1794 and
+        # 1873 of seed 0 are non-finite after 100 steps on both)
+        "nonfinite_tubes": {"count": len(nonfinite), "first": nonfinite[:16]},
     }
     if not a.no_cpu:
         try:
@@ -166,11 +190,94 @@ def bench_tubes(a):
     print(json.dumps(out), flush=True)
 
 
+def _rotor_form(a, precision, kernel, U_init, P_init):
+    """Warm up, then time a.steps RK3 steps of one precision form from the
+    initial state; live HIP-event timing of one kernel class."""
+    import ctypes
+    import torch
+    from srmhd import _lib
+    from srmhd import driver2d as D2, eos as E
+
+    eos = E.hybrid_eos(0.0, 5.0 / 3.0, 5.0 / 3.0)
+    step = D2.Stepper2D(a.grid, eos, solver=a.solver, limiter=a.limiter, precision=precision)
+
+    def one_step():
+        step.compute_dt(step.U, step.P, a.cfl)
+        step.step("rk3")
+
+    step.load([t.clone() for t in U_init], [t.clone() for t in P_init])
+    for _ in range(a.warmup):
+        one_step()
+    step.load([t.clone() for t in U_init], [t.clone() for t in P_init])
+    torch.cuda.synchronize()
+    print(f"{precision}: warmup done ({a.warmup} steps)", file=sys.stderr, flush=True)
+    lib = step.lib
+    _lib.check(lib.srmhd_timing_enable(_lib.KERNEL_CLASSES[kernel], 3 * a.steps),
+               "srmhd_timing_enable")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        one_step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    _lib.check(lib.srmhd_timing_read(ctypes.byref(ms), ctypes.byref(cnt)), "srmhd_timing_read")
+    lib.srmhd_timing_enable(-1, 0)
+    print(f"{precision}: {a.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+    avg_ms = ms.value / max(cnt.value, 1)
+    n, ng = a.n, a.ng
+    kb = algorithmic_bytes(kernel, n, ng, precision)
+    nt = n + 2 * ng
+    kelems = nt * nt if kernel.startswith("c2p") else (nt + 1) * nt
+    achieved = kb / (avg_ms * 1e-3) / 1e9
+    zc = n * n * a.steps / wall
+    traffic = MEASURED_TRAFFIC.get((precision, kernel, n))
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+            "avg_launch_ms": avg_ms, "launches": cnt.value, "algorithmic_bytes_per_launch": kb}
+    flops = FP64_FLOPS_PER_ELEMENT.get((precision, kernel))
+    valu = None
+    if flops is not None:
+        tf = flops * kelems / (avg_ms * 1e-3) / 1e12
+        valu = {"kernel": kernel, "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tf / FP64_PEAK_TFLOPS}
+    return {"step": step, "zc": zc, "wall": wall, "roofline": roof, "fp64_valu": valu,
+            "ms_per_step": 1e3 * wall / a.steps}
+
+
+def _rotor_fields(step, n, ng):
+    """rho, p, |B| (cell-centred from the faces) over the physical cells and
+    div B: max |div B| and max |div B| / (max |B_face| / dx)."""
+    import torch
+    sl = (slice(ng, ng + n), slice(ng, ng + n))
+    Bxf, Byf, Bz = step.U[6], step.U[7], step.U[5]
+    Bx = 0.5 * (Bxf[ng:ng + n, ng:ng + n] + Bxf[ng + 1:ng + n + 1, ng:ng + n])
+    By = 0.5 * (Byf[ng:ng + n, ng:ng + n] + Byf[ng:ng + n, ng + 1:ng + n + 1])
+    Bm = torch.sqrt(Bx * Bx + By * By + Bz[sl] * Bz[sl])
+    dx, dy = step.grid.dx, step.grid.dy
+    div = ((Bxf[ng + 1:ng + n + 1, ng:ng + n] - Bxf[ng:ng + n, ng:ng + n]) / dx
+           + (Byf[ng:ng + n, ng + 1:ng + n + 1] - Byf[ng:ng + n, ng:ng + n]) / dy)
+    dmax = float(div.abs().max())
+    bmax = max(float(Bxf.abs().max()), float(Byf.abs().max()))
+    return step.P[0][sl], step.P[4][sl], Bm, {"max_abs": dmax,
+                                               "max_normalised": dmax / (bmax / dx)}
+
+
+def _l1(a, b):
+    """The reference's l1_rel (scripts/rotor_compare.py:57-58):
+    mean |a - b| / mean |b|, b the FP64 form."""
+    return float((a - b).abs().mean() / b.abs().mean())
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="rotor", choices=("rotor", "tubes"),
                     help="rotor: the 2048^2 rotor (headline); tubes: 4096 batched tubes")
+    ap.add_argument("--precision", default="both", choices=("both", "fp64", "mixed"),
+                    help="rotor forms to time; both: mixed is the headline value, FP64 beside "
+                         "it, and the L1 distance between them after the timed steps")
     ap.add_argument("--tubes", type=int, default=4096)
     ap.add_argument("--loop-tubes", type=int, default=16,
                     help="tubes timed one launch each for the per-tube-loop comparison")
@@ -180,8 +287,9 @@ def main():
     ap.add_argument("--limiter", default="weno5z")
     ap.add_argument("--solver", default="hlld")
     ap.add_argument("--cfl", type=float, default=0.4)
-    ap.add_argument("--kernel", default="sweep_x_a",
-                    help="kernel class timed live for the roofline")
+    ap.add_argument("--kernel", default=None,
+                    help="kernel class timed live for the roofline (default: the dominant one "
+                         "of each form)")
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -200,46 +308,25 @@ def main():
         bench_tubes(a)
         return
     if a.steps is None:
-        a.steps = 10
+        a.steps = 200
 
-    import ctypes
     import torch
-    from srmhd import _lib
-    from srmhd import driver2d as D2, eos as E, grid as G, initial_data2d as I
+    from srmhd import eos as E, grid as G, initial_data2d as I
     from srmhd.reconstruction import ghosts_needed
 
     dev = torch.device("cuda:0")
     eos = E.hybrid_eos(0.0, 5.0 / 3.0, 5.0 / 3.0)
-    ng = ghosts_needed(a.limiter)
-    grid = G.Grid2D(-0.5, 0.5, a.n, -0.5, 0.5, a.n, ng=ng, device=dev)
-    P, Az = I.rotor_tapered(grid, eos)
-    st = I.init_state(grid, eos, P, Az, limiter=a.limiter)
-    step = D2.Stepper2D(grid, eos, solver=a.solver, limiter=a.limiter)
-    step.load(st.state_list(), st.prim_list())
-    del st, P, Az
-
-    def one_step():
-        step.compute_dt(step.U, step.P, a.cfl)
-        step.step("rk3")
-
-    for _ in range(a.warmup):
-        one_step()
-    torch.cuda.synchronize()
-    print(f"warmup done ({a.warmup} steps)", file=sys.stderr, flush=True)
-
-    lib = step.lib
-    kcls = _lib.KERNEL_CLASSES[a.kernel]
-    _lib.check(lib.srmhd_timing_enable(kcls, 3 * a.steps), "srmhd_timing_enable")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        one_step()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ctypes.c_double()
-    cnt = ctypes.c_int64()
-    _lib.check(lib.srmhd_timing_read(ctypes.byref(ms), ctypes.byref(cnt)), "srmhd_timing_read")
-    lib.srmhd_timing_enable(-1, 0)
+    a.ng = ng = ghosts_needed(a.limiter)
+    a.grid = G.Grid2D(-0.5, 0.5, a.n, -0.5, 0.5, a.n, ng=ng, device=dev)
+    P, Az = I.rotor_tapered(a.grid, eos)
+    st = I.init_state(a.grid, eos, P, Az, limiter=a.limiter)
+    U_init, P_init = st.state_list(), st.prim_list()
+    del P, Az
+    forms = ("fp64", "mixed") if a.precision == "both" else (a.precision,)
+    res = {}
+    for prec in forms:
+        kernel = a.kernel or DOMINANT_KERNEL[prec]
+        res[prec] = _rotor_form(a, prec, kernel, U_init, P_init)
 
     if a.copy_calibration:
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
@@ -249,40 +336,43 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    zc = a.n * a.n * a.steps / wall
-    avg_ms = ms.value / max(cnt.value, 1)
-    kb = algorithmic_bytes(a.kernel, a.n, ng)
-    nt = a.n + 2 * ng
-    kelems = nt * nt if a.kernel.startswith("c2p") else (nt + 1) * nt
-    achieved = kb / (avg_ms * 1e-3) / 1e9
+    head = "mixed" if "mixed" in res else "fp64"
+    h = res[head]
+    fields = {p: _rotor_fields(r["step"], a.n, ng) for p, r in res.items()}
+    finite = all(bool(torch.isfinite(fields[p][0]).all().item()) for p in fields)
     bytes_per_zc = 752.0  # RK3 2D algorithmic bytes per zone-cycle
-    rho = step.P[0][ng:ng + a.n, ng:ng + a.n]
-    finite = bool(torch.isfinite(rho).all().item())
-
+    form_desc = {"fp64": "float64 throughout (plain form)",
+                 "mixed": "reconstruction + Riemann solver in FP32, con2prim + CT + RK in FP64"}
     out = {
         "metric": "zone-cycles/s, 2D rotor 2048^2 (FP64 vs FP32-flux @ L1<=1e-6); HBM GB/s",
-        "value": zc, "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": 1e3 * wall / a.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "value": h["zc"], "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64 (mixed)" if head == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": f"tapered relativistic rotor {a.n}^2, {a.limiter.upper()} + "
                                f"{a.solver.upper()} + CT + Kastaun C2P + SSP-RK3, CFL {a.cfl}, "
-                               "float64 (plain form)",
-                   "n": a.n, "ng": ng, "limiter": a.limiter, "solver": a.solver, "scheme": "rk3"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": a.kernel, "avg_launch_ms": avg_ms, "launches": cnt.value,
-                     "algorithmic_bytes_per_launch": kb},
-        "fp64_valu": ({"kernel": a.kernel,
-                       "achieved": FP64_FLOPS_PER_ELEMENT[a.kernel] * kelems / (avg_ms * 1e-3) / 1e12,
-                       "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                       "frac": FP64_FLOPS_PER_ELEMENT[a.kernel] * kelems / (avg_ms * 1e-3) / 1e12
-                       / FP64_PEAK_TFLOPS}
-                      if a.kernel in FP64_FLOPS_PER_ELEMENT else None),
+                               f"{a.steps} steps, {form_desc[head]}",
+                   "n": a.n, "ng": ng, "limiter": a.limiter, "solver": a.solver, "scheme": "rk3",
+                   "precision": head},
+        "roofline": h["roofline"],
+        "fp64_valu": h["fp64_valu"],
         "step_hbm": {"bytes_per_zone_cycle": bytes_per_zc,
-                     "achieved_GBs": zc * bytes_per_zc / 1e9,
-                     "frac": zc * bytes_per_zc / 1e9 / HBM_PEAK_GBS},
+                     "achieved_GBs": h["zc"] * bytes_per_zc / 1e9,
+                     "frac": h["zc"] * bytes_per_zc / 1e9 / HBM_PEAK_GBS},
+        "div_b": fields[head][3],
         "state_finite": finite,
     }
+    if "fp64" in res and head != "fp64":
+        f = res["fp64"]
+        ra, pa, ba, _ = fields["fp64"]
+        rb, pb, bb, _ = fields["mixed"]
+        l1 = {"rho": _l1(rb, ra), "p": _l1(pb, pa), "absB": _l1(bb, ba)}
+        out["fp64_form"] = {"value": f["zc"], "ms_per_step": f["ms_per_step"],
+                            "roofline": f["roofline"], "fp64_valu": f["fp64_valu"],
+                            "div_b": fields["fp64"][3]}
+        out["speedup_vs_fp64"] = h["zc"] / f["zc"]
+        out["l1_vs_fp64"] = l1
+        out["l1_rho_gate_1e-6_met"] = l1["rho"] <= 1e-6
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.limiter)
