@@ -33,11 +33,11 @@ FP64_PEAK_TFLOPS = 78.6
 # sweep_x_a_rotor_counters.md): flops = 64 lanes x (add + mul + 2 fma + trans)
 # per wave, divided by the elements of the launch.
 FP64_FLOPS_PER_ELEMENT = {("fp64", "sweep_x_a"): 18878.0, ("fp64", "sweep_y_a"): 18878.0,
-                          ("fp64", "c2p_a"): 15487.0, ("mixed", "c2p_a"): 15487.0}
+                          ("fp64", "c2p_a"): 15487.0}
 
 # the kernel class each form spends most time in (rocprofv3 --stats of the
 # 2048^2 rotor: profiles/rotor2048_weno5z_hlld_kernel_stats.md)
-DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "c2p_a"}
+DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "sweep_x_b"}
 
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
@@ -52,7 +52,7 @@ def algorithmic_bytes(kernel, n, ng, precision="fp64"):
     nt = n + 2 * ng
     cells = nt * nt
     faces = (nt + 1) * nt
-    if precision == "mixed" and kernel in ("sweep_x_b", "sweep_y_b"):
+    if precision != "fp64" and kernel in ("sweep_x_b", "sweep_y_b"):
         # the one FP32 sweep kernel: prims rho, p, vx, vy, vz + cons Bz +
         # both face-B arrays in, 7 FP64 flux arrays out
         return 8 * (6 * cells + 2 * faces) + 56 * faces
@@ -62,6 +62,10 @@ def algorithmic_bytes(kernel, n, ng, precision="fp64"):
         return 8 * (6 * cells + 2 * faces) + 24 * faces + 56 * faces
     if kernel in ("sweep_x_a", "sweep_y_a"):
         return 8 * (6 * cells + 2 * faces) + 24 * faces
+    if precision != "fp64" and kernel == "c2p_a":
+        # the per-cell con2prim of the mixed forms: physical cells only, the
+        # 6 cell + 2 face state arrays in, 6 primitives out
+        return n * n * (8 * 8 + 6 * 8)
     if kernel == "c2p_a":
         # cons D, Sx, Sy, Sz, tau, Bz + both face arrays in; record out
         return 8 * (6 * cells + 2 * faces) + 24 * cells
@@ -275,9 +279,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="rotor", choices=("rotor", "tubes"),
                     help="rotor: the 2048^2 rotor (headline); tubes: 4096 batched tubes")
-    ap.add_argument("--precision", default="both", choices=("both", "fp64", "mixed"),
-                    help="rotor forms to time; both: mixed is the headline value, FP64 beside "
-                         "it, and the L1 distance between them after the timed steps")
+    ap.add_argument("--precision", default="fp64,mixed,mixed_r64",
+                    help="comma-separated rotor forms to time (fp64, mixed, mixed_r64); the "
+                         "first mixed form is the headline value, the others sit beside it "
+                         "with their L1 distance from FP64 after the timed steps")
     ap.add_argument("--tubes", type=int, default=4096)
     ap.add_argument("--loop-tubes", type=int, default=16,
                     help="tubes timed one launch each for the per-tube-loop comparison")
@@ -322,7 +327,10 @@ def main():
     st = I.init_state(a.grid, eos, P, Az, limiter=a.limiter)
     U_init, P_init = st.state_list(), st.prim_list()
     del P, Az
-    forms = ("fp64", "mixed") if a.precision == "both" else (a.precision,)
+    forms = tuple(f.strip() for f in a.precision.split(",") if f.strip())
+    for f in forms:
+        if f not in DOMINANT_KERNEL:
+            raise SystemExit(f"unknown precision form {f!r}")
     res = {}
     for prec in forms:
         kernel = a.kernel or DOMINANT_KERNEL[prec]
@@ -336,19 +344,23 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    head = "mixed" if "mixed" in res else "fp64"
+    mixed_forms = [f for f in forms if f != "fp64"]
+    head = mixed_forms[0] if mixed_forms else "fp64"
     h = res[head]
     fields = {p: _rotor_fields(r["step"], a.n, ng) for p, r in res.items()}
     finite = all(bool(torch.isfinite(fields[p][0]).all().item()) for p in fields)
     bytes_per_zc = 752.0  # RK3 2D algorithmic bytes per zone-cycle
     form_desc = {"fp64": "float64 throughout (plain form)",
-                 "mixed": "reconstruction + Riemann solver in FP32, con2prim + CT + RK in FP64"}
+                 "mixed": "reconstruction + Riemann-solver root finding in FP32; HLLD flux "
+                          "assembly, con2prim, CT and RK in FP64",
+                 "mixed_r64": "Riemann-solver root finding in FP32; reconstruction, HLLD flux "
+                              "assembly, con2prim, CT and RK in FP64"}
     out = {
         "metric": "zone-cycles/s, 2D rotor 2048^2 (FP64 vs FP32-flux @ L1<=1e-6); HBM GB/s",
         "value": h["zc"], "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64 (mixed)" if head == "mixed" else "f64", "data": "synthetic",
+        "dtype": "f32+f64 (mixed)" if head != "fp64" else "f64", "data": "synthetic",
         "config": {"workload": f"tapered relativistic rotor {a.n}^2, {a.limiter.upper()} + "
                                f"{a.solver.upper()} + CT + Kastaun C2P + SSP-RK3, CFL {a.cfl}, "
                                f"{a.steps} steps, {form_desc[head]}",
@@ -365,14 +377,23 @@ def main():
     if "fp64" in res and head != "fp64":
         f = res["fp64"]
         ra, pa, ba, _ = fields["fp64"]
-        rb, pb, bb, _ = fields["mixed"]
-        l1 = {"rho": _l1(rb, ra), "p": _l1(pb, pa), "absB": _l1(bb, ba)}
+
+        def l1_of(form):
+            rb, pb, bb, _ = fields[form]
+            return {"rho": _l1(rb, ra), "p": _l1(pb, pa), "absB": _l1(bb, ba)}
+
+        l1 = l1_of(head)
         out["fp64_form"] = {"value": f["zc"], "ms_per_step": f["ms_per_step"],
                             "roofline": f["roofline"], "fp64_valu": f["fp64_valu"],
                             "div_b": fields["fp64"][3]}
         out["speedup_vs_fp64"] = h["zc"] / f["zc"]
         out["l1_vs_fp64"] = l1
         out["l1_rho_gate_1e-6_met"] = l1["rho"] <= 1e-6
+        out["other_forms"] = {
+            form: {"value": res[form]["zc"], "ms_per_step": res[form]["ms_per_step"],
+                   "speedup_vs_fp64": res[form]["zc"] / f["zc"], "l1_vs_fp64": l1_of(form),
+                   "div_b": fields[form][3], "description": form_desc[form]}
+            for form in mixed_forms[1:]}
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.limiter)

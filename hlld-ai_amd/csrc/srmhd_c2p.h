@@ -26,34 +26,35 @@ constexpr double C2P_TOL = 1e-15;
 constexpr int C2P_SECANT = 60;
 constexpr int C2P_BISECT = 200;
 
-struct Kastaun {
-  double D, q, r[3], r2, bt[3], bt2, br, bperp2, v02, B[3];
+template <class T>
+struct KastaunT {
+  T D, q, r[3], r2, bt[3], bt2, br, bperp2, v02, B[3];
   srmhd_eos e;
 
   // c2p.py:48-106; cons order D, tau, Sx, Sy, Sz, Bx, By, Bz
-  SRMHD_DEV void init(const double c[8], const srmhd_eos& eos) {
+  SRMHD_DEV void init(const T c[8], const srmhd_eos& eos) {
     e = eos;
     D = c[CD];
-    double S[3] = {c[CS], c[CS + 1], c[CS + 2]};
-    const double tau = clo(c[CT], 0.0);
+    T S[3] = {c[CS], c[CS + 1], c[CS + 2]};
+    const T tau = clo(c[CT], 0.0);
     B[0] = c[CB];
     B[1] = c[CB + 1];
     B[2] = c[CB + 2];
-    double S2 = sq(S[0]) + sq(S[1]) + sq(S[2]);
-    const double Sn = csqrt(S2);
-    const double Smax = D + tau;
-    const double fac = (Sn > Smax) ? 0.9999 * Smax / (Sn + TINY) : 1.0;
+    T S2 = sq(S[0]) + sq(S[1]) + sq(S[2]);
+    const T Sn = csqrt(S2);
+    const T Smax = D + tau;
+    const T fac = (Sn > Smax) ? T(0.9999 * Smax / (Sn + TINY)) : T(1.0);
     S[0] = S[0] * fac;
     S[1] = S[1] * fac;
     S[2] = S[2] * fac;
     S2 = sq(S[0]) + sq(S[1]) + sq(S[2]);
-    const double B2 = sq(B[0]) + sq(B[1]) + sq(B[2]);
-    const double BS = B[0] * S[0] + B[1] * S[1] + B[2] * S[2];
+    const T B2 = sq(B[0]) + sq(B[1]) + sq(B[2]);
+    const T BS = B[0] * S[0] + B[1] * S[1] + B[2] * S[2];
     q = tau / (D + TINY);
 #pragma unroll
     for (int i = 0; i < 3; ++i) r[i] = S[i] / (D + TINY);
     r2 = S2 / (D * D + TINY);
-    const double sD = csqrt(D + TINY);
+    const T sD = csqrt(D + TINY);
 #pragma unroll
     for (int i = 0; i < 3; ++i) bt[i] = B[i] / sD;
     bt2 = B2 / (D + TINY);
@@ -62,31 +63,36 @@ struct Kastaun {
     v02 = r2 / (r2 + 1.0);
   }
 
-  SRMHD_DEV double chi(double mu) const { return 1.0 / (1.0 + mu * bt2); }
+  template <class M>
+  SRMHD_DEV auto chi(const M& mu) const -> decltype(mu + bt2) { return 1.0 / (1.0 + mu * bt2); }
 
-  SRMHD_DEV double rbar2(double mu, double c) const {
+  template <class M, class C>
+  SRMHD_DEV auto rbar2(const M& mu, const C& c) const -> decltype(mu + c + r2) {
     return r2 * (c * c) + mu * c * (1.0 + c) * (br * br);
   }
 
   // mu sqrt(h_min^2 + rbar^2) - 1   c2p.py:124-127
-  SRMHD_DEV double fa(double mu) const { return mu * csqrt(1.0 + rbar2(mu, chi(mu))) - 1.0; }
+  SRMHD_DEV T fa(const T& mu) const { return mu * csqrt(1.0 + rbar2(mu, chi(mu))) - 1.0; }
 
   // master function   c2p.py:129-158
-  SRMHD_DEV double master(double mu, double* rho_out = nullptr, double* eps_out = nullptr) const {
-    const double c = chi(mu);
-    const double mc = mu * c;
-    const double qb = q - 0.5 * bt2 - 0.5 * (mc * mc) * bperp2;
-    const double rb2 = rbar2(mu, c);
-    const double vh2 = chi_(mu * mu * rb2, v02);
-    const double Wh = 1.0 / csqrt(clo(1.0 - vh2, 1e-10));
-    const double rho = tmin(tmax(D / Wh, 1e-12), 1e15);
-    const double eps0 = Wh * (qb - mu * rb2) + vh2 * (Wh * Wh) / (1.0 + Wh);
-    const double eps = tmin(tmax(eps0, eos_eps_lo(e, rho)), 1e5);
-    const double p = eos_press(e, eps, rho);
-    const double a = p / (rho * (1.0 + eps) + TINY);
-    const double nuA = (1.0 + a) * (1.0 + eps) / Wh;
-    const double nuB = (1.0 + a) * (1.0 + qb - mu * rb2);
-    const double nu = tmax(nuA, nuB);
+  template <class M>
+  SRMHD_DEV auto master(const M& mu, decltype(mu + bt2)* rho_out = nullptr,
+                        decltype(mu + bt2)* eps_out = nullptr) const -> decltype(mu + bt2) {
+    using R = decltype(mu + bt2);
+    const R c = chi(mu);
+    const R mc = mu * c;
+    const R qb = q - 0.5 * bt2 - 0.5 * (mc * mc) * bperp2;
+    const R rb2 = rbar2(mu, c);
+    const R vh2 = chi_(mu * mu * rb2, v02);
+    const R Wh = 1.0 / csqrt(clo(1.0 - vh2, 1e-10));
+    const R rho = tmin(tmax(D / Wh, 1e-12), 1e15);
+    const R eps0 = Wh * (qb - mu * rb2) + vh2 * (Wh * Wh) / (1.0 + Wh);
+    const R eps = tmin(tmax(eps0, eos_eps_lo(e, rho)), 1e5);
+    const R p = eos_press(e, eps, rho);
+    const R a = p / (rho * (1.0 + eps) + TINY);
+    const R nuA = (1.0 + a) * (1.0 + eps) / Wh;
+    const R nuB = (1.0 + a) * (1.0 + qb - mu * rb2);
+    const R nu = tmax(nuA, nuB);
     if (rho_out) *rho_out = rho;
     if (eps_out) *eps_out = eps;
     return mu - sdiv(1.0, nu + mu * rb2);
@@ -149,18 +155,20 @@ struct Kastaun {
   }
 
   // primitives at mu   c2p.py:268-310; returns the v-clamp flag
-  SRMHD_DEV bool finish(double mu, Prim& P) const {
-    double rho, eps;
+  template <class M>
+  SRMHD_DEV bool finish(const M& mu, PrimT<decltype(mu + bt2)>& P) const {
+    using R = decltype(mu + bt2);
+    R rho, eps;
     master(mu, &rho, &eps);
-    const double mc = mu * chi(mu);
-    const double mbr = mu * br;
-    double v[3];
+    const R mc = mu * chi(mu);
+    const R mbr = mu * br;
+    R v[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) v[i] = mc * (r[i] + mbr * bt[i]);
-    const double v2 = sq(v[0]) + sq(v[1]) + sq(v[2]);
+    const R v2 = sq(v[0]) + sq(v[1]) + sq(v[2]);
     const double lim = 1.0 - 1e-10;
     const bool clamped = v2 > lim;
-    const double norm = clamped ? csqrt(chi_(v2, lim) / (v2 + TINY)) : 1.0;
+    const R norm = clamped ? R(csqrt(chi_(v2, lim) / (v2 + TINY))) : R(1.0);
 #pragma unroll
     for (int i = 0; i < 3; ++i) P.v[i] = v[i] * norm;
     P.rho = rho;
@@ -172,6 +180,7 @@ struct Kastaun {
     return clamped;
   }
 };
+using Kastaun = KastaunT<double>;
 
 // per-element record of pass A: mu_+ (sign = good bracket), state after 60 steps
 struct C2PRec {
@@ -242,6 +251,45 @@ SRMHD_DEV int c2p_pass_b(const Kastaun& k, const C2PRec& rec, int K, double atmo
   bool conv = fabs(f1) < C2P_TOL;
   bool clamped;
   const bool redo = force < 0 ? (!conv && good) : (force != 0);
+  if (redo) {
+    double fmu;
+    const double mu = k.bisection(mup, fmu);
+    conv = fabs(fmu) < C2P_TOL;
+    clamped = k.finish(mu, P);
+  } else {
+    clamped = k.finish(x1, P);
+  }
+  const bool atmo = P.rho < atmo_rho;
+  if (atmo) {
+    P.rho = atmo_rho;
+    P.eps = atmo_eps;
+    P.v[0] = 0.0;
+    P.v[1] = 0.0;
+    P.v[2] = 0.0;
+  }
+  P.p = eos_press(k.e, P.eps, P.rho);
+  return (good ? C2P_BRACKET_OK : 0) | (conv ? C2P_CONVERGED : 0) | (clamped ? C2P_V_CLAMPED : 0) |
+         (atmo ? C2P_ATMO : 0) | (redo ? C2P_REDONE : 0);
+}
+
+// conservative_to_primitive of ONE element as its own batch (c2p.py:313-363
+// with a batch of one): the secant stops as soon as this element converges.
+// The mixed-precision form uses it for the cells (the batch coupling of the
+// FP64 form only moves converged roots by a few ulps).  Returns the status
+// bits of c2p_pass_b.
+SRMHD_DEV int c2p_single(const Kastaun& k, double atmo_rho, double atmo_eps, Prim& P) {
+  const double mup = k.mu_plus();
+  double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
+  double f0 = k.master(x0), f1 = k.master(x1);
+  const bool good = (f0 * f1) < 0.0;
+#pragma unroll 1
+  for (int it = 0; it < C2P_SECANT; ++it) {
+    if (fabs(f1) < C2P_TOL) break;
+    k.secant_step(x0, f0, x1, f1);
+  }
+  bool conv = fabs(f1) < C2P_TOL;
+  bool clamped;
+  const bool redo = !conv && good;
   if (redo) {
     double fmu;
     const double mu = k.bisection(mup, fmu);

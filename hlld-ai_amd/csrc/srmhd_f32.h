@@ -553,10 +553,11 @@ F32_DEV float star_residual(const StarR& R, float p) {
   return (o.la[0] - o.la[1]) + R.Bn[0] * (sdiv(1.f - o.KR2, dR) + sdiv(1.f - o.KL2, dL));
 }
 
-// HLLD flux (hlld.py:1202-1543) in FP32; returns the face status bits of the
-// FP64 path (not converged, wave order, B_n = 0, fallback).
+// The p* root of hlld_flux in FP32: HLL state, its con2prim (p_hll), p0,
+// the bracket with its expansions and the clamped secant (hlld.py:1255-1356).
+// Returns whether |residual| < PSTAR_TOL was reached; ps, p_hll out.
 template <int N>
-F32_DEV int hlld(const Prim& L, const Prim& Rp, const Eos& e, float F[8]) {
+F32_DEV bool hlld_pstar(const Prim& L, const Prim& Rp, const Eos& e, float& ps, float& p_hll) {
   constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
   float uL[8], uR[8], fL[8], fR[8], cmax, cmin;
   srmhd_fluxes<N>(L, Rp, e, uL, uR, fL, fR, cmax, cmin);
@@ -566,7 +567,7 @@ F32_DEV int hlld(const Prim& L, const Prim& Rp, const Eos& e, float F[8]) {
     fH[k] = hll_flux(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
     uH[k] = hll_state(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
   }
-  const float p_hll = Kastaun(uH, e).total_pressure();
+  p_hll = Kastaun(uH, e).total_pressure();
 
   const float a = fH[CT] + fH[CD];
   const float b = -fH[CS + N] - uH[CT] - uH[CD];
@@ -582,7 +583,6 @@ F32_DEV int hlld(const Prim& L, const Prim& Rp, const Eos& e, float F[8]) {
   const bool use_p0 = (0.5f * (BnL * BnL + BnR * BnR) / p_hll) < 0.1f;
   float plo = use_p0 ? p0 * 0.99f : p_hll * 0.99f;
   float phi = use_p0 ? p0 * 1.01f : p_hll * 1.01f;
-  const bool bn_zero = (fabsf(BnL) < 1e-14f) && (fabsf(BnR) < 1e-14f);
 
   StarR R;
   R.cm = cmin;
@@ -631,7 +631,46 @@ F32_DEV int hlld(const Prim& L, const Prim& Rp, const Eos& e, float F[8]) {
     f1 = f2;
   }
   if (!conv && fabsf(f1) < PSTAR_TOL) conv = true;
-  const float ps = x1;
+  ps = x1;
+  return conv;
+}
+
+// HLLD flux (hlld.py:1202-1543) in FP32; returns the face status bits of the
+// FP64 path (not converged, wave order, B_n = 0, fallback).
+template <int N>
+F32_DEV int hlld(const Prim& L, const Prim& Rp, const Eos& e, float F[8]) {
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+  float ps, p_hll;
+  const bool conv = hlld_pstar<N>(L, Rp, e, ps, p_hll);
+  float uL[8], uR[8], fL[8], fR[8], cmax, cmin;
+  srmhd_fluxes<N>(L, Rp, e, uL, uR, fL, fR, cmax, cmin);
+  float fH[8], uH[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    fH[k] = hll_flux(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
+    uH[k] = hll_state(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
+  }
+  const float BnL = uL[CB + N], BnR = uR[CB + N];
+  const bool bn_zero = (fabsf(BnL) < 1e-14f) && (fabsf(BnR) < 1e-14f);
+  StarR R;
+  R.cm = cmin;
+  R.cx = cmax;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const float* u = s ? uR : uL;
+    const float* f = s ? fR : fL;
+    const float w = s ? cmax : -cmin;
+    R.RSn[s] = w * u[CS + N] - f[CS + N];
+    R.RS1[s] = w * u[CS + T1] - f[CS + T1];
+    R.RS2[s] = w * u[CS + T2] - f[CS + T2];
+    R.RBn[s] = w * u[CB + N] - f[CB + N];
+    R.RB1[s] = w * u[CB + T1] - f[CB + T1];
+    R.RB2[s] = w * u[CB + T2] - f[CB + T2];
+    R.RT[s] = w * (u[CT] + u[CD]) - (f[CT] + f[CD]);
+    R.RD[s] = w * u[CD] - f[CD];
+    R.Sg[s] = sgn(u[CB + N]);
+    R.Bn[s] = u[CB + N];
+  }
   StarOut o;
   star_solve(R, ps, o);
 

@@ -509,6 +509,34 @@ __global__ __launch_bounds__(BLK) void k_c2p2d_b(G2 g, St2 S, srmhd_eos e, const
   P.p[5][c] = q.eps;
 }
 
+// con2prim of the physical cells, each cell its own batch (the mixed form)
+__global__ __launch_bounds__(BLK) void k_c2p2d_single(G2 g, St2 S, srmhd_eos e, double atmo_rho,
+                                                      double atmo_eps, Pr2Out P,
+                                                      unsigned long long* redo_count) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  bool redo = false;
+  if (t < (int64_t)g.nx * g.ny) {
+    const int pi = (int)(t / g.ny), pj = (int)(t - (int64_t)pi * g.ny);
+    const int i = pi + g.ng, j = pj + g.ng;
+    double u[8];
+    cell_cons(g, S, i, j, u);
+    Kastaun k;
+    k.init(u, e);
+    Prim q;
+    const int st = c2p_single(k, atmo_rho, atmo_eps, q);
+    redo = st & C2P_REDONE;
+    const int64_t c = (int64_t)i * g.nyt + j;
+    P.p[0][c] = q.rho;
+    P.p[1][c] = q.v[0];
+    P.p[2][c] = q.v[1];
+    P.p[3][c] = q.v[2];
+    P.p[4][c] = q.p;
+    P.p[5][c] = q.eps;
+  }
+  const unsigned long long nr = __popcll(__ballot(redo));
+  if (lane0() && nr) atomicAdd(redo_count, nr);
+}
+
 __global__ __launch_bounds__(BLK) void k_prim_bc(G2 g, Pr2Out P) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (t >= (int64_t)g.nxt * g.nyt) return;
@@ -719,19 +747,19 @@ void fixup_dispatch(int lim, const G2& g, const St2& S, const Pr2& P, const srmh
   }
 }
 
-void sweeps_mixed(int lim, int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
-                  const Ws2& w, hipStream_t s) {
+void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
+                  const srmhd_eos& e, const Ws2& w, hipStream_t s) {
   // the C2PRec areas are unused in this form: they hold the fix-up lists
   unsigned* fx = (unsigned*)w.recx;
   unsigned* fy = (unsigned*)w.recy;
   k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)fx, 1);
   k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)fy, 1);
   tbegin(SRMHD_K_SWEEP_X_B, s);
-  sweep_mixed(0, lim, sol, g, S, P, e, w.diag + 0, w.Fx, fx, s);
+  sweep_mixed(0, lim, sol, r64, g, S, P, e, w.diag + 0, w.Fx, fx, s);
   tend(SRMHD_K_SWEEP_X_B, s);
   if (sol == SOLVER_HLLD) fixup_dispatch<0>(lim, g, S, P, e, fx, w.Fx, s);
   tbegin(SRMHD_K_SWEEP_Y_B, s);
-  sweep_mixed(1, lim, sol, g, S, P, e, w.diag + 1, w.Fy, fy, s);
+  sweep_mixed(1, lim, sol, r64, g, S, P, e, w.diag + 1, w.Fy, fy, s);
   tend(SRMHD_K_SWEEP_Y_B, s);
   if (sol == SOLVER_HLLD) fixup_dispatch<1>(lim, g, S, P, e, fy, w.Fy, s);
 }
@@ -748,10 +776,19 @@ void sweeps_dispatch(int lim, int sol, const G2& g, const St2& S, const Pr2& P,
   }
 }
 
-// con2prim of every cell of state S as one batch, then primitive BCs
+// con2prim of every cell of state S as one batch, then primitive BCs;
+// single: each cell its own batch (the mixed form)
 void c2p_cells(const G2& g, const St2& S, const srmhd_eos& e, double atmo_rho, double atmo_eps,
-               const Ws2& w, const Pr2Out& P, hipStream_t s) {
+               const Ws2& w, const Pr2Out& P, hipStream_t s, bool single = false) {
   const int64_t ncell = (int64_t)g.nxt * g.nyt;
+  if (single) {
+    tbegin(SRMHD_K_C2P_A, s);
+    k_c2p2d_single<<<blocks_for((int64_t)g.nx * g.ny), BLK, 0, s>>>(g, S, e, atmo_rho, atmo_eps,
+                                                                    P, w.words + 3);
+    tend(SRMHD_K_C2P_A, s);
+    k_prim_bc<<<blocks_for(ncell), BLK, 0, s>>>(g, P);
+    return;
+  }
   tbegin(SRMHD_K_C2P_A, s);
   k_c2p2d_a<<<blocks_for(ncell), BLK, 0, s>>>(g, S, e, w.recc, w.words + 0);
   tend(SRMHD_K_C2P_A, s);
@@ -1070,7 +1107,8 @@ int64_t srmhd2d_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s) 
 
 static bool scheme_ok(const srmhd_scheme2d* s) {
   return s && s->solver >= 0 && s->solver <= 2 && s->limiter >= 0 && s->limiter <= 5 &&
-         (s->precision == SRMHD_PREC_FP64 || s->precision == SRMHD_PREC_MIXED);
+         (s->precision == SRMHD_PREC_FP64 || s->precision == SRMHD_PREC_MIXED ||
+          s->precision == SRMHD_PREC_MIXED_R64);
 }
 
 int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_eos* eos,
@@ -1106,8 +1144,10 @@ int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_
   hipStream_t st = (hipStream_t)stream;
   k_zero_u64<<<1, 64, 0, st>>>((unsigned long long*)w.diag, N_WORDS);
   k_zero_u64<<<1, 64, 0, st>>>(w.words, 4);
-  if (sp->precision == SRMHD_PREC_MIXED) {
-    sweeps_mixed(sp->limiter, sp->solver, g, s1, pr, *eos, w, st);
+  const bool mixed = sp->precision != SRMHD_PREC_FP64;
+  if (mixed) {
+    sweeps_mixed(sp->limiter, sp->solver, sp->precision == SRMHD_PREC_MIXED_R64, g, s1, pr, *eos,
+                 w, st);
   } else {
     sweeps_dispatch(sp->limiter, sp->solver, g, s1, pr, *eos, w, st);
   }
@@ -1119,7 +1159,7 @@ int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_
   tbegin(SRMHD_K_UPDATE, st);
   k_update<<<blocks_for(ncor), BLK, 0, st>>>(g, s0, s1, w.Fx, w.Fy, w.Ez, a, b, c, dt_dev, o);
   tend(SRMHD_K_UPDATE, st);
-  c2p_cells(g, so, *eos, sp->atmo_rho, sp->atmo_eps, w, po, st);
+  c2p_cells(g, so, *eos, sp->atmo_rho, sp->atmo_eps, w, po, st, mixed);
   if (diag_xy) k_copy_diag<<<1, 64, 0, st>>>(w.diag, diag_xy, 2);
   return launch_status();
 }

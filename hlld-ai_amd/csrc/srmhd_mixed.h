@@ -9,11 +9,12 @@ inline int64_t n_faces_host(const G2& g, int dir) {
   return dir == 0 ? (int64_t)(g.nxt + 1) * g.nyt : (int64_t)g.nxt * (g.nyt + 1);
 }
 
-// One FP32 sweep along dir (0: x, 1: y) into the FP64 flux arrays F, with the
-// flux counters (and the |F_D| max of the upwind EMF) into *d, and the faces
-// to recompute in FP64 appended to fix (fix[0] = count, zeroed by the caller;
-// indices from fix[2]; room for every face).
-void sweep_mixed(int dir, int lim, int sol, const G2& g, const St2& S, const Pr2& P,
+// One mixed-precision sweep along dir (0: x, 1: y) into the FP64 flux arrays
+// F, with the flux counters (and the |F_D| max of the upwind EMF) into *d,
+// and the faces to recompute in FP64 appended to fix (fix[0] = count, zeroed
+// by the caller; indices from fix[2]; room for every face).  r64: the
+// reconstruction in FP64 (else in FP32).
+void sweep_mixed(int dir, int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
                  const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,
                  hipStream_t s);
 

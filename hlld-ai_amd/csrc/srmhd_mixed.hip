@@ -1,6 +1,8 @@
 // Mixed-precision sweeps of the 2D stage: reconstruction and the Riemann
-// solver in FP32, everything else of the stage (corner EMF, CT face-B
-// update, RK accumulation, con2prim) in the FP64 kernels.
+// solver's iterative work (HLLD's p* root and the HLL-state con2prim that
+// seeds it; HLLE / HLLC whole) in FP32, everything else of the stage (HLLD's
+// FP64 polish of p* and its flux assembly, corner EMF, CT face-B update, RK
+// accumulation, con2prim) in FP64.
 //
 // One kernel per sweep (no pass A: the FP32 HLL-state con2prim is per
 // element).  It reads the FP64 state, forms the reconstructed variables
@@ -23,9 +25,10 @@ namespace {
 // from the stencil's centre cell (formed in FP64) and the centre value is
 // added back in FP64: the FP32 arithmetic then only touches the small
 // differences, and the face value carries one FP32 rounding.
-template <int N, int LIM>
-__device__ void face_prims_f32(const G2& g, const St2& S, const Pr2& P, const f32::Eos& e, int i,
-                               int j, f32::Prim& L, f32::Prim& R) {
+template <int N, int LIM, bool RECON64>
+__device__ void face_prims_f32(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e64,
+                               const f32::Eos& e, int i, int j, f32::Prim& L, f32::Prim& R,
+                               Prim& L64, Prim& R64) {
   constexpr int w = stencil_halfwidth(LIM);
   constexpr int k0 = 3 - w, k1 = 4 + w;
   const int n = N == 0 ? g.nxt : g.nyt;
@@ -39,6 +42,10 @@ __device__ void face_prims_f32(const G2& g, const St2& S, const Pr2& P, const f3
   double ql[7], qr[7];   // rho, p, zx, zy, zz, B_t1, B_t2
   // deviation reconstruction of a[k0..k1] about a[3]
   auto recon = [&](double& l, double& r) {
+    if (RECON64) {   // the FP64 path's reconstruction
+      face_states<LIM>(a, f, n, l, r);
+      return;
+    }
     const double base = a[3];
 #pragma unroll
     for (int k = k0; k <= k1; ++k) q[k] = (float)(a[k] - base);
@@ -94,14 +101,25 @@ __device__ void face_prims_f32(const G2& g, const St2& S, const Pr2& P, const f3
   constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
   const float Bn = (float)(N == 0 ? S.p[6][(int64_t)i * g.nyt + j]
                                   : S.p[7][(int64_t)i * (g.nyt + 1) + j]);
+  const double Bn64 = N == 0 ? S.p[6][(int64_t)i * g.nyt + j] : S.p[7][(int64_t)i * (g.nyt + 1) + j];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     const double* x = s ? qr : ql;
     f32::Prim& Q = s ? R : L;
+    Prim& Q64 = s ? R64 : L64;
     const double fac = 1.0 / __builtin_sqrt(1.0 + (x[2] * x[2] + x[3] * x[3] + x[4] * x[4]));
-    Q.v[0] = (float)(x[2] * fac);
-    Q.v[1] = (float)(x[3] * fac);
-    Q.v[2] = (float)(x[4] * fac);
+    Q64.v[0] = x[2] * fac;
+    Q64.v[1] = x[3] * fac;
+    Q64.v[2] = x[4] * fac;
+    Q64.rho = fmax(x[0], 1e-12);
+    Q64.p = fmax(x[1], 1e-14);
+    Q64.eps = eos_eps(e64, Q64.p, Q64.rho);
+    Q64.B[N] = Bn64;
+    Q64.B[T1] = x[5];
+    Q64.B[T2] = x[6];
+    Q.v[0] = (float)Q64.v[0];
+    Q.v[1] = (float)Q64.v[1];
+    Q.v[2] = (float)Q64.v[2];
     Q.rho = fmaxf((float)x[0], 1e-12f);
     Q.p = fmaxf((float)x[1], 1e-14f);
     Q.eps = e.eps_of(Q.p, Q.rho);
@@ -111,9 +129,63 @@ __device__ void face_prims_f32(const G2& g, const St2& S, const Pr2& P, const f3
   }
 }
 
+// HLLD of the mixed form: the p* root is found in FP32 (the HLL-state
+// con2prim that seeds the bracket, the bracket and the secant: the iterative,
+// division-heavy part), then polished by FP64 secant steps on the FP64
+// residual from the FP64 face states, and the star states and the flux are
+// assembled in FP64 (hlld_assemble, the FP64 path's own code), so the flux
+// carries FP64 rounding.  Faces whose FP32 root does not converge, or whose
+// polish does not reach the FP64 tolerance within 8 steps, are flagged for
+// the full FP64 recomputation.
+// distance below which the FP64 wave-order test of a polished face is
+// treated as undecided (p* is polished to the FP64 tolerance, 1e-10 in the
+// residual; the flux assembly here is contracted, the FP64 path's is not)
+constexpr double WAVE_MARGIN64 = 1e-7;
+
+template <int N>
+__device__ int hlld_mixed(const f32::Prim& l, const f32::Prim& r, const f32::Eos& e,
+                          const Prim& L64, const Prim& R64, const srmhd_eos& e64, double F[8]) {
+  float ps32, p_hll32;
+  const bool conv32 = f32::hlld_pstar<N>(l, r, e, ps32, p_hll32);
+  HlldPrep<N> h;
+  h.run(L64, R64, e64);
+  StarR R;
+  hlld_rvec<N>(h, R);
+  double fH[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) fH[k] = hll_flux(h.fL[k], h.fR[k], h.uL[k], h.uR[k], h.cmin, h.cmax);
+  double x0 = (double)ps32 * (1.0 - 1e-6), x1 = (double)ps32;
+  double f0 = star_residual(R, x0), f1 = star_residual(R, x1);
+  bool conv = fabs(f1) < PSTAR_TOL;
+#pragma unroll 1
+  for (int it = 0; it < 8 && !conv; ++it) {
+    const double df = f1 - f0;
+    const double dx = (fabs(df) > 1e-30 * fabs(f1)) ? f1 * (x1 - x0) / df : 0.0;
+    const double x2 = fmax(x1 - dx, 1e-30);
+    const double f2 = star_residual(R, x2);
+    x0 = x1;
+    f0 = f1;
+    x1 = x2;
+    f1 = f2;
+    conv = fabs(f1) < PSTAR_TOL;
+  }
+  double ps, margin[2];
+  const int st = hlld_assemble<N>(h, R, fH, x1, conv32 && conv, (double)p_hll32, F,
+                                  (double*)nullptr, ps, margin);
+  // The wave-order test is decided by round-off where an Alfven wave meets
+  // its fast wave or the contact (e.g. a vanishing tangential field); such
+  // faces take the FP64 path's own arithmetic, so the HLLE-fallback switch
+  // is decided as the FP64 form decides it.
+  const bool bn_zero = st & FS_BN_ZERO;
+  const bool strong_bn = fmin(fabs(h.uL[CB + N]), fabs(h.uR[CB + N])) > f32::STRONG_BN;
+  const bool doubt = !bn_zero && (!(conv32 && conv) || !(margin[0] > WAVE_MARGIN64) ||
+                                  (strong_bn && !(margin[1] > WAVE_MARGIN64)));
+  return doubt ? (st | f32::FS_DOUBT) : st;
+}
+
 // fix: count + indices of the faces to recompute in FP64 (fix[0] = count,
 // indices from fix[2]); capacity >= the number of faces.
-template <int N, int LIM, int SOL>
+template <int N, int LIM, int SOL, bool R64>
 __global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, srmhd_eos e64,
                                                    srmhd_flux_diag* d, Flx F, unsigned* fix) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
@@ -125,61 +197,74 @@ __global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, srmhd_eos
     face_ij<N>(g, t, i, j);
     const f32::Eos e(e64);
     f32::Prim l, r;
-    face_prims_f32<N, LIM>(g, S, P, e, i, j, l, r);
-    float f[8];
-    if (SOL == SOLVER_HLLE) {
-      f32::hlle<N>(l, r, e, f);
-    } else if (SOL == SOLVER_HLLC) {
-      f32::hllc<N>(l, r, e, f);
-    } else {
-      st = f32::hlld<N>(l, r, e, f);
+    Prim l64, r64;
+    face_prims_f32<N, LIM, R64>(g, S, P, e64, e, i, j, l, r, l64, r64);
+    double f[8];
+    if (SOL == SOLVER_HLLD) {
+      st = hlld_mixed<N>(l, r, e, l64, r64, e64, f);
       if (st & f32::FS_DOUBT) fix[2 + atomicAdd(fix, 1u)] = (unsigned)t;
+    } else {
+      float f3[8];
+      if (SOL == SOLVER_HLLE) {
+        f32::hlle<N>(l, r, e, f3);
+      } else {
+        f32::hllc<N>(l, r, e, f3);
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) f[k] = (double)f3[k];
     }
-    F.p[0][t] = (double)f[CD];
-    F.p[1][t] = (double)f[CS];
-    F.p[2][t] = (double)f[CS + 1];
-    F.p[3][t] = (double)f[CS + 2];
-    F.p[4][t] = (double)f[CT];
-    F.p[5][t] = (double)f[CB + 2];
-    F.p[6][t] = (double)(N == 0 ? f[CB + 1] : f[CB + 0]);
-    fD = (double)f[CD];
+    F.p[0][t] = f[CD];
+    F.p[1][t] = f[CS];
+    F.p[2][t] = f[CS + 1];
+    F.p[3][t] = f[CS + 2];
+    F.p[4][t] = f[CT];
+    F.p[5][t] = f[CB + 2];
+    F.p[6][t] = N == 0 ? f[CB + 1] : f[CB + 0];
+    fD = f[CD];
   }
   diag_accumulate(d, active, st, 0, dbits(fabs(fD)));
 }
 
-template <int N, int LIM>
+template <int N, int LIM, bool R64>
 void launch_lim(int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
                 srmhd_flux_diag* d, const Flx& F, unsigned* fix, hipStream_t s) {
   const int nb = blocks_for(n_faces_host(g, N));
   switch (sol) {
-    case SOLVER_HLLE: k_sweep_f32<N, LIM, SOLVER_HLLE><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix); break;
-    case SOLVER_HLLC: k_sweep_f32<N, LIM, SOLVER_HLLC><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix); break;
-    default: k_sweep_f32<N, LIM, SOLVER_HLLD><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix);
+    case SOLVER_HLLE:
+      k_sweep_f32<N, LIM, SOLVER_HLLE, R64><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix);
+      break;
+    case SOLVER_HLLC:
+      k_sweep_f32<N, LIM, SOLVER_HLLC, R64><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix);
+      break;
+    default: k_sweep_f32<N, LIM, SOLVER_HLLD, R64><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix);
   }
 }
 
-template <int N>
+template <int N, bool R64>
 void launch_dir(int lim, int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
                 srmhd_flux_diag* d, const Flx& F, unsigned* fix, hipStream_t s) {
   switch (lim) {
-    case LIM_PCM: launch_lim<N, LIM_PCM>(sol, g, S, P, e, d, F, fix, s); break;
-    case LIM_MC: launch_lim<N, LIM_MC>(sol, g, S, P, e, d, F, fix, s); break;
-    case LIM_MINMOD: launch_lim<N, LIM_MINMOD>(sol, g, S, P, e, d, F, fix, s); break;
-    case LIM_WENO5Z: launch_lim<N, LIM_WENO5Z>(sol, g, S, P, e, d, F, fix, s); break;
-    case LIM_MP5: launch_lim<N, LIM_MP5>(sol, g, S, P, e, d, F, fix, s); break;
-    default: launch_lim<N, LIM_MP7>(sol, g, S, P, e, d, F, fix, s);
+    case LIM_PCM: launch_lim<N, LIM_PCM, R64>(sol, g, S, P, e, d, F, fix, s); break;
+    case LIM_MC: launch_lim<N, LIM_MC, R64>(sol, g, S, P, e, d, F, fix, s); break;
+    case LIM_MINMOD: launch_lim<N, LIM_MINMOD, R64>(sol, g, S, P, e, d, F, fix, s); break;
+    case LIM_WENO5Z: launch_lim<N, LIM_WENO5Z, R64>(sol, g, S, P, e, d, F, fix, s); break;
+    case LIM_MP5: launch_lim<N, LIM_MP5, R64>(sol, g, S, P, e, d, F, fix, s); break;
+    default: launch_lim<N, LIM_MP7, R64>(sol, g, S, P, e, d, F, fix, s);
   }
 }
 
 }  // namespace
 
-void sweep_mixed(int dir, int lim, int sol, const G2& g, const St2& S, const Pr2& P,
+void sweep_mixed(int dir, int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
                  const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,
                  hipStream_t s) {
-  if (dir == 0)
-    launch_dir<0>(lim, sol, g, S, P, e, d, F, fix, s);
-  else
-    launch_dir<1>(lim, sol, g, S, P, e, d, F, fix, s);
+  if (dir == 0) {
+    if (r64) launch_dir<0, true>(lim, sol, g, S, P, e, d, F, fix, s);
+    else launch_dir<0, false>(lim, sol, g, S, P, e, d, F, fix, s);
+  } else {
+    if (r64) launch_dir<1, true>(lim, sol, g, S, P, e, d, F, fix, s);
+    else launch_dir<1, false>(lim, sol, g, S, P, e, d, F, fix, s);
+  }
 }
 
 }  // namespace srmhd

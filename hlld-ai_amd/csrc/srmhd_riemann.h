@@ -17,25 +17,26 @@ constexpr double PSTAR_TOL = 1.0e-10;   // hlld.py:1080
 constexpr int PSTAR_MAXIT = 30;         // hlld.py:1327
 
 // HLL total-pressure estimate of hlle/hllc   hlld.py:1036-1042
-SRMHD_DEV double p_hll_sides(const Prim& L, const Prim& R, const srmhd_eos& e) {
-  double pl, pr, cs;
+template <class T>
+SRMHD_DEV T p_hll_sides(const PrimT<T>& L, const PrimT<T>& R, const srmhd_eos& e) {
+  T pl, pr, cs;
   eos_press_cs2(e, L.eps, L.rho, pl, cs);
   eos_press_cs2(e, R.eps, R.rho, pr, cs);
-  const double b2l = b_squared(L.v, L.B, lorentz_W(L.v));
-  const double b2r = b_squared(R.v, R.B, lorentz_W(R.v));
+  const T b2l = b_squared(L.v, L.B, lorentz_W(L.v));
+  const T b2r = b_squared(R.v, R.B, lorentz_W(R.v));
   return clo(0.5 * (pl + 0.5 * b2l + pr + 0.5 * b2r), 1e-30);
 }
 
 // hlld.py:1005-1044
-template <int N>
-SRMHD_DEV void hlle(const Prim& L, const Prim& R, const srmhd_eos& e, double F[8], double U[8],
-                    double& pstar) {
-  double uL[8], uR[8], fL[8], fR[8], cmax, cmin;
+template <int N, class T>
+SRMHD_DEV void hlle(const PrimT<T>& L, const PrimT<T>& R, const srmhd_eos& e, T F[8], T U[8],
+                    T& pstar) {
+  T uL[8], uR[8], fL[8], fR[8], cmax, cmin;
   srmhd_fluxes<N>(L, R, e, uL, uR, fL, fR, cmax, cmin);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const double fh = hll_flux(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
-    const double uh = hll_state(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
+    const T fh = hll_flux(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
+    const T uh = hll_state(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
     F[k] = (cmax <= 0.0) ? fR[k] : ((cmin <= 0.0) ? fL[k] : fh);
     U[k] = (cmin <= 0.0) ? uL[k] : ((cmax <= 0.0) ? uR[k] : uh);
   }
@@ -43,72 +44,72 @@ SRMHD_DEV void hlle(const Prim& L, const Prim& R, const srmhd_eos& e, double F[8
 }
 
 // Mignone-style HLLC   hlld.py:1549-1765
-template <int N>
-SRMHD_DEV void hllc(const Prim& L, const Prim& R, const srmhd_eos& e, double F[8], double U[8],
-                    double& pstar) {
+template <int N, class T>
+SRMHD_DEV void hllc(const PrimT<T>& L, const PrimT<T>& R, const srmhd_eos& e, T F[8], T U[8],
+                    T& pstar) {
   constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
-  double uL[8], uR[8], fL[8], fR[8], cmax, cmin;
+  T uL[8], uR[8], fL[8], fR[8], cmax, cmin;
   srmhd_fluxes<N>(L, R, e, uL, uR, fL, fR, cmax, cmin);
-  const double SL = -cmin, SR = cmax;
-  double fH[8], uH[8];
+  const T SL = -cmin, SR = cmax;
+  T fH[8], uH[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     fH[k] = hll_flux(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
     uH[k] = hll_state(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
   }
-  const double Bn = uL[CB + N];
+  const T Bn = uL[CB + N];
   const bool hasB = fabs(Bn) > 1e-15;
-  const double bt1 = uH[CB + T1], bt2 = uH[CB + T2];
-  const double fb1 = fH[CB + T1], fb2 = fH[CB + T2];
-  const double cross = hasB ? bt1 * fb1 + bt2 * fb2 : 0.0;
-  const double btan2 = hasB ? sq(bt1) + sq(bt2) : 0.0;
-  const double fbtan2 = hasB ? sq(fb1) + sq(fb2) : 0.0;
-  const double E = uH[CT] + uH[CD];
-  const double fE = fH[CT] + fH[CD];
-  const double qa = fE - cross;
-  const double qb = -fH[CS + N] - E + btan2 + fbtan2;
-  const double qc = uH[CS + N] - cross;
-  const double disc = csqrt(clo(qb * qb - 4.0 * qa * qc, 0.0));
-  const double vq = sdiv(-0.5 * (qb + disc), qa);
-  const double vlin = sdiv(-qc, qb);
-  const double vc = (fabs(qa) < 1e-15) ? ((fabs(qb) < 1e-15) ? 0.0 : vlin) : vq;
-  const double vt1 = hasB ? sdiv(uH[CB + T1] * vc - fH[CB + T1], Bn) : 0.0;
-  const double vt2 = hasB ? sdiv(uH[CB + T2] * vc - fH[CB + T2], Bn) : 0.0;
-  const double v2 = sq(vc) + sq(vt1) + sq(vt2);
-  const double g = 1.0 / csqrt(1.0 - chi_(v2, 1.0 - 1e-10));
-  const double vB = hasB ? vc * uH[CB + N] + vt1 * uH[CB + T1] + vt2 * uH[CB + T2] : 0.0;
-  const double pB = fH[CS + N] + sq(Bn / g) - vc * (fH[CT] + fH[CD] - Bn * vB);
-  const double p0 = fH[CS + N] - vc * (fH[CT] + fH[CD]);
-  const double ps = hasB ? pB : p0;
+  const T bt1 = uH[CB + T1], bt2 = uH[CB + T2];
+  const T fb1 = fH[CB + T1], fb2 = fH[CB + T2];
+  const T cross = hasB ? T(bt1 * fb1 + bt2 * fb2) : T(0.0);
+  const T btan2 = hasB ? T(sq(bt1) + sq(bt2)) : T(0.0);
+  const T fbtan2 = hasB ? T(sq(fb1) + sq(fb2)) : T(0.0);
+  const T E = uH[CT] + uH[CD];
+  const T fE = fH[CT] + fH[CD];
+  const T qa = fE - cross;
+  const T qb = -fH[CS + N] - E + btan2 + fbtan2;
+  const T qc = uH[CS + N] - cross;
+  const T disc = csqrt(clo(qb * qb - 4.0 * qa * qc, 0.0));
+  const T vq = sdiv(-0.5 * (qb + disc), qa);
+  const T vlin = sdiv(-qc, qb);
+  const T vc = (fabs(qa) < 1e-15) ? ((fabs(qb) < 1e-15) ? T(0.0) : vlin) : vq;
+  const T vt1 = hasB ? T(sdiv(uH[CB + T1] * vc - fH[CB + T1], Bn)) : T(0.0);
+  const T vt2 = hasB ? T(sdiv(uH[CB + T2] * vc - fH[CB + T2], Bn)) : T(0.0);
+  const T v2 = sq(vc) + sq(vt1) + sq(vt2);
+  const T g = 1.0 / csqrt(1.0 - chi_(v2, 1.0 - 1e-10));
+  const T vB = hasB ? T(vc * uH[CB + N] + vt1 * uH[CB + T1] + vt2 * uH[CB + T2]) : T(0.0);
+  const T pB = fH[CS + N] + sq(Bn / g) - vc * (fH[CT] + fH[CD] - Bn * vB);
+  const T p0 = fH[CS + N] - vc * (fH[CT] + fH[CD]);
+  const T ps = hasB ? pB : p0;
 
-  double ucL[8], ucR[8];
+  T ucL[8], ucR[8];
 #pragma unroll
   for (int side = 0; side < 2; ++side) {
-    const double* u = side ? uR : uL;
-    const double* f = side ? fR : fL;
-    double* s = side ? ucR : ucL;
-    const double S = side ? SR : SL;
-    const double vd = side ? R.v[N] : L.v[N];
-    const double den = S - vc;
+    const T* u = side ? uR : uL;
+    const T* f = side ? fR : fL;
+    T* s = side ? ucR : ucL;
+    const T S = side ? SR : SL;
+    const T vd = side ? R.v[N] : L.v[N];
+    const T den = S - vc;
     s[CB + N] = Bn;
     s[CB + T1] = uH[CB + T1];
     s[CB + T2] = uH[CB + T2];
     s[CD] = sdiv(u[CD] * (S - vd), den);
     s[CT] = sdiv(S * u[CT] - f[CT] + ps * vc - vB * Bn, den);
     s[CS + N] = (s[CT] + s[CD] + ps) * vc - vB * Bn;
-    const double w1 = sdiv(-Bn * (uH[CB + T1] / (g * g) + vB * vt1) - f[CS + T1] + S * u[CS + T1], den);
-    const double n1 = sdiv(u[CS + T1] * (S - vd), den);
+    const T w1 = sdiv(-Bn * (uH[CB + T1] / (g * g) + vB * vt1) - f[CS + T1] + S * u[CS + T1], den);
+    const T n1 = sdiv(u[CS + T1] * (S - vd), den);
     s[CS + T1] = hasB ? w1 : n1;
-    const double w2 = sdiv(-Bn * (uH[CB + T2] / (g * g) + vB * vt2) - f[CS + T2] + S * u[CS + T2], den);
-    const double n2 = sdiv(u[CS + T2] * (S - vd), den);
+    const T w2 = sdiv(-Bn * (uH[CB + T2] / (g * g) + vB * vt2) - f[CS + T2] + S * u[CS + T2], den);
+    const T n2 = sdiv(u[CS + T2] * (S - vd), den);
     s[CS + T2] = hasB ? w2 : n2;
   }
   const bool degen = (vc <= SL) || (vc >= SR) || (v2 >= 1.0);
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const double fsL = fL[k] + SL * (ucL[k] - uL[k]);
-    const double fsR = fR[k] + SR * (ucR[k] - uR[k]);
-    double f = fH[k], u = uH[k];
+    const T fsL = fL[k] + SL * (ucL[k] - uL[k]);
+    const T fsR = fR[k] + SR * (ucR[k] - uR[k]);
+    T f = fH[k], u = uH[k];
     if (0.0 <= SL) { f = fL[k]; u = uL[k]; }
     if ((SL < 0.0) && (0.0 < vc)) { f = fsL; u = ucL[k]; }
     if ((vc <= 0.0) && (0.0 < SR)) { f = fsR; u = ucR[k]; }
@@ -120,7 +121,7 @@ SRMHD_DEV void hllc(const Prim& L, const Prim& R, const srmhd_eos& e, double F[8
   // non-finite safety, per key on (F, U) jointly   hlld.py:1750-1754
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    const bool bad = !(fabs(F[k]) <= DBL_MAX) || !(fabs(U[k]) <= DBL_MAX);
+    const bool bad = !(fabs(val(F[k])) <= DBL_MAX) || !(fabs(val(U[k])) <= DBL_MAX);
     if (bad) { F[k] = fH[k]; U[k] = uH[k]; }
   }
   pstar = p_hll_sides(L, R, e);
@@ -129,58 +130,64 @@ SRMHD_DEV void hllc(const Prim& L, const Prim& R, const srmhd_eos& e, double F[8
 // ── HLLD (hlld.py:366-605, 669-746, 1202-1543) ──────────────────────────────
 
 // R-vectors of HLLDComputation._build_R on the total-energy form
-struct StarR {
-  double RSn[2], RS1[2], RS2[2], RBn[2], RB1[2], RB2[2], RT[2], RD[2];
-  double Sg[2], Bn[2], cm, cx;
+template <class T>
+struct StarRT {
+  T RSn[2], RS1[2], RS2[2], RBn[2], RB1[2], RB2[2], RT[2], RD[2];
+  double Sg[2];
+  T Bn[2], cm, cx;
 };
+using StarR = StarRT<double>;
 
 // compute_all_variables at a trial total pressure p   hlld.py:431-583
-struct StarOut {
-  double va[2], v1[2], v2[2], B1[2], B2[2], sq[2], Da[2], Ta[2], Sa[2], S1[2], S2[2];
-  double la[2], K1L, K2L, Bc1, Bc2, KL2, KR2, KBL, KBR;
+template <class T>
+struct StarOutT {
+  T va[2], v1[2], v2[2], B1[2], B2[2], sq[2], Da[2], Ta[2], Sa[2], S1[2], S2[2];
+  T la[2], K1L, K2L, Bc1, Bc2, KL2, KR2, KBL, KBR;
 };
+using StarOut = StarOutT<double>;
 
-SRMHD_DEV void star_solve(const StarR& R, double p, StarOut& o) {
-  const double cm = R.cm, cx = R.cx;
-  const double BnL = R.Bn[0], BnR = R.Bn[1];
-  const double RSnL = R.RSn[0], RSnR = R.RSn[1], RS1L = R.RS1[0], RS1R = R.RS1[1];
-  const double RS2L = R.RS2[0], RS2R = R.RS2[1], RB1L = R.RB1[0], RB1R = R.RB1[1];
-  const double RB2L = R.RB2[0], RB2R = R.RB2[1], RTL = R.RT[0], RTR = R.RT[1];
+template <class T>
+SRMHD_DEV void star_solve(const StarRT<T>& R, const T& p, StarOutT<T>& o) {
+  const T cm = R.cm, cx = R.cx;
+  const T BnL = R.Bn[0], BnR = R.Bn[1];
+  const T RSnL = R.RSn[0], RSnR = R.RSn[1], RS1L = R.RS1[0], RS1R = R.RS1[1];
+  const T RS2L = R.RS2[0], RS2R = R.RS2[1], RB1L = R.RB1[0], RB1R = R.RB1[1];
+  const T RB2L = R.RB2[0], RB2R = R.RB2[1], RTL = R.RT[0], RTR = R.RT[1];
 
-  const double AL = RSnL - (-cm) * RTL + p * (1.0 - cm * cm);
-  const double AR = RSnR - cx * RTR + p * (1.0 - cx * cx);
-  const double GL = sq(RB1L) + sq(RB2L);
-  const double GR = sq(RB1R) + sq(RB2R);
-  const double CL = RS1L * RB1L + RS2L * RB2L;
-  const double CR = RS1R * RB1R + RS2R * RB2R;
-  const double QL = -AL - GL + sq(BnL) * (1.0 - cm * cm);
-  const double QR = -AR - GR + sq(BnR) * (1.0 - cx * cx);
-  const double XL = BnL * (AL * (-cm) * BnL + CL) - (AL + GL) * ((-cm) * p + RTL);
-  const double XR = BnR * (AR * cx * BnR + CR) - (AR + GR) * (cx * p + RTR);
+  const T AL = RSnL - (-cm) * RTL + p * (1.0 - cm * cm);
+  const T AR = RSnR - cx * RTR + p * (1.0 - cx * cx);
+  const T GL = sq(RB1L) + sq(RB2L);
+  const T GR = sq(RB1R) + sq(RB2R);
+  const T CL = RS1L * RB1L + RS2L * RB2L;
+  const T CR = RS1R * RB1R + RS2R * RB2R;
+  const T QL = -AL - GL + sq(BnL) * (1.0 - cm * cm);
+  const T QR = -AR - GR + sq(BnR) * (1.0 - cx * cx);
+  const T XL = BnL * (AL * (-cm) * BnL + CL) - (AL + GL) * ((-cm) * p + RTL);
+  const T XR = BnR * (AR * cx * BnR + CR) - (AR + GR) * (cx * p + RTR);
 
-  const double vaL = sdiv(BnL * (AL * BnL + (-cm) * CL) - (AL + GL) * (p + RSnL), XL);
-  const double vaR = sdiv(BnR * (AR * BnR + cx * CR) - (AR + GR) * (p + RSnR), XR);
-  const double v1L = sdiv(QL * RS1L + RB1L * (CL + BnL * ((-cm) * RSnL - RTL)), XL);
-  const double v1R = sdiv(QR * RS1R + RB1R * (CR + BnR * (cx * RSnR - RTR)), XR);
-  const double v2L = sdiv(QL * RS2L + RB2L * (CL + BnL * ((-cm) * RSnL - RTL)), XL);
-  const double v2R = sdiv(QR * RS2R + RB2R * (CR + BnR * (cx * RSnR - RTR)), XR);
+  const T vaL = sdiv(BnL * (AL * BnL + (-cm) * CL) - (AL + GL) * (p + RSnL), XL);
+  const T vaR = sdiv(BnR * (AR * BnR + cx * CR) - (AR + GR) * (p + RSnR), XR);
+  const T v1L = sdiv(QL * RS1L + RB1L * (CL + BnL * ((-cm) * RSnL - RTL)), XL);
+  const T v1R = sdiv(QR * RS1R + RB1R * (CR + BnR * (cx * RSnR - RTR)), XR);
+  const T v2L = sdiv(QL * RS2L + RB2L * (CL + BnL * ((-cm) * RSnL - RTL)), XL);
+  const T v2R = sdiv(QR * RS2R + RB2R * (CR + BnR * (cx * RSnR - RTR)), XR);
 
-  const double dLa = -cm - vaL, dRa = cx - vaR;
-  const double B1L = sdiv(RB1L - BnL * v1L, dLa);
-  const double B1R = sdiv(RB1R - BnR * v1R, dRa);
-  const double B2L = sdiv(RB2L - BnL * v2L, dLa);
-  const double B2R = sdiv(RB2R - BnR * v2R, dRa);
+  const T dLa = -cm - vaL, dRa = cx - vaR;
+  const T B1L = sdiv(RB1L - BnL * v1L, dLa);
+  const T B1R = sdiv(RB1R - BnR * v1R, dRa);
+  const T B2L = sdiv(RB2L - BnL * v2L, dLa);
+  const T B2R = sdiv(RB2R - BnR * v2R, dRa);
 
-  const double vRSL = vaL * RSnL + v1L * RS1L + v2L * RS2L;
-  const double vRSR = vaR * RSnR + v1R * RS1R + v2R * RS2R;
-  const double waL = p + sdiv(RTL - vRSL, dLa);
-  const double waR = p + sdiv(RTR - vRSR, dRa);
-  const double DaL = sdiv(R.RD[0], dLa);
-  const double DaR = sdiv(R.RD[1], dRa);
-  const double vBL = vaL * BnL + v1L * B1L + v2L * B2L;
-  const double vBR = vaR * BnR + v1R * B1R + v2R * B2R;
-  const double TaL = sdiv(RTL + p * vaL - vBL * BnL, dLa) - DaL;
-  const double TaR = sdiv(RTR + p * vaR - vBR * BnR, dRa) - DaR;
+  const T vRSL = vaL * RSnL + v1L * RS1L + v2L * RS2L;
+  const T vRSR = vaR * RSnR + v1R * RS1R + v2R * RS2R;
+  const T waL = p + sdiv(RTL - vRSL, dLa);
+  const T waR = p + sdiv(RTR - vRSR, dRa);
+  const T DaL = sdiv(R.RD[0], dLa);
+  const T DaR = sdiv(R.RD[1], dRa);
+  const T vBL = vaL * BnL + v1L * B1L + v2L * B2L;
+  const T vBR = vaR * BnR + v1R * B1R + v2R * B2R;
+  const T TaL = sdiv(RTL + p * vaL - vBL * BnL, dLa) - DaL;
+  const T TaR = sdiv(RTR + p * vaR - vBR * BnR, dRa) - DaR;
   o.Sa[0] = (TaL + DaL + p) * vaL - vBL * BnL;
   o.Sa[1] = (TaR + DaR + p) * vaR - vBR * BnR;
   o.S1[0] = (TaL + DaL + p) * v1L - vBL * B1L;
@@ -188,22 +195,22 @@ SRMHD_DEV void star_solve(const StarR& R, double p, StarOut& o) {
   o.S2[0] = (TaL + DaL + p) * v2L - vBL * B2L;
   o.S2[1] = (TaR + DaR + p) * v2R - vBR * B2R;
 
-  const double sqL = csqrt(clo(waL, 0.0));
-  const double sqR = csqrt(clo(waR, 0.0));
-  const double SgL = R.Sg[0], SgR = R.Sg[1];
-  const double dL = (-cm) * p + RTL - BnL * SgL * sqL;
-  const double dR = cx * p + RTR + BnR * SgR * sqR;
-  const double KnL = sdiv(RSnL + p - R.RBn[0] * SgL * sqL, dL);
-  const double KnR = sdiv(RSnR + p + R.RBn[1] * SgR * sqR, dR);
-  const double K1L = sdiv(RS1L - RB1L * SgL * sqL, dL);
-  const double K1R = sdiv(RS1R + RB1R * SgR * sqR, dR);
-  const double K2L = sdiv(RS2L - RB2L * SgL * sqL, dL);
-  const double K2R = sdiv(RS2R + RB2R * SgR * sqR, dR);
+  const T sqL = csqrt(clo(waL, 0.0));
+  const T sqR = csqrt(clo(waR, 0.0));
+  const double SgL = R.Sg[0], SgR = R.Sg[1];  // signs: no tangent
+  const T dL = (-cm) * p + RTL - BnL * SgL * sqL;
+  const T dR = cx * p + RTR + BnR * SgR * sqR;
+  const T KnL = sdiv(RSnL + p - R.RBn[0] * SgL * sqL, dL);
+  const T KnR = sdiv(RSnR + p + R.RBn[1] * SgR * sqR, dR);
+  const T K1L = sdiv(RS1L - RB1L * SgL * sqL, dL);
+  const T K1R = sdiv(RS1R + RB1R * SgR * sqR, dR);
+  const T K2L = sdiv(RS2L - RB2L * SgL * sqL, dL);
+  const T K2R = sdiv(RS2R + RB2R * SgR * sqR, dR);
 
-  const double dll = KnR - KnL;
-  const double Bc1 = sdiv((B1R * (KnR - vaR) + BnR * v1R) - (B1L * (KnL - vaL) + BnL * v1L), dll);
-  const double Bc2 = sdiv((B2R * (KnR - vaR) + BnR * v2R) - (B2L * (KnL - vaL) + BnL * v2L), dll);
-  const double Bcn = BnR;
+  const T dll = KnR - KnL;
+  const T Bc1 = sdiv((B1R * (KnR - vaR) + BnR * v1R) - (B1L * (KnL - vaL) + BnL * v1L), dll);
+  const T Bc2 = sdiv((B2R * (KnR - vaR) + BnR * v2R) - (B2L * (KnL - vaL) + BnL * v2L), dll);
+  const T Bcn = BnR;
 
   o.va[0] = vaL; o.va[1] = vaR;
   o.v1[0] = v1L; o.v1[1] = v1R;
@@ -223,19 +230,20 @@ SRMHD_DEV void star_solve(const StarR& R, double p, StarOut& o) {
 }
 
 // HLLDComputation.residual   hlld.py:586-602
-SRMHD_DEV double star_residual(const StarR& R, double p) {
-  StarOut o;
+template <class T>
+SRMHD_DEV T star_residual(const StarRT<T>& R, const T& p) {
+  StarOutT<T> o;
   star_solve(R, p, o);
-  const double dR = R.Sg[1] * o.sq[1] - o.KBR;
-  const double dL = R.Sg[0] * o.sq[0] + o.KBL;
+  const T dR = R.Sg[1] * o.sq[1] - o.KBR;
+  const T dL = R.Sg[0] * o.sq[0] + o.KBL;
   return (o.la[0] - o.la[1]) + R.Bn[0] * (sdiv(1.0 - o.KR2, dR) + sdiv(1.0 - o.KL2, dL));
 }
 
 // Everything of hlld_flux that precedes the con2prim of the HLL state.
-template <int N>
+template <int N, class T = double>
 struct HlldPrep {
-  double uL[8], uR[8], fL[8], fR[8], cmax, cmin, uH[8];
-  SRMHD_DEV void run(const Prim& L, const Prim& R, const srmhd_eos& e) {
+  T uL[8], uR[8], fL[8], fR[8], cmax, cmin, uH[8];
+  SRMHD_DEV void run(const PrimT<T>& L, const PrimT<T>& R, const srmhd_eos& e) {
     srmhd_fluxes<N>(L, R, e, uL, uR, fL, fR, cmax, cmin);
 #pragma unroll
     for (int k = 0; k < 8; ++k) uH[k] = hll_state(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
@@ -243,52 +251,25 @@ struct HlldPrep {
 };
 
 // p_hll from the con2prim of the HLL state   hlld.py:1255-1263
-SRMHD_DEV double p_hll_from_prim(const Prim& P, const srmhd_eos& e) {
-  double p, cs;
+template <class T>
+SRMHD_DEV T p_hll_from_prim(const PrimT<T>& P, const srmhd_eos& e) {
+  T p, cs;
   eos_press_cs2(e, P.eps, P.rho, p, cs);
   return p + 0.5 * b_squared(P.v, P.B, lorentz_W(P.v));
 }
 
-// The rest of hlld_flux given p_hll   hlld.py:1265-1534.  Writes F (and U
-// when non-null), p*, and returns the face status bits; *iters gets the
-// secant iteration count (iters_used).
-template <int N>
-SRMHD_DEV int hlld_finish(const HlldPrep<N>& h, const Prim& L, const Prim& Rp, double p_hll,
-                          double F[8], double* U, double& pstar, int& iters) {
+// R-vectors on the total-energy form (E = tau + D)   hlld.py:401-428
+template <int N, class T>
+SRMHD_DEV void hlld_rvec(const HlldPrep<N, T>& h, StarRT<T>& R) {
   constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
-  const double cmin = h.cmin, cmax = h.cmax;
-  double fH[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) fH[k] = hll_flux(h.fL[k], h.fR[k], h.uL[k], h.uR[k], cmin, cmax);
-  const double* uH = h.uH;
-
-  // p0 from the Bn = 0 quadratic   hlld.py:1265-1287
-  const double a = fH[CT] + fH[CD];
-  const double b = -fH[CS + N] - uH[CT] - uH[CD];
-  const double c = uH[CS + N];
-  const double disc = clo(b * b - 4.0 * a * c, 0.0);
-  const bool big = fabs(a) > 1e-14;
-  const double sa = big ? a : 1.0;
-  const double sb = big ? b : 1.0;
-  double vs = big ? (-b - csqrt(disc)) / (2.0 * sa) : -c / sb;
-  vs = chi_(clo(vs, -(1.0 - 1e-10)), 1.0 - 1e-10);
-  const double p0 = clo(fH[CS + N] - (fH[CT] + fH[CD]) * vs, 1e-30);
-
-  const double BnL = h.uL[CB + N], BnR = h.uR[CB + N];
-  const bool use_p0 = (0.5 * (BnL * BnL + BnR * BnR) / p_hll) < 0.1;
-  double plo = use_p0 ? p0 * 0.99 : p_hll * 0.99;
-  double phi = use_p0 ? p0 * 1.01 : p_hll * 1.01;
-  const bool bn_zero = (fabs(BnL) < 1e-14) && (fabs(BnR) < 1e-14);
-
-  // R-vectors on the total-energy form (E = tau + D)   hlld.py:401-428
-  StarR R;
+  const T cmin = h.cmin, cmax = h.cmax;
   R.cm = cmin;
   R.cx = cmax;
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
-    const double* u = s ? h.uR : h.uL;
-    const double* f = s ? h.fR : h.fL;
-    const double uE = u[CT] + u[CD], fE = f[CT] + f[CD];
+    const T* u = s ? h.uR : h.uL;
+    const T* f = s ? h.fR : h.fL;
+    const T uE = u[CT] + u[CD], fE = f[CT] + f[CD];
     if (s == 0) {
       R.RSn[0] = -cmin * u[CS + N] - f[CS + N];
       R.RS1[0] = -cmin * u[CS + T1] - f[CS + T1];
@@ -311,6 +292,124 @@ SRMHD_DEV int hlld_finish(const HlldPrep<N>& h, const Prim& L, const Prim& Rp, d
     R.Sg[s] = tsign(u[CB + N]);
     R.Bn[s] = u[CB + N];
   }
+}
+
+// The HLLD flux at total pressure ps (the Alfven and contact states, the
+// region at xi = 0, the HLLE fallback)   hlld.py:1360-1534.  conv: the p*
+// root finder's convergence flag.  Writes F (and U when non-null) and
+// returns the face status bits; pstar gets ps, or the fallback sentinel
+// -|p_hll| - 1e-10.
+template <int N, class T>
+SRMHD_DEV int hlld_assemble(const HlldPrep<N, T>& h, const StarRT<T>& R, const T fH[8],
+                            const T& ps, bool conv, double p_hll, T F[8], T* U, T& pstar,
+                            double* margins = nullptr) {
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+  const T cmin = h.cmin, cmax = h.cmax;
+  const T* uH = h.uH;
+  const T BnL = h.uL[CB + N], BnR = h.uR[CB + N];
+  const bool bn_zero = (fabs(BnL) < 1e-14) && (fabs(BnR) < 1e-14);
+
+  StarOutT<T> o;
+  star_solve(R, ps, o);
+
+  // Alfven (a) states in the tau form, their fluxes   hlld.py:1360-1385
+  T uaL[8], uaR[8], faL[8], faR[8];
+  uaL[CB + N] = BnL; uaL[CB + T1] = o.B1[0]; uaL[CB + T2] = o.B2[0];
+  uaL[CD] = o.Da[0]; uaL[CT] = o.Ta[0];
+  uaL[CS + N] = o.Sa[0]; uaL[CS + T1] = o.S1[0]; uaL[CS + T2] = o.S2[0];
+  uaR[CB + N] = BnR; uaR[CB + T1] = o.B1[1]; uaR[CB + T2] = o.B2[1];
+  uaR[CD] = o.Da[1]; uaR[CT] = o.Ta[1];
+  uaR[CS + N] = o.Sa[1]; uaR[CS + T1] = o.S1[1]; uaR[CS + T2] = o.S2[1];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    faL[k] = h.fL[k] + (-cmin) * (uaL[k] - h.uL[k]);
+    faR[k] = h.fR[k] + cmax * (uaR[k] - h.uR[k]);
+  }
+
+  // contact velocity and states   hlld.py:1396-1445
+  const T fac = sdiv(1.0 - o.KL2, -R.Sg[0] * o.sq[0] - o.KBL);
+  const T laL = o.la[0], laR = o.la[1];
+  const T vc = laL - BnR * fac;
+  const T vt1 = o.K1L - o.Bc1 * fac;
+  const T vt2 = o.K2L - o.Bc2 * fac;
+  const T vBc = vc * BnR + vt1 * o.Bc1 + vt2 * o.Bc2;
+  T ucL[8], ucR[8];
+  ucL[CB + N] = BnR; ucL[CB + T1] = o.Bc1; ucL[CB + T2] = o.Bc2;
+  ucR[CB + N] = BnR; ucR[CB + T1] = o.Bc1; ucR[CB + T2] = o.Bc2;
+  ucL[CD] = uaL[CD] * sdiv(laL - o.va[0], laL - vc);
+  ucR[CD] = uaR[CD] * sdiv(laR - o.va[1], laR - vc);
+  ucL[CT] = sdiv(laL * uaL[CT] - faL[CT] + ps * vc - vBc * BnR, laL - vc);
+  ucR[CT] = sdiv(laR * uaR[CT] - faR[CT] + ps * vc - vBc * BnR, laR - vc);
+  {
+    const T gl = ucL[CT] + ps + ucL[CD];
+    const T gr = ucR[CT] + ps + ucR[CD];
+    ucL[CS + N] = gl * vc - vBc * BnR;
+    ucL[CS + T1] = gl * vt1 - vBc * o.Bc1;
+    ucL[CS + T2] = gl * vt2 - vBc * o.Bc2;
+    ucR[CS + N] = gr * vc - vBc * BnR;
+    ucR[CS + T1] = gr * vt1 - vBc * o.Bc1;
+    ucR[CS + T2] = gr * vt2 - vBc * o.Bc2;
+  }
+
+  const bool wave_bad = (o.va[1] > cmax) || (o.va[0] < -cmin) || (vc < laL) || (vc > laR);
+  const bool failed = !conv || wave_bad || bn_zero;
+  if (margins) {   // distances of the wave-order tests from flipping
+    margins[0] = tmin(fabs(val(cmax) - val(o.va[1])), fabs(val(o.va[0]) + val(cmin)));
+    margins[1] = tmin(fabs(val(vc) - val(laL)), fabs(val(laR) - val(vc)));
+  }
+  // region at xi = 0   hlld.py:1448-1480
+  const bool mL = 0.0 <= -cmin, maL = (-cmin < 0.0) && (0.0 <= laL);
+  const bool mcL = (laL < 0.0) && (0.0 < vc), mcR = (vc <= 0.0) && (0.0 < laR);
+  const bool maR = (laR <= 0.0) && (0.0 < cmax), mR = 0.0 >= cmax;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    T f = fH[k], u = uH[k];
+    if (mL) { f = h.fL[k]; u = h.uL[k]; }
+    if (maL) { f = faL[k]; u = uaL[k]; }
+    if (mcL) { f = faL[k] + laL * (ucL[k] - uaL[k]); u = ucL[k]; }
+    if (mcR) { f = faR[k] + laR * (ucR[k] - uaR[k]); u = ucR[k]; }
+    if (maR) { f = faR[k]; u = uaR[k]; }
+    if (mR) { f = h.fR[k]; u = h.uR[k]; }
+    if (failed) { f = fH[k]; u = uH[k]; }
+    F[k] = f;
+    if (U) U[k] = u;
+  }
+  pstar = failed ? T(-fabs(p_hll) - 1e-10) : ps;
+  return (conv ? 0 : FS_NOT_CONVERGED) | (wave_bad ? FS_WAVE_ORDER : 0) |
+         (bn_zero ? FS_BN_ZERO : 0) | (failed ? FS_FALLBACK : 0);
+}
+
+// The rest of hlld_flux given p_hll   hlld.py:1265-1534.  Writes F (and U
+// when non-null), p*, and returns the face status bits; *iters gets the
+// secant iteration count (iters_used).
+template <int N>
+SRMHD_DEV int hlld_finish(const HlldPrep<N>& h, const Prim& L, const Prim& Rp, double p_hll,
+                          double F[8], double* U, double& pstar, int& iters) {
+  const double cmin = h.cmin, cmax = h.cmax;
+  double fH[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) fH[k] = hll_flux(h.fL[k], h.fR[k], h.uL[k], h.uR[k], cmin, cmax);
+  const double* uH = h.uH;
+
+  // p0 from the Bn = 0 quadratic   hlld.py:1265-1287
+  const double a = fH[CT] + fH[CD];
+  const double b = -fH[CS + N] - uH[CT] - uH[CD];
+  const double c = uH[CS + N];
+  const double disc = clo(b * b - 4.0 * a * c, 0.0);
+  const bool big = fabs(a) > 1e-14;
+  const double sa = big ? a : 1.0;
+  const double sb = big ? b : 1.0;
+  double vs = big ? (-b - csqrt(disc)) / (2.0 * sa) : -c / sb;
+  vs = chi_(clo(vs, -(1.0 - 1e-10)), 1.0 - 1e-10);
+  const double p0 = clo(fH[CS + N] - (fH[CT] + fH[CD]) * vs, 1e-30);
+
+  const double BnL = h.uL[CB + N], BnR = h.uR[CB + N];
+  const bool use_p0 = (0.5 * (BnL * BnL + BnR * BnR) / p_hll) < 0.1;
+  double plo = use_p0 ? p0 * 0.99 : p_hll * 0.99;
+  double phi = use_p0 ? p0 * 1.01 : p_hll * 1.01;
+
+  StarR R;
+  hlld_rvec<N>(h, R);
 
   // bracket expansion   hlld.py:1311-1321
   double flo = star_residual(R, plo), fhi = star_residual(R, phi);
@@ -350,73 +449,7 @@ SRMHD_DEV int hlld_finish(const HlldPrep<N>& h, const Prim& L, const Prim& Rp, d
     used = PSTAR_MAXIT;
   }
   iters = used;
-  double ps = x1;
-
-  StarOut o;
-  star_solve(R, ps, o);
-
-  // Alfven (a) states in the tau form, their fluxes   hlld.py:1360-1385
-  double uaL[8], uaR[8], faL[8], faR[8];
-  uaL[CB + N] = BnL; uaL[CB + T1] = o.B1[0]; uaL[CB + T2] = o.B2[0];
-  uaL[CD] = o.Da[0]; uaL[CT] = o.Ta[0];
-  uaL[CS + N] = o.Sa[0]; uaL[CS + T1] = o.S1[0]; uaL[CS + T2] = o.S2[0];
-  uaR[CB + N] = BnR; uaR[CB + T1] = o.B1[1]; uaR[CB + T2] = o.B2[1];
-  uaR[CD] = o.Da[1]; uaR[CT] = o.Ta[1];
-  uaR[CS + N] = o.Sa[1]; uaR[CS + T1] = o.S1[1]; uaR[CS + T2] = o.S2[1];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    faL[k] = h.fL[k] + (-cmin) * (uaL[k] - h.uL[k]);
-    faR[k] = h.fR[k] + cmax * (uaR[k] - h.uR[k]);
-  }
-
-  // contact velocity and states   hlld.py:1396-1445
-  const double fac = sdiv(1.0 - o.KL2, -R.Sg[0] * o.sq[0] - o.KBL);
-  const double laL = o.la[0], laR = o.la[1];
-  const double vc = laL - BnR * fac;
-  const double vt1 = o.K1L - o.Bc1 * fac;
-  const double vt2 = o.K2L - o.Bc2 * fac;
-  const double vBc = vc * BnR + vt1 * o.Bc1 + vt2 * o.Bc2;
-  double ucL[8], ucR[8];
-  ucL[CB + N] = BnR; ucL[CB + T1] = o.Bc1; ucL[CB + T2] = o.Bc2;
-  ucR[CB + N] = BnR; ucR[CB + T1] = o.Bc1; ucR[CB + T2] = o.Bc2;
-  ucL[CD] = uaL[CD] * sdiv(laL - o.va[0], laL - vc);
-  ucR[CD] = uaR[CD] * sdiv(laR - o.va[1], laR - vc);
-  ucL[CT] = sdiv(laL * uaL[CT] - faL[CT] + ps * vc - vBc * BnR, laL - vc);
-  ucR[CT] = sdiv(laR * uaR[CT] - faR[CT] + ps * vc - vBc * BnR, laR - vc);
-  {
-    const double gl = ucL[CT] + ps + ucL[CD];
-    const double gr = ucR[CT] + ps + ucR[CD];
-    ucL[CS + N] = gl * vc - vBc * BnR;
-    ucL[CS + T1] = gl * vt1 - vBc * o.Bc1;
-    ucL[CS + T2] = gl * vt2 - vBc * o.Bc2;
-    ucR[CS + N] = gr * vc - vBc * BnR;
-    ucR[CS + T1] = gr * vt1 - vBc * o.Bc1;
-    ucR[CS + T2] = gr * vt2 - vBc * o.Bc2;
-  }
-
-  const bool wave_bad = (o.va[1] > cmax) || (o.va[0] < -cmin) || (vc < laL) || (vc > laR);
-  const bool failed = !conv || wave_bad || bn_zero;
-  // region at xi = 0   hlld.py:1448-1480
-  const bool mL = 0.0 <= -cmin, maL = (-cmin < 0.0) && (0.0 <= laL);
-  const bool mcL = (laL < 0.0) && (0.0 < vc), mcR = (vc <= 0.0) && (0.0 < laR);
-  const bool maR = (laR <= 0.0) && (0.0 < cmax), mR = 0.0 >= cmax;
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    double f = fH[k], u = uH[k];
-    if (mL) { f = h.fL[k]; u = h.uL[k]; }
-    if (maL) { f = faL[k]; u = uaL[k]; }
-    if (mcL) { f = faL[k] + laL * (ucL[k] - uaL[k]); u = ucL[k]; }
-    if (mcR) { f = faR[k] + laR * (ucR[k] - uaR[k]); u = ucR[k]; }
-    if (maR) { f = faR[k]; u = uaR[k]; }
-    if (mR) { f = h.fR[k]; u = h.uR[k]; }
-    if (failed) { f = fH[k]; u = uH[k]; }
-    F[k] = f;
-    if (U) U[k] = u;
-  }
-  if (failed) ps = -fabs(p_hll) - 1e-10;
-  pstar = ps;
-  return (conv ? 0 : FS_NOT_CONVERGED) | (wave_bad ? FS_WAVE_ORDER : 0) |
-         (bn_zero ? FS_BN_ZERO : 0) | (failed ? FS_FALLBACK : 0);
+  return hlld_assemble<N>(h, R, fH, x1, conv, p_hll, F, U, pstar);
 }
 
 }  // namespace srmhd

@@ -100,11 +100,16 @@ typedef struct srmhd_grid2d {
 } srmhd_grid2d;
 
 /* precision: SRMHD_PREC_FP64 (the reference's float64 arithmetic, bit-faithful)
- * or SRMHD_PREC_MIXED (no counterpart in the reference: reconstruction and
- * the Riemann solver in FP32, with the HLL-state con2prim that seeds HLLD's
- * p* bracket solved per face in FP32; corner EMF, CT face-B update, RK
- * accumulation and con2prim stay FP64, so div B is untouched). */
-enum { SRMHD_PREC_FP64 = 0, SRMHD_PREC_MIXED = 1 };
+ * or one of two mixed forms with no counterpart in the reference:
+ *   SRMHD_PREC_MIXED      reconstruction in FP32; the Riemann solver's
+ *                         iterative work in FP32 (HLLD's p* root and the
+ *                         HLL-state con2prim seeding its bracket, solved per
+ *                         face; HLLE / HLLC whole), HLLD's p* polished and its
+ *                         flux assembled in FP64;
+ *   SRMHD_PREC_MIXED_R64  the same with the reconstruction in FP64.
+ * In both, corner EMF, CT face-B update and RK accumulation are FP64 (div B
+ * is untouched) and the cell con2prim is FP64 with each cell its own batch. */
+enum { SRMHD_PREC_FP64 = 0, SRMHD_PREC_MIXED = 1, SRMHD_PREC_MIXED_R64 = 2 };
 
 typedef struct srmhd_scheme2d {
   int32_t solver, limiter, upwind, precision;

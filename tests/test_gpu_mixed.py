@@ -54,7 +54,8 @@ def _phys(S, g, ng):
     return S.P[0][sl], S.P[4][sl], Bm, float(div.abs().max()) / (bmax / g.dx)
 
 
-def test_mixed_rotor_close_to_fp64_divb_and_determinism():
+@pytest.mark.parametrize("form", ["mixed", "mixed_r64"])
+def test_mixed_rotor_close_to_fp64_divb_and_determinism(form):
     """Tapered rotor 256^2, WENO5-Z + HLLD, 20 RK3 steps: L1(rho), L1(p),
     L1(|B|) vs FP64 below 3e-5 (measured 7e-6, 8e-6, 2e-6; the rotor's edge
     amplifies FP32-sized differences: an FP64 run whose initial state is
@@ -64,8 +65,8 @@ def test_mixed_rotor_close_to_fp64_divb_and_determinism():
     _dev()
     print("mixed rotor 256^2 vs fp64", flush=True)
     A, g, ng = _run(256, "weno5z", "hlld", "fp64", 20)
-    B, _, _ = _run(256, "weno5z", "hlld", "mixed", 20)
-    C, _, _ = _run(256, "weno5z", "hlld", "mixed", 20)
+    B, _, _ = _run(256, "weno5z", "hlld", form, 20)
+    C, _, _ = _run(256, "weno5z", "hlld", form, 20)
     ra, pa, ba, _ = _phys(A, g, ng)
     rb, pb, bb, divb = _phys(B, g, ng)
     l1 = (_l1(ra, rb), _l1(pa, pb), _l1(ba, bb))
@@ -76,13 +77,14 @@ def test_mixed_rotor_close_to_fp64_divb_and_determinism():
         assert torch.equal(u, v)
 
 
+@pytest.mark.parametrize("form", ["mixed", "mixed_r64"])
 @pytest.mark.parametrize("limiter,solver", [("mc", "hlld"), ("mp5", "hlle"), ("pcm", "hllc"),
                                             ("minmod", "hlld"), ("mp7", "hlld")])
-def test_mixed_every_limiter_and_solver(limiter, solver):
+def test_mixed_every_limiter_and_solver(limiter, solver, form):
     _dev()
-    print(f"mixed {limiter} {solver}", flush=True)
+    print(f"{form} {limiter} {solver}", flush=True)
     A, g, ng = _run(96, limiter, solver, "fp64", 8)
-    B, _, _ = _run(96, limiter, solver, "mixed", 8)
+    B, _, _ = _run(96, limiter, solver, form, 8)
     ra, pa, ba, _ = _phys(A, g, ng)
     rb, pb, bb, divb = _phys(B, g, ng)
     assert torch.isfinite(rb).all() and torch.isfinite(pb).all()
