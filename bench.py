@@ -16,6 +16,7 @@ the same problem.
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -274,11 +275,154 @@ def _l1(a, b):
     return float((a - b).abs().mean() / b.abs().mean())
 
 
+def cpu_baseline_adjoint(n_sample=16, steps=2):
+    """torch autograd through the CPU oracle (float64, IEEE arithmetic) on the
+    same differentiable Orszag-Tang problem at n_sample^2, ``steps`` RK2
+    steps with the first dt of the device run repeated: forward + backward,
+    timed."""
+    import torch
+    import oracle.eos as OE
+    import oracle.step2d as OS
+    from oracle.grid import Grid2D
+    threads = torch.get_num_threads()
+    eos = OE.ideal(4.0 / 3.0)
+    ng = 2
+    g = Grid2D(0.0, 2 * math.pi, n_sample, 0.0, 2 * math.pi, n_sample, ng=ng)
+    t0 = time.perf_counter()
+    r = torch.ones(n_sample, n_sample, dtype=torch.float64, requires_grad=True)
+    a = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    rho = torch.nn.functional.pad(r[None, None], (ng, ng, ng, ng), mode="circular")[0, 0]
+    amp = 0.99 / math.sqrt(2.0)
+    z = torch.zeros(g.shape, dtype=torch.float64)
+    P = {"rho": rho, "p": torch.ones_like(z), "vx": -amp * torch.sin(g.Y),
+         "vy": amp * torch.sin(g.X), "vz": z, "Bx": a * (-torch.sin(g.Y)),
+         "By": a * torch.sin(2.0 * g.X), "Bz": z}
+    P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
+    XF = g.xf.unsqueeze(1).expand(g.nxt + 1, g.nyt + 1)
+    YF = g.yf.unsqueeze(0).expand(g.nxt + 1, g.nyt + 1)
+    Az0 = torch.cos(YF) + 0.5 * torch.cos(2.0 * XF)
+    c = {k: v.clone() for k, v in OS.prims_to_cons_2d(P, g).items()}
+    st = OS.State2D({k: c[k] for k in OS.EVOLVED_KEYS}, a * ((Az0[:, 1:] - Az0[:, :-1]) / g.dy),
+                    a * (-(Az0[1:, :] - Az0[:-1, :]) / g.dx), P)
+    st = OS.sync_state(st, g, eos, "periodic", "periodic")
+    dt = OS.compute_dt_2d(st.prims, g, eos, 0.4)
+    for _ in range(steps):
+        st, _ = OS.rk_step_ct(st, g, eos, dt, scheme="rk2", bc_x="periodic", bc_y="periodic",
+                              limiter="mc")
+    L = (st.prims["rho"][ng:ng + n_sample, ng:ng + n_sample] ** 2).sum()
+    L.backward()
+    wall = time.perf_counter() - t0
+    return {"value": n_sample * n_sample * steps / wall, "unit": "zone-cycles/s (forward + backward)",
+            "cores": threads, "kind": "port",
+            "sample": (f"torch {torch.__version__} autograd through the CPU oracle (float64, "
+                       f"{threads} threads): the same differentiable Orszag-Tang problem at "
+                       f"{n_sample}^2, {steps} MC/HLLD/CT RK2 steps, forward + backward "
+                       f"({wall:.1f} s)")}
+
+
+def bench_adjoint(a):
+    """Config 5: the gradient of L = sum (rho_final - rho_target)^2 of the
+    relativistic Orszag-Tang problem (512^2 periodic, MC + HLLD + upwind CT
+    + SSP-RK2, 200 steps, dt frozen from the A = 1 run, target from A = 1.05)
+    with respect to the initial density field and the field amplitude A,
+    through the hand-written adjoint with a checkpoint every 20 steps; dL/dA
+    checked against central finite differences (h = 1e-6)."""
+    import ctypes
+    import torch
+    from srmhd import _lib, adjoint as AD, eos as E, grid as G
+    dev = torch.device("cuda:0")
+    eos = E.hybrid_eos(0.0, 4.0 / 3.0, 4.0 / 3.0)
+    n, steps, K = a.n, a.steps, a.checkpoint
+    g = G.Grid2D(0.0, 2 * math.pi, n, 0.0, 2 * math.pi, n, ng=2, device=dev)
+    roll = AD.Rollout2D(g, eos, solver="hlld", limiter="mc", bc_x="periodic", bc_y="periodic",
+                        scheme="rk2", checkpoint_every=K)
+    prob = AD.OTProblem(g, eos)
+    rho0 = torch.ones(n, n, dtype=torch.float64, device=dev)
+    one = torch.tensor(1.0, dtype=torch.float64, device=dev)
+    U, P, _ = prob.initial_state(roll, rho0, one)
+    dts = AD.frozen_dts(roll, U, P, steps, a.cfl)
+    target = AD.ot_final_rho(prob, roll, rho0, one * 1.05, dts)
+    torch.cuda.synchronize()
+    print(f"dt sequence and target ready ({steps} steps)", file=sys.stderr, flush=True)
+
+    def gradient(dt_seq):
+        rho = rho0.clone().requires_grad_(True)
+        A = one.clone().requires_grad_(True)
+        rf = AD.OTRollout.apply(rho, A, prob, roll, dt_seq)
+        L = ((rf - target[:rf.shape[0], :rf.shape[1]]) ** 2).sum()
+        L.backward()
+        return L.detach(), rho.grad, A.grad
+
+    for _ in range(a.warmup):
+        gradient(dts[:2])
+    torch.cuda.synchronize()
+    lib = roll.lib
+    _lib.check(lib.srmhd_timing_enable(_lib.KERNEL_CLASSES["adj_sweep_x"], 2 * steps),
+               "srmhd_timing_enable")
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    L, g_rho, g_A = gradient(dts)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    _lib.check(lib.srmhd_timing_read(ctypes.byref(ms), ctypes.byref(cnt)), "srmhd_timing_read")
+    lib.srmhd_timing_enable(-1, 0)
+    peak = torch.cuda.max_memory_allocated()
+    print(f"gradient in {wall:.2f} s", file=sys.stderr, flush=True)
+    t1 = time.perf_counter()
+    h = 1e-6
+
+    def loss(A):
+        r = AD.ot_final_rho(prob, roll, rho0, A, dts)
+        return float(((r - target) ** 2).sum())
+
+    fd = (loss(one + h) - loss(one - h)) / (2 * h)
+    fwd_wall = (time.perf_counter() - t1) / 2
+    avg_ms = ms.value / max(cnt.value, 1)
+    nfx = (n + 5) * (n + 4)
+    # the x-sweep adjoint per face: 8 state/primitive arrays of the stencil and
+    # p* in, 7 flux adjoints in, 7 x 4 window adjoints + the B_n adjoint out
+    kb = nfx * (8 * 8 + 8 + 7 * 8 + 29 * 8)
+    achieved = kb / (avg_ms * 1e-3) / 1e9
+    out = {
+        "metric": "zone-cycles/s of the differentiable rollout (forward + hand-written adjoint), "
+                  "relativistic Orszag-Tang 512^2",
+        "value": n * n * steps / wall, "unit": "zone-cycles/s", "n_gpus": 1, "steps": steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * wall / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"relativistic Orszag-Tang {n}^2 periodic, MC + HLLD + upwind CT "
+                               f"+ SSP-RK2, {steps} steps, dt frozen from the A = 1 run, loss "
+                               f"sum (rho - rho_target)^2 with the target from A = 1.05, "
+                               f"gradient w.r.t. the initial rho field and A, checkpoint every "
+                               f"{K} steps", "n": n, "checkpoint_every": K},
+        "forward_only_ms_per_step": 1e3 * fwd_wall / steps,
+        "adjoint_over_forward": wall / fwd_wall,
+        "loss": float(L), "dLdA": {"adjoint": float(g_A), "central_fd_h1e-6": fd,
+                                   "rel": abs(float(g_A) - fd) / abs(fd)},
+        "grad_rho_l2": float(g_rho.norm()),
+        "peak_device_memory_GB": peak / 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_adj_sweep<0, 1, 2> (x-sweep adjoint)", "avg_launch_ms": avg_ms,
+                     "launches": cnt.value, "algorithmic_bytes_per_launch": kb},
+    }
+    if not a.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline_adjoint()
+        except Exception as exc:  # the baseline never hides the GPU result
+            out["cpu_baseline"] = {"error": repr(exc)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="rotor", choices=("rotor", "tubes"),
-                    help="rotor: the 2048^2 rotor (headline); tubes: 4096 batched tubes")
+    ap.add_argument("--workload", default="rotor", choices=("rotor", "tubes", "adjoint"),
+                    help="rotor: the 2048^2 rotor (headline); tubes: 4096 batched tubes; "
+                         "adjoint: the differentiable 512^2 Orszag-Tang rollout")
+    ap.add_argument("--checkpoint", type=int, default=20,
+                    help="adjoint workload: steps between checkpoints")
     ap.add_argument("--precision", default="fp64,mixed,mixed_r64",
                     help="comma-separated rotor forms to time (fp64, mixed, mixed_r64); the "
                          "first mixed form is the headline value, the others sit beside it "
@@ -311,6 +455,13 @@ def main():
         if a.limiter == "weno5z":
             a.limiter = "mp5"
         bench_tubes(a)
+        return
+    if a.workload == "adjoint":
+        if a.steps is None:
+            a.steps = 200
+        if a.n == 2048:
+            a.n = 512
+        bench_adjoint(a)
         return
     if a.steps is None:
         a.steps = 200

@@ -277,7 +277,9 @@ SRMHD_DEV int c2p_pass_b(const Kastaun& k, const C2PRec& rec, int K, double atmo
 // The mixed-precision form uses it for the cells (the batch coupling of the
 // FP64 form only moves converged roots by a few ulps).  Returns the status
 // bits of c2p_pass_b.
-SRMHD_DEV int c2p_single(const Kastaun& k, double atmo_rho, double atmo_eps, Prim& P) {
+// mu_out (may be NULL) receives the root the primitives were finished at.
+SRMHD_DEV int c2p_single(const Kastaun& k, double atmo_rho, double atmo_eps, Prim& P,
+                         double* mu_out = nullptr) {
   const double mup = k.mu_plus();
   double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
   double f0 = k.master(x0), f1 = k.master(x1);
@@ -295,8 +297,10 @@ SRMHD_DEV int c2p_single(const Kastaun& k, double atmo_rho, double atmo_eps, Pri
     const double mu = k.bisection(mup, fmu);
     conv = fabs(fmu) < C2P_TOL;
     clamped = k.finish(mu, P);
+    if (mu_out) *mu_out = mu;
   } else {
     clamped = k.finish(x1, P);
+    if (mu_out) *mu_out = x1;
   }
   const bool atmo = P.rho < atmo_rho;
   if (atmo) {

@@ -23,14 +23,27 @@ constexpr double TINY = 1e-300;  // hlld.py:29, c2p.py:29
 // the plain double expression.  Mixed arguments promote to the Dual type.
 using ::fabs;
 SRMHD_DEV double qnan() { return __builtin_nan(""); }
+// torch.maximum / torch.minimum; at a tie of two Dual values the tangent is
+// the mean of both (torch's convention for the gradient of a tie)
+template <class R>
+SRMHD_DEV R tie_mean(const R& a, const R& b) { return b; }
+template <int K>
+SRMHD_DEV Dual<K> tie_mean(const Dual<K>& a, const Dual<K>& b) {
+  Dual<K> r = b;
+#pragma unroll
+  for (int k = 0; k < K; ++k) r.d[k] = 0.5 * (a.d[k] + b.d[k]);
+  return r;
+}
 template <class A, class B>
 SRMHD_DEV auto tmax(const A& a, const B& b) -> decltype(a + b) {   // torch.maximum
   using R = decltype(a + b);
+  if (val(a) == val(b)) return tie_mean(R(a), R(b));
   return (val(a) > val(b) || val(a) != val(a)) ? R(a) : R(b);
 }
 template <class A, class B>
 SRMHD_DEV auto tmin(const A& a, const B& b) -> decltype(a + b) {   // torch.minimum
   using R = decltype(a + b);
+  if (val(a) == val(b)) return tie_mean(R(a), R(b));
   return (val(a) < val(b) || val(a) != val(a)) ? R(a) : R(b);
 }
 template <class A, class B>

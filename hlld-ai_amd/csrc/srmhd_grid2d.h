@@ -140,6 +140,19 @@ __device__ inline double bcy(const G2& g, const double* Byf, int i, int j) {
   return 0.5 * (Byf[(int64_t)i * (g.nyt + 1) + j] + Byf[(int64_t)i * (g.nyt + 1) + j + 1]);
 }
 
+// con2prim input of cell (i, j): cons with B from the faces (driver2d.py:208-212)
+__device__ inline void cell_cons(const G2& g, const St2& S, int i, int j, double u[8]) {
+  const int64_t c = (int64_t)i * g.nyt + j;
+  u[CD] = S.p[0][c];
+  u[CS] = S.p[1][c];
+  u[CS + 1] = S.p[2][c];
+  u[CS + 2] = S.p[3][c];
+  u[CT] = S.p[4][c];
+  u[CB] = bcx(g, S.p[6], i, j);
+  u[CB + 1] = bcy(g, S.p[7], i, j);
+  u[CB + 2] = S.p[5][c];
+}
+
 template <int N>
 __device__ inline void face_ij(const G2& g, int64_t t, int& i, int& j) {
   const int ny = N == 0 ? g.nyt : g.nyt + 1;

@@ -277,7 +277,8 @@ __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e
 
 template <int N, int LIM, int SOL>
 __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e,
-                                                 const C2PRec* rec, srmhd_flux_diag* d, Flx F) {
+                                                 const C2PRec* rec, srmhd_flux_diag* d, Flx F,
+                                                 double* pstar_out) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = t < n_faces<N>(g);
   int st = 0, iters = 0;
@@ -310,6 +311,7 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
     F.p[5][t] = f[CB + 2];
     F.p[6][t] = N == 0 ? f[CB + 1] : f[CB + 0];   // F^x(By) / F^y(Bx): the face EMF
     fD = f[CD];
+    if (pstar_out) pstar_out[t] = ps;
   }
   // the deadband scale of the upwind EMF is needed for every solver
   diag_accumulate(d, active, st, iters, dbits(fabs(fD)));
@@ -453,19 +455,6 @@ __global__ __launch_bounds__(BLK) void k_bc_state(G2 g, St2Out U) {
     if (si != i || fj != j)
       U.p[7][(int64_t)i * (g.nyt + 1) + j] = U.p[7][(int64_t)si * (g.nyt + 1) + fj];
   }
-}
-
-// con2prim input of cell (i, j): cons with B from the faces (driver2d.py:208-212)
-__device__ inline void cell_cons(const G2& g, const St2& S, int i, int j, double u[8]) {
-  const int64_t c = (int64_t)i * g.nyt + j;
-  u[CD] = S.p[0][c];
-  u[CS] = S.p[1][c];
-  u[CS + 1] = S.p[2][c];
-  u[CS + 2] = S.p[3][c];
-  u[CT] = S.p[4][c];
-  u[CB] = bcx(g, S.p[6], i, j);
-  u[CB + 1] = bcy(g, S.p[7], i, j);
-  u[CB + 2] = S.p[5][c];
 }
 
 __global__ __launch_bounds__(BLK) void k_c2p2d_a(G2 g, St2 S, srmhd_eos e, C2PRec* rec,
@@ -620,6 +609,14 @@ inline void tend(int cls, hipStream_t s) {
   }
 }
 
+}  // namespace
+
+// the live-timing hooks for the other translation units (srmhd_launch.h)
+void srmhd::timing_begin(int cls, hipStream_t s) { tbegin(cls, s); }
+void srmhd::timing_end(int cls, hipStream_t s) { tend(cls, s); }
+
+namespace {
+
 
 template <int N>
 int flux_dispatch(int solver, int64_t n, const PrimPtrs& L, const PrimPtrs& R, const srmhd_eos& e,
@@ -666,6 +663,8 @@ bool grid_ok(const srmhd_grid2d* g, int limiter) {
 }
 
 struct Ws2 {
+  double* psx = nullptr;        // optional p* tape of the x and y sweeps (not carved)
+  double* psy = nullptr;
   Flx Fx, Fy;
   double* Ez;
   C2PRec *recx, *recy, *recc;
@@ -705,7 +704,7 @@ void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const W
     tend(SRMHD_K_SWEEP_X_A, s);
   }
   tbegin(SRMHD_K_SWEEP_X_B, s);
-  k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, w.Fx);
+  k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, w.Fx, w.psx);
   tend(SRMHD_K_SWEEP_X_B, s);
   if (SOL == SOLVER_HLLD) {
     tbegin(SRMHD_K_SWEEP_Y_A, s);
@@ -713,7 +712,7 @@ void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const W
     tend(SRMHD_K_SWEEP_Y_A, s);
   }
   tbegin(SRMHD_K_SWEEP_Y_B, s);
-  k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, w.Fy);
+  k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, w.Fy, w.psy);
   tend(SRMHD_K_SWEEP_Y_B, s);
 }
 
@@ -1111,17 +1110,31 @@ static bool scheme_ok(const srmhd_scheme2d* s) {
           s->precision == SRMHD_PREC_MIXED_R64);
 }
 
-int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_eos* eos,
-                  double a, double b, double c, const double* dt_dev,
-                  const double* const* U0, const double* const* U, const double* const* P,
-                  double* const* out, double* const* Pout, srmhd_flux_diag* diag_xy,
-                  void* work, int64_t work_bytes, void* stream) {
+// the deadband thresholds of the upwind EMF, as k_corner forms them
+__global__ void k_tape_thresh(const srmhd_flux_diag* dg, double* th) {
+  if (threadIdx.x || blockIdx.x) return;
+  th[0] = 1e-9 * __longlong_as_double(dg[0].absF_D_max);
+  th[1] = 1e-9 * __longlong_as_double(dg[1].absF_D_max);
+}
+
+static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_eos* eos,
+                      double a, double b, double c, const double* dt_dev,
+                      const double* const* U0, const double* const* U, const double* const* P,
+                      double* const* out, double* const* Pout, srmhd_flux_diag* diag_xy,
+                      const srmhd2d_tape* tape, void* work, int64_t work_bytes, void* stream) {
   if (!scheme_ok(sp) || !grid_ok(gp, sp->limiter) || !eos_ok(eos) || !dt_dev || !U0 || !U ||
       !P || !out || !Pout || !work)
     return SRMHD_EARG;
+  if (tape && (sp->precision != SRMHD_PREC_FP64 || !tape->pstar_x || !tape->pstar_y ||
+               !tape->FD_x || !tape->FD_y || !tape->thresh))
+    return SRMHD_EARG;
   const G2 g = make_g2(gp);
-  const Ws2 w = carve2d(g, work);
+  Ws2 w = carve2d(g, work);
   if (work_bytes < w.bytes) return SRMHD_EARG;
+  if (tape) {
+    w.psx = tape->pstar_x;
+    w.psy = tape->pstar_y;
+  }
   St2 s0, s1, so;
   St2Out o;
   Pr2 pr, pro;
@@ -1151,6 +1164,15 @@ int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_
   } else {
     sweeps_dispatch(sp->limiter, sp->solver, g, s1, pr, *eos, w, st);
   }
+  if (tape) {
+    const int64_t nfx = (int64_t)(g.nxt + 1) * g.nyt, nfy = (int64_t)g.nxt * (g.nyt + 1);
+    if (hipMemcpyAsync(tape->FD_x, w.Fx.p[0], nfx * sizeof(double), hipMemcpyDeviceToDevice,
+                       st) != hipSuccess ||
+        hipMemcpyAsync(tape->FD_y, w.Fy.p[0], nfy * sizeof(double), hipMemcpyDeviceToDevice,
+                       st) != hipSuccess)
+      return launch_status();
+    k_tape_thresh<<<1, 1, 0, st>>>(w.diag, tape->thresh);
+  }
   const int64_t ncor = (int64_t)(g.nxt + 1) * (g.nyt + 1);
   tbegin(SRMHD_K_CORNER, st);
   k_corner<<<blocks_for(ncor), BLK, 0, st>>>(g, s1, pr, w.Fx.p[0], w.Fx.p[6], w.Fy.p[0],
@@ -1162,6 +1184,25 @@ int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_
   c2p_cells(g, so, *eos, sp->atmo_rho, sp->atmo_eps, w, po, st, mixed);
   if (diag_xy) k_copy_diag<<<1, 64, 0, st>>>(w.diag, diag_xy, 2);
   return launch_status();
+}
+
+int srmhd2d_stage(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_eos* eos,
+                  double a, double b, double c, const double* dt_dev,
+                  const double* const* U0, const double* const* U, const double* const* P,
+                  double* const* out, double* const* Pout, srmhd_flux_diag* diag_xy,
+                  void* work, int64_t work_bytes, void* stream) {
+  return stage_impl(gp, sp, eos, a, b, c, dt_dev, U0, U, P, out, Pout, diag_xy, nullptr, work,
+                    work_bytes, stream);
+}
+
+int srmhd2d_stage_taped(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_eos* eos,
+                        double a, double b, double c, const double* dt_dev,
+                        const double* const* U0, const double* const* U, const double* const* P,
+                        double* const* out, double* const* Pout, const srmhd2d_tape* tape,
+                        void* work, int64_t work_bytes, void* stream) {
+  if (!tape) return SRMHD_EARG;
+  return stage_impl(gp, sp, eos, a, b, c, dt_dev, U0, U, P, out, Pout, nullptr, tape, work,
+                    work_bytes, stream);
 }
 
 int srmhd2d_sync(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const srmhd_eos* eos,

@@ -54,4 +54,9 @@ struct Carver {
 
 inline bool eos_ok(const srmhd_eos* e) { return e && e->gamma > 1.0; }
 
+// live kernel timing (srmhd_timing_enable): HIP event pairs around the
+// launches of one kernel class, defined in srmhd_kernels.hip
+void timing_begin(int cls, hipStream_t s);
+void timing_end(int cls, hipStream_t s);
+
 }  // namespace srmhd

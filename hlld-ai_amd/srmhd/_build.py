@@ -18,12 +18,12 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(ROOT, "csrc")
 OUT_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(OUT_DIR, "libsrmhd_hip.so")
-SOURCES = ["srmhd_kernels.hip", "srmhd_tubes.hip", "srmhd_mixed.hip"]
+SOURCES = ["srmhd_kernels.hip", "srmhd_tubes.hip", "srmhd_mixed.hip", "srmhd_adjoint.hip"]
 # the mixed-precision sweeps: FP32 contraction and approximate FP32 division
 # and sqrt (this file only; the FP64 sources keep -ffp-contract=off)
 EXTRA = {"srmhd_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
 HEADERS = ["srmhd_common.h", "srmhd_c2p.h", "srmhd_recon.h", "srmhd_riemann.h", "srmhd_launch.h",
-           "srmhd_grid2d.h", "srmhd_mixed.h", "srmhd_f32.h"]
+           "srmhd_grid2d.h", "srmhd_mixed.h", "srmhd_f32.h", "srmhd_dual.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -51,6 +51,12 @@ class Scheme2D(ctypes.Structure):
                 ("atmo_eps", ctypes.c_double)]
 
 
+class Tape2D(ctypes.Structure):
+    _fields_ = [("pstar_x", ctypes.c_void_p), ("pstar_y", ctypes.c_void_p),
+                ("FD_x", ctypes.c_void_p), ("FD_y", ctypes.c_void_p),
+                ("thresh", ctypes.c_void_p)]
+
+
 class TubeLayout(ctypes.Structure):
     _fields_ = [("off_c", ctypes.c_void_p), ("ncells", ctypes.c_void_p), ("dx", ctypes.c_void_p),
                 ("Bn", ctypes.c_void_p), ("order", ctypes.c_void_p),
@@ -78,6 +84,19 @@ _SIGS = {
                                      _P, _P, _I64, _P]),
     "srmhd2d_sync": (ctypes.c_int, [ctypes.POINTER(Grid2D), ctypes.POINTER(Scheme2D),
                                     ctypes.POINTER(Eos), _PP, _PP, _P, _I64, _P]),
+    "srmhd2d_stage_taped": (ctypes.c_int, [ctypes.POINTER(Grid2D), ctypes.POINTER(Scheme2D),
+                                           ctypes.POINTER(Eos), _D, _D, _D, _P, _PP, _PP, _PP,
+                                           _PP, _PP, ctypes.POINTER(Tape2D), _P, _I64, _P]),
+    "srmhd2d_adjoint_workspace_bytes": (_I64, [ctypes.POINTER(Grid2D),
+                                               ctypes.POINTER(Scheme2D)]),
+    "srmhd2d_stage_adjoint": (ctypes.c_int, [ctypes.POINTER(Grid2D), ctypes.POINTER(Scheme2D),
+                                             ctypes.POINTER(Eos), _D, _D, _D, _P, _PP, _PP, _PP,
+                                             ctypes.POINTER(Tape2D), _PP, _PP, _PP, _PP, _PP, _P,
+                                             _I64, _P]),
+    "srmhd2d_sync_adjoint": (ctypes.c_int, [ctypes.POINTER(Grid2D), ctypes.POINTER(Scheme2D),
+                                            ctypes.POINTER(Eos), _PP, _PP, _PP, _PP, _P, _I64,
+                                            _P]),
+    "srmhd_p2c_adjoint": (ctypes.c_int, [_I64, _PP, _PP, _PP, _P]),
     "srmhd2d_dt": (ctypes.c_int, [ctypes.POINTER(Grid2D), ctypes.POINTER(Eos), _D, _PP, _PP, _P,
                                   _P, _I64, _P]),
     "srmhd1d_workspace_bytes": (_I64, [ctypes.POINTER(Grid1D)]),
@@ -101,7 +120,7 @@ _SIGS = {
                                          ctypes.POINTER(ctypes.c_int64)]),
 }
 KERNEL_CLASSES = {"sweep_x_a": 0, "sweep_x_b": 1, "sweep_y_a": 2, "sweep_y_b": 3, "corner": 4,
-                  "update": 5, "c2p_a": 6, "c2p_b": 7}
+                  "update": 5, "c2p_a": 6, "c2p_b": 7, "adj_sweep_x": 8}
 EXPORTED = tuple(_SIGS)
 
 

@@ -130,6 +130,64 @@ int srmhd2d_stage(const srmhd_grid2d* g, const srmhd_scheme2d* s, const srmhd_eo
                   double* const* out, double* const* Pout, srmhd_flux_diag* diag_xy,
                   void* work, int64_t work_bytes, void* stream);
 
+/* The stage's record for the adjoint (srmhd2d_stage_adjoint): HLLD's p* of
+ * every x-face ((nxt+1) x nyt) and y-face (nxt x (nyt+1)), negative where the
+ * face fell back to HLLE (the reference's sentinel, hlld.py:1531); the mass
+ * fluxes of both sweeps and the two deadband thresholds 1e-9 max|F_D| of the
+ * upwind EMF (ct.py:96-116).  Device memory, caller-owned. */
+typedef struct srmhd2d_tape {
+  double* pstar_x;
+  double* pstar_y;
+  double* FD_x;
+  double* FD_y;
+  double* thresh;  /* 2 doubles */
+} srmhd2d_tape;
+
+/* srmhd2d_stage (FP64 precision only) that also fills *tape.  No counterpart
+ * in the reference: the forward half of the adjoint. */
+int srmhd2d_stage_taped(const srmhd_grid2d* g, const srmhd_scheme2d* s, const srmhd_eos* eos,
+                        double a, double b, double c, const double* dt_dev,
+                        const double* const* U0, const double* const* U, const double* const* P,
+                        double* const* out, double* const* Pout, const srmhd2d_tape* tape,
+                        void* work, int64_t work_bytes, void* stream);
+
+/* ── adjoint of the 2D step (no counterpart in the reference) ───────────────
+ * Reverse-mode derivative of one FP64 stage out = a U0 + b U + (c dt) L(U, P)
+ * followed by its BCs and con2prim, for the differentiable rollout of
+ * driver2d.rk_step_ct (src/physics/driver2d.py:221-249) with dt held fixed.
+ * Transposed by hand: the con2prim (implicit differentiation of the Kastaun
+ * root, c2p.py:129-158), the BC fills, the RK combination, the flux
+ * divergence, ct_rhs and the Gardiner-Stone corner EMF with the taped upwind
+ * signs (ct.py:119-185), and per face the reconstruction (limiter branches as
+ * taken) and the Riemann solver (HLLD: implicit differentiation of p* at the
+ * taped root, hlld.py:586-602; the HLLE fallback where taped).
+ *   U, P      the stage inputs U^(k) (8) and P^(k) (6);
+ *   Uout      the stage output U^(k+1) (8), BCs applied;
+ *   lamUout   adjoint of U^(k+1) (8), lamPout adjoint of P^(k+1) (6): inputs,
+ *             overwritten (used as scratch);
+ *   lamU0, lamU, lamP: accumulated (+=) adjoints of U0, U^(k), P^(k); lamU0
+ *             and lamU may be the same buffers (stage 0, where U = U0).
+ * Every accumulation is a gather in a fixed order: results are deterministic. */
+int64_t srmhd2d_adjoint_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s);
+int srmhd2d_stage_adjoint(const srmhd_grid2d* g, const srmhd_scheme2d* s, const srmhd_eos* eos,
+                          double a, double b, double c, const double* dt_dev,
+                          const double* const* U, const double* const* P,
+                          const double* const* Uout, const srmhd2d_tape* tape,
+                          double* const* lamUout, double* const* lamPout,
+                          double* const* lamU0, double* const* lamU, double* const* lamP,
+                          void* work, int64_t work_bytes, void* stream);
+
+/* Adjoint of srmhd2d_sync: U is the synced state (8), lamU (8) and lamP (6)
+ * the adjoints of the synced state and primitives (overwritten); lamUin (8)
+ * receives (=) the adjoint of the state before the sync (ghost entries 0). */
+int srmhd2d_sync_adjoint(const srmhd_grid2d* g, const srmhd_scheme2d* s, const srmhd_eos* eos,
+                         const double* const* U, double* const* lamU, double* const* lamP,
+                         double* const* lamUin, void* work, int64_t work_bytes, void* stream);
+
+/* Adjoint of srmhd_p2c: lam_prim (9) += J^T lam_cons (8) at each of n points. */
+int srmhd_p2c_adjoint(int64_t n, const double* const* prim, const double* const* lam_cons,
+                      double* const* lam_prim, void* stream);
+
 /* sync_state (src/physics/driver2d.py:196-218) in place on state U (face BCs,
  * cell BCs) and into P: the initial-data and restart path. */
 int srmhd2d_sync(const srmhd_grid2d* g, const srmhd_scheme2d* s, const srmhd_eos* eos,
@@ -239,7 +297,8 @@ int srmhd_tubes_shared_step(int n_tubes, int64_t total_cells, const srmhd_tube_l
  * Not thread-safe: one timing session per process. */
 enum {
   SRMHD_K_SWEEP_X_A = 0, SRMHD_K_SWEEP_X_B = 1, SRMHD_K_SWEEP_Y_A = 2, SRMHD_K_SWEEP_Y_B = 3,
-  SRMHD_K_CORNER = 4, SRMHD_K_UPDATE = 5, SRMHD_K_C2P_A = 6, SRMHD_K_C2P_B = 7
+  SRMHD_K_CORNER = 4, SRMHD_K_UPDATE = 5, SRMHD_K_C2P_A = 6, SRMHD_K_C2P_B = 7,
+  SRMHD_K_ADJ_SWEEP_X = 8  /* srmhd2d_stage_adjoint: the x-sweep adjoint */
 };
 int srmhd_timing_enable(int kernel_class, int max_launches);
 int srmhd_timing_read(double* total_ms, int64_t* launches);

@@ -160,6 +160,7 @@ def cons_to_prim(cons, eos, atmo_rho=1e-10, atmo_eps=1e-10, return_status=False,
         ks = Kastaun({c: cons[c][idx] for c in _C2P_IN}, eos)
         mu2, conv2, _ = ks.bisection()
         P2, vcl2 = ks.finish(mu2)
+        P = {c: v.clone() for c, v in P.items()}   # out of place for torch autograd
         for c in P:
             P[c][idx] = P2[c]
         conv = conv.clone()

@@ -27,11 +27,33 @@ def set_sqrt_mode(mode):
     SQRT_MODE = mode
 
 
+def _ieee_sqrt(x):
+    with np.errstate(invalid="ignore"):
+        return torch.from_numpy(np.asarray(np.sqrt(x.detach().numpy()), dtype=np.float64))
+
+
+class _IeeeSqrt(torch.autograd.Function):
+    """The correctly rounded square root with the derivative 1 / (2 sqrt x),
+    so that torch autograd runs through the IEEE-mode oracle."""
+
+    @staticmethod
+    def forward(ctx, x):
+        y = _ieee_sqrt(x)
+        ctx.save_for_backward(y)
+        return y
+
+    @staticmethod
+    def backward(ctx, g):
+        (y,) = ctx.saved_tensors
+        return g / (2.0 * y)
+
+
 def sqrt(x):
     if SQRT_MODE == "torch":
         return torch.sqrt(x)
-    with np.errstate(invalid="ignore"):
-        return torch.from_numpy(np.asarray(np.sqrt(x.detach().numpy()), dtype=np.float64))
+    if x.requires_grad:
+        return _IeeeSqrt.apply(x)
+    return _ieee_sqrt(x)
 
 
 CONS_KEYS = ("D", "tau", "Sx", "Sy", "Sz", "Bx", "By", "Bz")   # hlld.py:298-332 order

@@ -187,9 +187,12 @@ def sync_state(state, grid, eos, bc_x="outflow", bc_y="outflow", atmo_rho=1e-10)
     Bcx, Bcy = cell_centred_B(state.Bxf, state.Byf)
     cons = dict(state.cons)
     cons["Bx"], cons["By"] = Bcx, Bcy
-    prims = cons_to_prims_2d(cons, grid, eos, atmo_rho)
-    prims["Bx"], prims["By"] = Bcx, Bcy
-    prims["Bz"] = cons["Bz"]
+    # copies: the BC fill below writes in place, and the con2prim keeps its
+    # inputs and outputs for torch autograd (same values as the reference's
+    # aliases)
+    prims = {k: v.clone() for k, v in cons_to_prims_2d(cons, grid, eos, atmo_rho).items()}
+    prims["Bx"], prims["By"] = Bcx.clone(), Bcy.clone()
+    prims["Bz"] = cons["Bz"].clone()
     grid.bc_cells(prims, bc_x, bc_y)
     state.prims = prims
     return state
@@ -271,7 +274,9 @@ def orszag_tang_2pi(grid, eos, A=1.0, vmax=0.99, rho=1.0, press=1.0):
 def init_state(grid, eos, prims, Az, bc=("outflow", "outflow")):
     """run_2d.py:227-231: faces from Az, cons from prims, sync."""
     Bxf, Byf = b_from_potential(Az, grid.dx, grid.dy)
-    c = prims_to_cons_2d(prims, grid)
+    # copies: the sync below fills BCs in place (torch autograd keeps the
+    # primitive_to_conserved outputs)
+    c = {k: v.clone() for k, v in prims_to_cons_2d(prims, grid).items()}
     st = State2D({k: c[k] for k in EVOLVED_KEYS}, Bxf.contiguous(), Byf.contiguous(), prims)
     return sync_state(st, grid, eos, bc[0], bc[1])
 

@@ -19,7 +19,8 @@ def test_header_declares_the_expected_entry_points():
     names = _declared()
     for n in ("srmhd_flux", "srmhd_c2p", "srmhd_p2c", "srmhd2d_stage", "srmhd2d_sync",
               "srmhd2d_dt", "srmhd_flux_workspace_bytes", "srmhd_c2p_workspace_bytes",
-              "srmhd2d_workspace_bytes"):
+              "srmhd2d_workspace_bytes", "srmhd2d_stage_taped", "srmhd2d_stage_adjoint",
+              "srmhd2d_sync_adjoint", "srmhd_p2c_adjoint", "srmhd2d_adjoint_workspace_bytes"):
         assert n in names
 
 
@@ -56,3 +57,20 @@ def test_product_path_refuses_cpu_tensors():
          ("rho", "vx", "vy", "vz", "p", "eps", "Bx", "By", "Bz")}
     with pytest.raises(RuntimeError):
         H.hlld_flux(s, s, E.hybrid_eos(0.0, 5 / 3, 5 / 3))
+
+
+def test_adjoint_workspace_query_without_gpu():
+    """FP64 only, every limiter; the window adjoints grow with the stencil."""
+    from srmhd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load_library()
+    g = _lib.Grid2D(64, 64, 3, 1, 1, 0, 0.1, 0.1)
+    sizes = []
+    for lim in range(6):
+        s = _lib.Scheme2D(2, lim, 1, 0, 1e-10, 1e-10)
+        sizes.append(lib.srmhd2d_adjoint_workspace_bytes(ctypes.byref(g), ctypes.byref(s)))
+    assert all(b > 0 for b in sizes)
+    assert sizes[0] < sizes[1] < sizes[3] < sizes[5]
+    mixed = _lib.Scheme2D(2, 1, 1, 1, 1e-10, 1e-10)
+    assert lib.srmhd2d_adjoint_workspace_bytes(ctypes.byref(g), ctypes.byref(mixed)) == -1
