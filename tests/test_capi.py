@@ -65,7 +65,7 @@ def test_adjoint_workspace_query_without_gpu():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("library not built")
     lib = _lib.load_library()
-    g = _lib.Grid2D(64, 64, 3, 1, 1, 0, 0.1, 0.1)
+    g = _lib.Grid2D(64, 64, 4, 1, 1, 0, 0.1, 0.1)   # ng 4: room for MP7
     sizes = []
     for lim in range(6):
         s = _lib.Scheme2D(2, lim, 1, 0, 1e-10, 1e-10)
