@@ -43,7 +43,8 @@ DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "swee
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
 # (form, kernel class, n)
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {("mixed", "sweep_x_b", 2048): 553.4e6, ("fp64", "sweep_x_a", 2048): 416.8e6,
+                    ("fp64", "c2p_a", 2048): 376.3e6, ("mixed", "c2p_a", 2048): 490.0e6}
 
 
 def algorithmic_bytes(kernel, n, ng, precision="fp64"):
@@ -443,7 +444,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy a 512 MiB float64 buffer three times "
-                         "(the plain copy kernel that scales the FETCH/WRITE counters)")
+                         "with an elementwise kernel (the plain copy kernel that scales the "
+                         "FETCH/WRITE counters)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -488,10 +490,13 @@ def main():
         res[prec] = _rotor_form(a, prec, kernel, U_init, P_init)
 
     if a.copy_calibration:
+        # an elementwise kernel of known traffic (torch routes copy_ through
+        # the DMA engines, which the counters do not see): 512 MiB read,
+        # 512 MiB written per launch
         src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
         for _ in range(3):
-            dst.copy_(src)
+            torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
 
