@@ -69,8 +69,72 @@ def orszag_tang_2pi(grid, eos, A=1.0, vmax=0.99, rho=1.0, press=1.0):
     return P, A * (torch.cos(YF) + 0.5 * torch.cos(2.0 * XF))
 
 
+def orszag_tang(grid, eos, v_max=0.99, B0=None, press=None, rho=None):
+    """The reference's relativistic Orszag-Tang vortex on the periodic unit
+    square (initial_data2d.py:194-253): rho = 25/(36 pi), p = 5/(12 pi),
+    B0 = 1/sqrt(4 pi), v = v_max/sqrt2 (-sin 2 pi y, sin 2 pi x, 0),
+    B = B0 (-sin 2 pi y, sin 4 pi x, 0) from
+    Az = B0 (cos(4 pi x)/(4 pi) + cos(2 pi y)/(2 pi))."""
+    if B0 is None:
+        B0 = 1.0 / math.sqrt(4.0 * math.pi)
+    if rho is None:
+        rho = 25.0 / (36.0 * math.pi)
+    if press is None:
+        press = 5.0 / (12.0 * math.pi)
+    amp = v_max / math.sqrt(2.0)
+    X, Y = grid.X, grid.Y
+    z = torch.zeros(grid.shape, dtype=torch.float64)
+    P = {"rho": torch.full_like(z, rho), "p": torch.full_like(z, press),
+         "vx": -amp * torch.sin(2 * math.pi * Y), "vy": amp * torch.sin(2 * math.pi * X),
+         "vz": z.clone(), "Bx": -B0 * torch.sin(2 * math.pi * Y),
+         "By": B0 * torch.sin(4 * math.pi * X), "Bz": z.clone()}
+    P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
+    XF, YF = _corners(grid)
+    Az = B0 * (torch.cos(4 * math.pi * XF) / (4 * math.pi)
+               + torch.cos(2 * math.pi * YF) / (2 * math.pi))
+    return P, Az
+
+
+def orszag_tang_bs(grid, eos):
+    """Beckwith & Stone's flow speed, 0.5 per component
+    (initial_data2d.py:256-270)."""
+    return orszag_tang(grid, eos, v_max=0.5 * math.sqrt(2.0))
+
+
+def riemann2d(grid, eos, B0=0.5, angle_deg=45.0, angle_z_deg=0.0):
+    """The four-quadrant Riemann problem threaded by a uniform field
+    B = B0 (cos a cos b, sin a cos b, sin b) (initial_data2d.py:128-191):
+    (rho, p, vx, vy) top left (0.1, 1, 0.99, 0), top right (0.1, 0.01, 0,
+    0), bottom left (0.5, 1, 0, 0), bottom right (0.1, 1, 0, 0.99); at
+    45 degrees sin a is taken equal to cos a (the problem's x <-> y
+    symmetry)."""
+    X, Y = grid.X, grid.Y
+    z = torch.zeros(grid.shape, dtype=torch.float64)
+    right, top = X > 0.0, Y > 0.0
+
+    def full(v):
+        return torch.full_like(z, float(v))
+
+    def pick(tl, tr, bl, br):
+        return torch.where(top, torch.where(right, full(tr), full(tl)),
+                           torch.where(right, full(br), full(bl)))
+
+    a, b = math.radians(angle_deg), math.radians(angle_z_deg)
+    ca, sa = math.cos(a), math.sin(a)
+    if angle_deg == 45.0:
+        sa = ca
+    bx, by, bz = B0 * ca * math.cos(b), B0 * sa * math.cos(b), B0 * math.sin(b)
+    P = {"rho": pick(0.1, 0.1, 0.5, 0.1), "p": pick(1.0, 0.01, 1.0, 1.0),
+         "vx": pick(0.99, 0.0, 0.0, 0.0), "vy": pick(0.0, 0.0, 0.0, 0.99), "vz": z.clone(),
+         "Bx": full(bx), "By": full(by), "Bz": full(bz)}
+    P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
+    XF, YF = _corners(grid)
+    return P, bx * YF - by * XF
+
+
 PROBLEMS = {"rotor": magnetic_rotor, "rotor_tapered": rotor_tapered,
-            "orszag_tang_2pi": orszag_tang_2pi}
+            "orszag_tang_2pi": orszag_tang_2pi, "orszag_tang": orszag_tang,
+            "orszag_tang_bs": orszag_tang_bs, "riemann2d": riemann2d}
 
 
 def init_state(grid, eos, prims, Az, bc_x="outflow", bc_y="outflow", limiter="mc"):

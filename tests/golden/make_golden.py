@@ -273,6 +273,27 @@ def gen_tubes():
     np.savez_compressed(HERE / "tubes_shared.npz", **out)
 
 
+def gen_init2d():
+    """The reference's remaining 2D initial data (initial_data2d.py:128-270):
+    unit-square Orszag-Tang, Beckwith & Stone's variant and the
+    four-quadrant Riemann problem (coplanar 45 degrees and tilted), on a 24^2
+    grid with two ghosts: cell primitives and the corner potential."""
+    from src.physics.initial_data2d import orszag_tang, orszag_tang_bs, riemann2d
+    out = {}
+    cases = {"ot": (orszag_tang, (0.0, 1.0), 4.0 / 3.0, {}),
+             "otbs": (orszag_tang_bs, (0.0, 1.0), 5.0 / 3.0, {}),
+             "r2d": (riemann2d, (-0.5, 0.5), 5.0 / 3.0, {}),
+             "r2dtilt": (riemann2d, (-0.5, 0.5), 5.0 / 3.0, {"B0": 0.7, "angle_z_deg": 30.0})}
+    for name, (fn, box, gamma, kw) in cases.items():
+        eos = hybrid_eos(K=0.0, gamma=gamma, gamma_th=gamma)
+        g = Grid2D(box[0], box[1], 24, box[0], box[1], 24, ng=2)
+        P, Az = fn(g, eos, **kw)
+        for k, v in P.items():
+            out[f"{name}_{k}"] = _np(v)
+        out[f"{name}_Az"] = _np(Az)
+    np.savez_compressed(HERE / "init2d.npz", **out)
+
+
 def gen_host():
     """torch.sqrt of fixed inputs on the host that wrote the fixtures: torch's
     CPU float64 sqrt is not correctly rounded on every host, so bitwise
@@ -283,7 +304,7 @@ def gen_host():
 
 if __name__ == "__main__":
     torch.set_num_threads(4)
-    which = sys.argv[1:] or ["flux", "c2p", "recon", "2d", "1d", "tubes", "host"]
+    which = sys.argv[1:] or ["flux", "c2p", "recon", "2d", "1d", "tubes", "host", "init2d"]
     for w in which:
         print("generating", w, flush=True)
         globals()[f"gen_{w}"]()
