@@ -334,10 +334,8 @@ __global__ __launch_bounds__(BLK) void k_fixup64(G2 g, St2 S, Pr2 P, srmhd_eos e
     h.run(l, r, e);
     Kastaun k;
     k.init(h.uH, e);
-    C2PRec rec;
-    const uint64_t m = c2p_pass_a(k, rec);
     Prim ph;
-    c2p_pass_b(k, rec, c2p_break_iter(m), 1e-10, 1e-10, ph);
+    c2p_single(k, 1e-10, 1e-10, ph);   // the face's HLL state as its own batch
     double f[8], ps;
     int it;
     hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, nullptr, ps, it);

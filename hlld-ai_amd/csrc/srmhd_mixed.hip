@@ -1,8 +1,8 @@
-// Mixed-precision sweeps of the 2D stage: reconstruction and the Riemann
-// solver's iterative work (HLLD's p* root and the HLL-state con2prim that
-// seeds it; HLLE / HLLC whole) in FP32, everything else of the stage (HLLD's
-// FP64 polish of p* and its flux assembly, corner EMF, CT face-B update, RK
-// accumulation, con2prim) in FP64.
+// Mixed-precision sweeps of the 2D stage.  Form MIXED: reconstruction and
+// the Riemann solver in FP32 (HLLD with its HLL-state con2prim per face).
+// Form MIXED_R64: reconstruction in FP64, HLLD's p* root found in FP32 and
+// polished, and its flux assembled, in FP64.  Everything else of the stage
+// (corner EMF, CT face-B update, RK accumulation, con2prim) is FP64.
 //
 // One kernel per sweep (no pass A: the FP32 HLL-state con2prim is per
 // element).  It reads the FP64 state, forms the reconstructed variables
@@ -200,9 +200,15 @@ __global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, srmhd_eos
     Prim l64, r64;
     face_prims_f32<N, LIM, R64>(g, S, P, e64, e, i, j, l, r, l64, r64);
     double f[8];
-    if (SOL == SOLVER_HLLD) {
+    if (SOL == SOLVER_HLLD && R64) {
       st = hlld_mixed<N>(l, r, e, l64, r64, e64, f);
       if (st & f32::FS_DOUBT) fix[2 + atomicAdd(fix, 1u)] = (unsigned)t;
+    } else if (SOL == SOLVER_HLLD) {
+      float f3[8];
+      st = f32::hlld<N>(l, r, e, f3);
+      if (st & f32::FS_DOUBT) fix[2 + atomicAdd(fix, 1u)] = (unsigned)t;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) f[k] = (double)f3[k];
     } else {
       float f3[8];
       if (SOL == SOLVER_HLLE) {

@@ -101,12 +101,12 @@ typedef struct srmhd_grid2d {
 
 /* precision: SRMHD_PREC_FP64 (the reference's float64 arithmetic, bit-faithful)
  * or one of two mixed forms with no counterpart in the reference:
- *   SRMHD_PREC_MIXED      reconstruction in FP32; the Riemann solver's
- *                         iterative work in FP32 (HLLD's p* root and the
- *                         HLL-state con2prim seeding its bracket, solved per
- *                         face; HLLE / HLLC whole), HLLD's p* polished and its
- *                         flux assembled in FP64;
- *   SRMHD_PREC_MIXED_R64  the same with the reconstruction in FP64.
+ *   SRMHD_PREC_MIXED      reconstruction and the Riemann solver in FP32 (HLLD
+ *                         with the con2prim of its HLL state solved per face);
+ *                         faces whose HLLE-fallback decision FP32 cannot settle
+ *                         are recomputed in FP64;
+ *   SRMHD_PREC_MIXED_R64  reconstruction in FP64; HLLD's p* root found in FP32,
+ *                         polished and its flux assembled in FP64.
  * In both, corner EMF, CT face-B update and RK accumulation are FP64 (div B
  * is untouched) and the cell con2prim is FP64 with each cell its own batch. */
 enum { SRMHD_PREC_FP64 = 0, SRMHD_PREC_MIXED = 1, SRMHD_PREC_MIXED_R64 = 2 };
