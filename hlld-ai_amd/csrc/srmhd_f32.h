@@ -640,6 +640,15 @@ F32_DEV bool hlld_pstar(const Prim& L, const Prim& Rp, const Eos& e, float& ps, 
 template <int N>
 F32_DEV int hlld(const Prim& L, const Prim& Rp, const Eos& e, float F[8]) {
   constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+  // B_n = 0 on both sides: the flux is the HLLE fallback whatever p* is
+  // (hlld.py:1531), so the root is not sought
+  if ((fabsf(L.B[N]) < 1e-14f) && (fabsf(Rp.B[N]) < 1e-14f)) {
+    float uL[8], uR[8], fL[8], fR[8], cmax, cmin;
+    srmhd_fluxes<N>(L, Rp, e, uL, uR, fL, fR, cmax, cmin);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) F[k] = hll_flux(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
+    return FS_BN_ZERO | FS_FALLBACK;
+  }
   float ps, p_hll;
   const bool conv = hlld_pstar<N>(L, Rp, e, ps, p_hll);
   float uL[8], uR[8], fL[8], fR[8], cmax, cmin;
@@ -651,7 +660,7 @@ F32_DEV int hlld(const Prim& L, const Prim& Rp, const Eos& e, float F[8]) {
     uH[k] = hll_state(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
   }
   const float BnL = uL[CB + N], BnR = uR[CB + N];
-  const bool bn_zero = (fabsf(BnL) < 1e-14f) && (fabsf(BnR) < 1e-14f);
+  const bool bn_zero = false;
   StarR R;
   R.cm = cmin;
   R.cx = cmax;
