@@ -38,7 +38,8 @@ FP64_FLOPS_PER_ELEMENT = {("fp64", "sweep_x_a"): 18878.0, ("fp64", "sweep_y_a"):
 
 # the kernel class each form spends most time in (rocprofv3 --stats of the
 # 2048^2 rotor: profiles/rotor2048_weno5z_hlld_kernel_stats.md)
-DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "sweep_x_b"}
+DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "sweep_x_b",
+                   "fp64_local": "sweep_x_b"}
 
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
@@ -54,6 +55,9 @@ def algorithmic_bytes(kernel, n, ng, precision="fp64"):
     nt = n + 2 * ng
     cells = nt * nt
     faces = (nt + 1) * nt
+    if precision == "fp64_local" and kernel in ("sweep_x_b", "sweep_y_b"):
+        # the FP64 sweep without a pass-A record
+        return 8 * (6 * cells + 2 * faces) + 56 * faces
     if precision != "fp64" and kernel in ("sweep_x_b", "sweep_y_b"):
         # the one FP32 sweep kernel: prims rho, p, vx, vy, vz + cons Bz +
         # both face-B arrays in, 7 FP64 flux arrays out
@@ -252,6 +256,26 @@ def _rotor_form(a, precision, kernel, U_init, P_init):
             "ms_per_step": 1e3 * wall / a.steps}
 
 
+def _rotor_l1_floor(a, U_init, P_init):
+    """The FP64 form again, from the initial state with the conserved density
+    D raised by one ulp (nextafter) in every cell: how far 200 steps of this
+    problem carry a round-off-sized difference.  Untimed."""
+    import torch
+    from srmhd import driver2d as D2, eos as E
+
+    eos = E.hybrid_eos(0.0, 5.0 / 3.0, 5.0 / 3.0)
+    step = D2.Stepper2D(a.grid, eos, solver=a.solver, limiter=a.limiter, precision="fp64")
+    U = [t.clone() for t in U_init]
+    U[0] = torch.nextafter(U[0], torch.full_like(U[0], float("inf")))
+    step.load(U, [t.clone() for t in P_init])
+    for _ in range(a.steps):
+        step.compute_dt(step.U, step.P, a.cfl)
+        step.step("rk3")
+    torch.cuda.synchronize()
+    print(f"fp64 (D + 1 ulp): {a.steps} steps done", file=sys.stderr, flush=True)
+    return step
+
+
 def _rotor_fields(step, n, ng):
     """rho, p, |B| (cell-centred from the faces) over the physical cells and
     div B: max |div B| and max |div B| / (max |B_face| / dx)."""
@@ -442,6 +466,8 @@ def main():
                          "of each form)")
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-floor", action="store_true",
+                    help="skip the untimed FP64 run from the one-ulp-perturbed initial state")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed region, copy a 512 MiB float64 buffer three times "
                          "with an elementwise kernel (the plain copy kernel that scales the "
@@ -510,7 +536,9 @@ def main():
                  "mixed": "reconstruction + Riemann-solver root finding in FP32; HLLD flux "
                           "assembly, con2prim, CT and RK in FP64",
                  "mixed_r64": "Riemann-solver root finding in FP32; reconstruction, HLLD flux "
-                              "assembly, con2prim, CT and RK in FP64"}
+                              "assembly, con2prim, CT and RK in FP64",
+                 "fp64_local": "float64 throughout; every con2prim solved per element to its "
+                               "own convergence instead of batch-coupled"}
     out = {
         "metric": "zone-cycles/s, 2D rotor 2048^2 (FP64 vs FP32-flux @ L1<=1e-6); HBM GB/s",
         "value": h["zc"], "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps,
@@ -534,9 +562,12 @@ def main():
         f = res["fp64"]
         ra, pa, ba, _ = fields["fp64"]
 
-        def l1_of(form):
-            rb, pb, bb, _ = fields[form]
+        def l1_of_fields(fb):
+            rb, pb, bb, _ = fb
             return {"rho": _l1(rb, ra), "p": _l1(pb, pa), "absB": _l1(bb, ba)}
+
+        def l1_of(form):
+            return l1_of_fields(fields[form])
 
         l1 = l1_of(head)
         out["fp64_form"] = {"value": f["zc"], "ms_per_step": f["ms_per_step"],
@@ -545,6 +576,14 @@ def main():
         out["speedup_vs_fp64"] = h["zc"] / f["zc"]
         out["l1_vs_fp64"] = l1
         out["l1_rho_gate_1e-6_met"] = l1["rho"] <= 1e-6
+        if not a.no_floor:
+            fl = _rotor_l1_floor(a, U_init, P_init)
+            out["l1_floor"] = {
+                "l1_vs_fp64": l1_of_fields(_rotor_fields(fl, a.n, ng)),
+                "description": "the FP64 form from the initial state with D raised by one ulp "
+                               "in every cell, after the same steps: the distance round-off "
+                               "alone reaches on this problem"}
+            del fl
         out["other_forms"] = {
             form: {"value": res[form]["zc"], "ms_per_step": res[form]["ms_per_step"],
                    "speedup_vs_fp64": res[form]["zc"] / f["zc"], "l1_vs_fp64": l1_of(form),

@@ -278,9 +278,12 @@ SRMHD_DEV int c2p_pass_b(const Kastaun& k, const C2PRec& rec, int K, double atmo
 // FP64 form only moves converged roots by a few ulps).  Returns the status
 // bits of c2p_pass_b.
 // mu_out (may be NULL) receives the root the primitives were finished at.
+// mup_in >= 0 replaces the mu_+ bisection by a given upper bracket end (any
+// value at or above mu_+ keeps the root bracketed; the secant converges to
+// the same root from it).
 SRMHD_DEV int c2p_single(const Kastaun& k, double atmo_rho, double atmo_eps, Prim& P,
-                         double* mu_out = nullptr) {
-  const double mup = k.mu_plus();
+                         double* mu_out = nullptr, double mup_in = -1.0) {
+  const double mup = mup_in >= 0.0 ? mup_in : k.mu_plus();
   double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
   double f0 = k.master(x0), f1 = k.master(x1);
   const bool good = (f0 * f1) < 0.0;

@@ -275,10 +275,13 @@ __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e
   block_or_atomic(&d->c2p_mask, m);
 }
 
-template <int N, int LIM, int SOL>
+// LOCAL: the con2prim of each face's HLL state is solved as its own batch
+// (c2p_single) instead of finishing pass A's batch-coupled iteration
+template <int N, int LIM, int SOL, bool LOCAL = false>
 __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e,
                                                  const C2PRec* rec, srmhd_flux_diag* d, Flx F,
                                                  double* pstar_out) {
+  static_assert(!LOCAL || SOL == SOLVER_HLLD, "the local con2prim concerns HLLD only");
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = t < n_faces<N>(g);
   int st = 0, iters = 0;
@@ -299,7 +302,8 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
       Kastaun k;
       k.init(h.uH, e);
       Prim ph;
-      const int cst = c2p_pass_b(k, rec[t], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
+      const int cst = LOCAL ? c2p_single(k, 1e-10, 1e-10, ph)
+                            : c2p_pass_b(k, rec[t], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
       st = hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, nullptr, ps, iters);
       if (cst & C2P_REDONE) st |= FS_C2P_REDONE;
     }
@@ -496,7 +500,7 @@ __global__ __launch_bounds__(BLK) void k_c2p2d_b(G2 g, St2 S, srmhd_eos e, const
   P.p[5][c] = q.eps;
 }
 
-// con2prim of the physical cells, each cell its own batch (the mixed form)
+// con2prim of the physical cells, each cell its own batch (the FP64_LOCAL form)
 __global__ __launch_bounds__(BLK) void k_c2p2d_single(G2 g, St2 S, srmhd_eos e, double atmo_rho,
                                                       double atmo_eps, Pr2Out P,
                                                       unsigned long long* redo_count) {
@@ -714,13 +718,33 @@ void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const W
   tend(SRMHD_K_SWEEP_Y_B, s);
 }
 
+// HLLD sweeps with each face's HLL-state con2prim solved locally (no pass A)
 template <int LIM>
-void sweeps_solver(int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
-                   const Ws2& w, hipStream_t s) {
+void sweeps_local(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const Ws2& w,
+                  hipStream_t s) {
+  const int bx = blocks_for((int64_t)(g.nxt + 1) * g.nyt);
+  const int by = blocks_for((int64_t)g.nxt * (g.nyt + 1));
+  tbegin(SRMHD_K_SWEEP_X_B, s);
+  k_sweep_b<0, LIM, SOLVER_HLLD, true><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, w.Fx,
+                                                          w.psx);
+  tend(SRMHD_K_SWEEP_X_B, s);
+  tbegin(SRMHD_K_SWEEP_Y_B, s);
+  k_sweep_b<1, LIM, SOLVER_HLLD, true><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, w.Fy,
+                                                          w.psy);
+  tend(SRMHD_K_SWEEP_Y_B, s);
+}
+
+template <int LIM>
+void sweeps_solver(int sol, bool local, const G2& g, const St2& S, const Pr2& P,
+                   const srmhd_eos& e, const Ws2& w, hipStream_t s) {
   switch (sol) {
     case SOLVER_HLLE: sweeps<LIM, SOLVER_HLLE>(g, S, P, e, w, s); break;
     case SOLVER_HLLC: sweeps<LIM, SOLVER_HLLC>(g, S, P, e, w, s); break;
-    default: sweeps<LIM, SOLVER_HLLD>(g, S, P, e, w, s);
+    default:
+      if (local)
+        sweeps_local<LIM>(g, S, P, e, w, s);
+      else
+        sweeps<LIM, SOLVER_HLLD>(g, S, P, e, w, s);
   }
 }
 
@@ -761,27 +785,31 @@ void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const P
   if (sol == SOLVER_HLLD) fixup_dispatch<1>(lim, g, S, P, e, fy, w.Fy, s);
 }
 
-void sweeps_dispatch(int lim, int sol, const G2& g, const St2& S, const Pr2& P,
+void sweeps_dispatch(int lim, int sol, bool local, const G2& g, const St2& S, const Pr2& P,
                      const srmhd_eos& e, const Ws2& w, hipStream_t s) {
   switch (lim) {
-    case LIM_PCM: sweeps_solver<LIM_PCM>(sol, g, S, P, e, w, s); break;
-    case LIM_MC: sweeps_solver<LIM_MC>(sol, g, S, P, e, w, s); break;
-    case LIM_MINMOD: sweeps_solver<LIM_MINMOD>(sol, g, S, P, e, w, s); break;
-    case LIM_WENO5Z: sweeps_solver<LIM_WENO5Z>(sol, g, S, P, e, w, s); break;
-    case LIM_MP5: sweeps_solver<LIM_MP5>(sol, g, S, P, e, w, s); break;
-    default: sweeps_solver<LIM_MP7>(sol, g, S, P, e, w, s);
+    case LIM_PCM: sweeps_solver<LIM_PCM>(sol, local, g, S, P, e, w, s); break;
+    case LIM_MC: sweeps_solver<LIM_MC>(sol, local, g, S, P, e, w, s); break;
+    case LIM_MINMOD: sweeps_solver<LIM_MINMOD>(sol, local, g, S, P, e, w, s); break;
+    case LIM_WENO5Z: sweeps_solver<LIM_WENO5Z>(sol, local, g, S, P, e, w, s); break;
+    case LIM_MP5: sweeps_solver<LIM_MP5>(sol, local, g, S, P, e, w, s); break;
+    default: sweeps_solver<LIM_MP7>(sol, local, g, S, P, e, w, s);
   }
 }
 
 // con2prim of every cell of state S as one batch, then primitive BCs;
 // single: each cell its own batch (the mixed form)
 void c2p_cells(const G2& g, const St2& S, const srmhd_eos& e, double atmo_rho, double atmo_eps,
-               const Ws2& w, const Pr2Out& P, hipStream_t s, bool single = false) {
+               const Ws2& w, const Pr2Out& P, hipStream_t s, bool single = false,
+               bool mup32 = false) {
   const int64_t ncell = (int64_t)g.nxt * g.nyt;
   if (single) {
     tbegin(SRMHD_K_C2P_A, s);
-    k_c2p2d_single<<<blocks_for((int64_t)g.nx * g.ny), BLK, 0, s>>>(g, S, e, atmo_rho, atmo_eps,
-                                                                    P, w.words + 3);
+    if (mup32)
+      c2p_cells_mixed(g, S, e, atmo_rho, atmo_eps, P, w.words + 3, s);
+    else
+      k_c2p2d_single<<<blocks_for((int64_t)g.nx * g.ny), BLK, 0, s>>>(g, S, e, atmo_rho,
+                                                                      atmo_eps, P, w.words + 3);
     tend(SRMHD_K_C2P_A, s);
     k_prim_bc<<<blocks_for(ncell), BLK, 0, s>>>(g, P);
     return;
@@ -1105,7 +1133,7 @@ int64_t srmhd2d_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s) 
 static bool scheme_ok(const srmhd_scheme2d* s) {
   return s && s->solver >= 0 && s->solver <= 2 && s->limiter >= 0 && s->limiter <= 5 &&
          (s->precision == SRMHD_PREC_FP64 || s->precision == SRMHD_PREC_MIXED ||
-          s->precision == SRMHD_PREC_MIXED_R64);
+          s->precision == SRMHD_PREC_MIXED_R64 || s->precision == SRMHD_PREC_FP64_LOCAL);
 }
 
 // the deadband thresholds of the upwind EMF, as k_corner forms them
@@ -1155,12 +1183,13 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   hipStream_t st = (hipStream_t)stream;
   k_zero_u64<<<1, 64, 0, st>>>((unsigned long long*)w.diag, N_WORDS);
   k_zero_u64<<<1, 64, 0, st>>>(w.words, 4);
-  const bool mixed = sp->precision != SRMHD_PREC_FP64;
+  const bool mixed = sp->precision == SRMHD_PREC_MIXED || sp->precision == SRMHD_PREC_MIXED_R64;
+  const bool local = sp->precision != SRMHD_PREC_FP64;   // per-element con2prim
   if (mixed) {
     sweeps_mixed(sp->limiter, sp->solver, sp->precision == SRMHD_PREC_MIXED_R64, g, s1, pr, *eos,
                  w, st);
   } else {
-    sweeps_dispatch(sp->limiter, sp->solver, g, s1, pr, *eos, w, st);
+    sweeps_dispatch(sp->limiter, sp->solver, local, g, s1, pr, *eos, w, st);
   }
   if (tape) {
     const int64_t nfx = (int64_t)(g.nxt + 1) * g.nyt, nfy = (int64_t)g.nxt * (g.nyt + 1);
@@ -1179,7 +1208,7 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   tbegin(SRMHD_K_UPDATE, st);
   k_update<<<blocks_for(ncor), BLK, 0, st>>>(g, s0, s1, w.Fx, w.Fy, w.Ez, a, b, c, dt_dev, o);
   tend(SRMHD_K_UPDATE, st);
-  c2p_cells(g, so, *eos, sp->atmo_rho, sp->atmo_eps, w, po, st, mixed);
+  c2p_cells(g, so, *eos, sp->atmo_rho, sp->atmo_eps, w, po, st, local, mixed);
   if (diag_xy) k_copy_diag<<<1, 64, 0, st>>>(w.diag, diag_xy, 2);
   return launch_status();
 }

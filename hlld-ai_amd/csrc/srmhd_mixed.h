@@ -18,4 +18,11 @@ void sweep_mixed(int dir, int lim, int sol, bool r64, const G2& g, const St2& S,
                  const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,
                  hipStream_t s);
 
+// con2prim of the physical cells of S into P for the mixed forms: each cell
+// its own batch, mu_+ bracketed from FP32 (no primitive BCs; the caller
+// applies them)
+void c2p_cells_mixed(const G2& g, const St2& S, const srmhd_eos& e, double atmo_rho,
+                     double atmo_eps, const Pr2Out& P, unsigned long long* redo_count,
+                     hipStream_t s);
+
 }  // namespace srmhd

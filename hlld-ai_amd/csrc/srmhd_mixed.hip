@@ -12,6 +12,7 @@
 // approximate FP32 division / sqrt (this translation unit only).
 #include "srmhd_mixed.h"
 
+#include "srmhd_c2p.h"
 #include "srmhd_f32.h"
 #include "srmhd_grid2d.h"
 #include "srmhd_recon.h"
@@ -259,7 +260,51 @@ void launch_dir(int lim, int sol, const G2& g, const St2& S, const Pr2& P, const
   }
 }
 
+// con2prim of the physical cells, each cell its own batch, with mu_+ from
+// the FP32 bisection: mu_+ only closes the secant's bracket, which any value
+// at or above it does, so the FP32 root inflated by 4e-6 relative (its error
+// is ~2e-7) plus 4e-7 stands in for the 50 FP64 bisection steps; the FP64
+// secant then converges to the same root, |f| < 1e-15.
+__global__ __launch_bounds__(BLK) void k_c2p2d_mixed(G2 g, St2 S, srmhd_eos e, double atmo_rho,
+                                                     double atmo_eps, Pr2Out P,
+                                                     unsigned long long* redo_count) {
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  bool redo = false;
+  if (t < (int64_t)g.nx * g.ny) {
+    const int pi = (int)(t / g.ny), pj = (int)(t - (int64_t)pi * g.ny);
+    const int i = pi + g.ng, j = pj + g.ng;
+    double u[8];
+    cell_cons(g, S, i, j, u);
+    float u32[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) u32[k] = (float)u[k];
+    const f32::Kastaun k32(u32, f32::Eos(e));
+    const double mup = fmin((double)k32.mu_plus() * (1.0 + 4e-6) + 4e-7, 1.0 / (1.0 + TINY));
+    Kastaun k;
+    k.init(u, e);
+    Prim q;
+    const int st = c2p_single(k, atmo_rho, atmo_eps, q, nullptr, mup);
+    redo = st & C2P_REDONE;
+    const int64_t c = (int64_t)i * g.nyt + j;
+    P.p[0][c] = q.rho;
+    P.p[1][c] = q.v[0];
+    P.p[2][c] = q.v[1];
+    P.p[3][c] = q.v[2];
+    P.p[4][c] = q.p;
+    P.p[5][c] = q.eps;
+  }
+  const unsigned long long nr = __popcll(__ballot(redo));
+  if (lane0() && nr) atomicAdd(redo_count, nr);
+}
+
 }  // namespace
+
+void c2p_cells_mixed(const G2& g, const St2& S, const srmhd_eos& e, double atmo_rho,
+                     double atmo_eps, const Pr2Out& P, unsigned long long* redo_count,
+                     hipStream_t s) {
+  k_c2p2d_mixed<<<blocks_for((int64_t)g.nx * g.ny), BLK, 0, s>>>(g, S, e, atmo_rho, atmo_eps, P,
+                                                                  redo_count);
+}
 
 void sweep_mixed(int dir, int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
                  const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,

@@ -100,16 +100,27 @@ typedef struct srmhd_grid2d {
 } srmhd_grid2d;
 
 /* precision: SRMHD_PREC_FP64 (the reference's float64 arithmetic, bit-faithful)
- * or one of two mixed forms with no counterpart in the reference:
+ * or one of three forms with no counterpart in the reference:
+ *   SRMHD_PREC_FP64_LOCAL FP64 arithmetic throughout, but every con2prim (of a
+ *                         cell, and of the HLL state that seeds HLLD's p*) is
+ *                         solved as its own batch and stops at its own
+ *                         convergence (|f| < 1e-15), instead of iterating with
+ *                         its batch until the whole batch has converged;
  *   SRMHD_PREC_MIXED      reconstruction and the Riemann solver in FP32 (HLLD
  *                         with the con2prim of its HLL state solved per face);
  *                         faces whose HLLE-fallback decision FP32 cannot settle
  *                         are recomputed in FP64;
  *   SRMHD_PREC_MIXED_R64  reconstruction in FP64; HLLD's p* root found in FP32,
  *                         polished and its flux assembled in FP64.
- * In both, corner EMF, CT face-B update and RK accumulation are FP64 (div B
- * is untouched) and the cell con2prim is FP64 with each cell its own batch. */
-enum { SRMHD_PREC_FP64 = 0, SRMHD_PREC_MIXED = 1, SRMHD_PREC_MIXED_R64 = 2 };
+ * In both mixed forms, corner EMF, CT face-B update and RK accumulation are
+ * FP64 (div B is untouched) and the cell con2prim is FP64 with each cell its
+ * own batch. */
+enum {
+  SRMHD_PREC_FP64 = 0,
+  SRMHD_PREC_MIXED = 1,
+  SRMHD_PREC_MIXED_R64 = 2,
+  SRMHD_PREC_FP64_LOCAL = 3
+};
 
 typedef struct srmhd_scheme2d {
   int32_t solver, limiter, upwind, precision;

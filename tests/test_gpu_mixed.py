@@ -1,6 +1,7 @@
-"""The mixed-precision form of the 2D stage (reconstruction and the Riemann
-solver in FP32; CT, RK accumulation and con2prim in FP64) against the FP64
-form on the device.
+"""The mixed-precision forms of the 2D stage (reconstruction and the Riemann
+solver in FP32; CT, RK accumulation and con2prim in FP64), and the FP64 form
+whose con2prims stop per element (fp64_local), against the FP64 form on the
+device.
 
 There is no reference counterpart, so these are accuracy bounds, not parity:
 L1 norms of rho, p and |B| after a few steps, div B at round-off (the CT
@@ -54,7 +55,7 @@ def _phys(S, g, ng):
     return S.P[0][sl], S.P[4][sl], Bm, float(div.abs().max()) / (bmax / g.dx)
 
 
-@pytest.mark.parametrize("form", ["mixed", "mixed_r64"])
+@pytest.mark.parametrize("form", ["mixed", "mixed_r64", "fp64_local"])
 def test_mixed_rotor_close_to_fp64_divb_and_determinism(form):
     """Tapered rotor 256^2, WENO5-Z + HLLD, 20 RK3 steps: L1(rho), L1(p),
     L1(|B|) vs FP64 below 3e-5 (measured 7e-6, 8e-6, 2e-6; the rotor's edge
@@ -77,7 +78,7 @@ def test_mixed_rotor_close_to_fp64_divb_and_determinism(form):
         assert torch.equal(u, v)
 
 
-@pytest.mark.parametrize("form", ["mixed", "mixed_r64"])
+@pytest.mark.parametrize("form", ["mixed", "mixed_r64", "fp64_local"])
 @pytest.mark.parametrize("limiter,solver", [("mc", "hlld"), ("mp5", "hlle"), ("pcm", "hllc"),
                                             ("minmod", "hlld"), ("mp7", "hlld")])
 def test_mixed_every_limiter_and_solver(limiter, solver, form):
