@@ -3,6 +3,7 @@
 #pragma once
 
 #include "srmhd_launch.h"
+#include "srmhd_recon.h"
 #include "srmhd_riemann.h"
 
 namespace srmhd {
@@ -43,6 +44,21 @@ __device__ inline void block_max_atomic2(unsigned long long* dst0, unsigned long
     if (a) atomicMax(dst0, a);
     if (b) atomicMax(dst1, b);
   }
+}
+
+// The counters of a 2D sweep are accumulated in DIAG_SHARDS copies, one
+// 128-byte line each, picked by block index: every block ends with up to ten
+// atomics, and ~16 k blocks on one line serialise in the L2 (the sweep whose
+// faces mostly have B_n = 0 ran 2.4x longer on them).  k_fold_diag folds
+// the copies into the srmhd_flux_diag the downstream kernels read.
+constexpr int DIAG_SHARDS = 64;
+constexpr int DIAG_SHARD_BYTES = 128;
+static_assert(sizeof(srmhd_flux_diag) <= DIAG_SHARD_BYTES, "a shard holds one diag");
+
+// this block's copy among the shards starting at base
+__device__ inline srmhd_flux_diag* diag_shard(srmhd_flux_diag* base) {
+  return (srmhd_flux_diag*)((char*)base +
+                            (int64_t)DIAG_SHARD_BYTES * (blockIdx.x & (DIAG_SHARDS - 1)));
 }
 
 // Per-face counters of one flux call (hlld.py:1516-1529) plus the max |F_D|
@@ -163,6 +179,102 @@ __device__ inline void face_ij(const G2& g, int64_t t, int& i, int& j) {
 template <int N>
 __device__ inline int64_t n_faces(const G2& g) {
   return N == 0 ? (int64_t)(g.nxt + 1) * g.nyt : (int64_t)g.nxt * (g.nyt + 1);
+}
+
+// Reconstructed face states of face (i, j) of the sweep along N, as
+// reconstruct_prims (reconstruction.py:272-338, vel_var "Wv") followed by
+// B_n := face value (sweep.py:82-89).
+template <int N, int LIM>
+__device__ void face_prims2d(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, int i,
+                             int j, Prim& L, Prim& R) {
+  constexpr int w = stencil_halfwidth(LIM);
+  constexpr int k0 = 3 - w, k1 = 4 + w;
+  const int n = N == 0 ? g.nxt : g.nyt;
+  const int f = N == 0 ? i : j;
+  auto cidx = [&](int k) -> int64_t {
+    const int c = clampi(f + k - 4, 0, n - 1);
+    return N == 0 ? (int64_t)c * g.nyt + j : (int64_t)i * g.nyt + c;
+  };
+  auto ccoord = [&](int k, int& ci, int& cj) {
+    const int c = clampi(f + k - 4, 0, n - 1);
+    ci = N == 0 ? c : i;
+    cj = N == 0 ? j : c;
+  };
+  double q[8];
+  double ql[7], qr[7];   // rho, p, zx, zy, zz, Bt1, Bt2
+
+  // rho and p first: they decide the PCM fallback of every variable
+#pragma unroll
+  for (int v = 0; v < 2; ++v) {
+    const double* a = P.p[v == 0 ? 0 : 4];
+#pragma unroll
+    for (int k = k0; k <= k1; ++k) q[k] = a[cidx(k)];
+    face_states<LIM>(q, f, n, ql[v], qr[v]);
+  }
+  const bool bad = (ql[0] <= 1e-12) || (qr[0] <= 1e-12) || (ql[1] <= 1e-14) || (qr[1] <= 1e-14);
+  if (bad) {
+#pragma unroll
+    for (int v = 0; v < 2; ++v) {
+      const double* a = P.p[v == 0 ? 0 : 4];
+      ql[v] = a[cidx(f == 0 ? 4 : 3)] + 0.5 * 0.0;
+      qr[v] = a[cidx(f == n ? 3 : 4)] - 0.5 * 0.0;
+    }
+  }
+  // W v with W from clamp(1 - v^2, 1e-14)
+  double W[8];
+#pragma unroll
+  for (int k = k0; k <= k1; ++k) {
+    const int64_t c = cidx(k);
+    const double vx = P.p[1][c], vy = P.p[2][c], vz = P.p[3][c];
+    W[k] = 1.0 / csqrt(clo(1.0 - (sq(vx) + sq(vy) + sq(vz)), 1e-14));
+  }
+  // 5 more variables: zx, zy, zz, B_t1, B_t2
+#pragma unroll
+  for (int v = 2; v < 7; ++v) {
+#pragma unroll
+    for (int k = k0; k <= k1; ++k) {
+      double val;
+      if (v <= 4) {
+        val = W[k] * P.p[v - 1][cidx(k)];
+      } else {
+        // N=0: t1 = y (faces), t2 = z (cons);  N=1: t1 = z (cons), t2 = x (faces)
+        const bool fromBz = (N == 0) ? (v == 6) : (v == 5);
+        if (fromBz) {
+          val = S.p[5][cidx(k)];
+        } else {
+          int ci, cj;
+          ccoord(k, ci, cj);
+          val = (N == 0) ? bcy(g, S.p[7], ci, cj) : bcx(g, S.p[6], ci, cj);
+        }
+      }
+      q[k] = val;
+    }
+    if (bad) {
+      // PCM: the window slots of cells max(f-1, 0) and min(f, n-1)
+      ql[v] = (f == 0 ? q[4] : q[3]) + 0.5 * 0.0;
+      qr[v] = (f == n ? q[3] : q[4]) - 0.5 * 0.0;
+    } else {
+      face_states<LIM>(q, f, n, ql[v], qr[v]);
+    }
+  }
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+  const double Bn = N == 0 ? S.p[6][(int64_t)i * g.nyt + j] : S.p[7][(int64_t)i * (g.nyt + 1) + j];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const double* x = s ? qr : ql;
+    Prim& Q = s ? R : L;
+    const double z2 = sq(x[2]) + sq(x[3]) + sq(x[4]);
+    const double fac = 1.0 / csqrt(1.0 + z2);
+    Q.v[0] = x[2] * fac;
+    Q.v[1] = x[3] * fac;
+    Q.v[2] = x[4] * fac;
+    Q.rho = clo(x[0], 1e-12);
+    Q.p = clo(x[1], 1e-14);
+    Q.eps = eos_eps(e, Q.p, Q.rho);
+    Q.B[N] = Bn;
+    Q.B[T1] = x[5];
+    Q.B[T2] = x[6];
+  }
 }
 
 }  // namespace srmhd

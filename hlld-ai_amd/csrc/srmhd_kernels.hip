@@ -160,105 +160,10 @@ __global__ __launch_bounds__(BLK) void k_p2c(int64_t n, PrimPtrs P, ConsPtrs C) 
   for (int k = 0; k < 8; ++k) C.p[k][i] = c[k];
 }
 
-// Reconstructed face states of face (i, j) of the sweep along N, as
-// reconstruct_prims (reconstruction.py:272-338, vel_var "Wv") followed by
-// B_n := face value (sweep.py:82-89).
-template <int N, int LIM>
-__device__ void face_prims2d(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, int i,
-                             int j, Prim& L, Prim& R) {
-  constexpr int w = stencil_halfwidth(LIM);
-  constexpr int k0 = 3 - w, k1 = 4 + w;
-  const int n = N == 0 ? g.nxt : g.nyt;
-  const int f = N == 0 ? i : j;
-  auto cidx = [&](int k) -> int64_t {
-    const int c = clampi(f + k - 4, 0, n - 1);
-    return N == 0 ? (int64_t)c * g.nyt + j : (int64_t)i * g.nyt + c;
-  };
-  auto ccoord = [&](int k, int& ci, int& cj) {
-    const int c = clampi(f + k - 4, 0, n - 1);
-    ci = N == 0 ? c : i;
-    cj = N == 0 ? j : c;
-  };
-  double q[8];
-  double ql[7], qr[7];   // rho, p, zx, zy, zz, Bt1, Bt2
-
-  // rho and p first: they decide the PCM fallback of every variable
-#pragma unroll
-  for (int v = 0; v < 2; ++v) {
-    const double* a = P.p[v == 0 ? 0 : 4];
-#pragma unroll
-    for (int k = k0; k <= k1; ++k) q[k] = a[cidx(k)];
-    face_states<LIM>(q, f, n, ql[v], qr[v]);
-  }
-  const bool bad = (ql[0] <= 1e-12) || (qr[0] <= 1e-12) || (ql[1] <= 1e-14) || (qr[1] <= 1e-14);
-  if (bad) {
-#pragma unroll
-    for (int v = 0; v < 2; ++v) {
-      const double* a = P.p[v == 0 ? 0 : 4];
-      ql[v] = a[cidx(f == 0 ? 4 : 3)] + 0.5 * 0.0;
-      qr[v] = a[cidx(f == n ? 3 : 4)] - 0.5 * 0.0;
-    }
-  }
-  // W v with W from clamp(1 - v^2, 1e-14)
-  double W[8];
-#pragma unroll
-  for (int k = k0; k <= k1; ++k) {
-    const int64_t c = cidx(k);
-    const double vx = P.p[1][c], vy = P.p[2][c], vz = P.p[3][c];
-    W[k] = 1.0 / csqrt(clo(1.0 - (sq(vx) + sq(vy) + sq(vz)), 1e-14));
-  }
-  // 5 more variables: zx, zy, zz, B_t1, B_t2
-#pragma unroll
-  for (int v = 2; v < 7; ++v) {
-#pragma unroll
-    for (int k = k0; k <= k1; ++k) {
-      double val;
-      if (v <= 4) {
-        val = W[k] * P.p[v - 1][cidx(k)];
-      } else {
-        // N=0: t1 = y (faces), t2 = z (cons);  N=1: t1 = z (cons), t2 = x (faces)
-        const bool fromBz = (N == 0) ? (v == 6) : (v == 5);
-        if (fromBz) {
-          val = S.p[5][cidx(k)];
-        } else {
-          int ci, cj;
-          ccoord(k, ci, cj);
-          val = (N == 0) ? bcy(g, S.p[7], ci, cj) : bcx(g, S.p[6], ci, cj);
-        }
-      }
-      q[k] = val;
-    }
-    if (bad) {
-      // PCM: the window slots of cells max(f-1, 0) and min(f, n-1)
-      ql[v] = (f == 0 ? q[4] : q[3]) + 0.5 * 0.0;
-      qr[v] = (f == n ? q[3] : q[4]) - 0.5 * 0.0;
-    } else {
-      face_states<LIM>(q, f, n, ql[v], qr[v]);
-    }
-  }
-  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
-  const double Bn = N == 0 ? S.p[6][(int64_t)i * g.nyt + j] : S.p[7][(int64_t)i * (g.nyt + 1) + j];
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const double* x = s ? qr : ql;
-    Prim& Q = s ? R : L;
-    const double z2 = sq(x[2]) + sq(x[3]) + sq(x[4]);
-    const double fac = 1.0 / csqrt(1.0 + z2);
-    Q.v[0] = x[2] * fac;
-    Q.v[1] = x[3] * fac;
-    Q.v[2] = x[4] * fac;
-    Q.rho = clo(x[0], 1e-12);
-    Q.p = clo(x[1], 1e-14);
-    Q.eps = eos_eps(e, Q.p, Q.rho);
-    Q.B[N] = Bn;
-    Q.B[T1] = x[5];
-    Q.B[T2] = x[6];
-  }
-}
 
 template <int N, int LIM>
 __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e, C2PRec* rec,
-                                                 srmhd_flux_diag* d) {
+                                                 srmhd_flux_diag* dsh) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   uint64_t m = 0;
   if (t < n_faces<N>(g)) {
@@ -272,14 +177,15 @@ __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e
     k.init(h.uH, e);
     m = c2p_pass_a(k, rec[t]);
   }
-  block_or_atomic(&d->c2p_mask, m);
+  block_or_atomic(&diag_shard(dsh)->c2p_mask, m);
 }
 
 // LOCAL: the con2prim of each face's HLL state is solved as its own batch
 // (c2p_single) instead of finishing pass A's batch-coupled iteration
 template <int N, int LIM, int SOL, bool LOCAL = false>
 __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e,
-                                                 const C2PRec* rec, srmhd_flux_diag* d, Flx F,
+                                                 const C2PRec* rec, const srmhd_flux_diag* d,
+                                                 srmhd_flux_diag* dsh, Flx F,
                                                  double* pstar_out) {
   static_assert(!LOCAL || SOL == SOLVER_HLLD, "the local con2prim concerns HLLD only");
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
@@ -318,7 +224,30 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
     if (pstar_out) pstar_out[t] = ps;
   }
   // the deadband scale of the upwind EMF is needed for every solver
-  diag_accumulate(d, active, st, iters, dbits(fabs(fD)));
+  diag_accumulate(diag_shard(dsh), active, st, iters, dbits(fabs(fD)));
+}
+
+// d[k] = the fold of the DIAG_SHARDS copies of sweep k (k < blockDim.x / 64):
+// sums of the counts, maxima, OR of the con2prim mask.  One wave per sweep.
+__global__ void k_fold_diag(const srmhd_flux_diag* dsh, srmhd_flux_diag* d) {
+  static_assert(DIAG_SHARDS == 64, "one lane per shard");
+  const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const srmhd_flux_diag& c = *(const srmhd_flux_diag*)((const char*)dsh +
+                                                       (int64_t)DIAG_SHARD_BYTES *
+                                                           (k * DIAG_SHARDS + lane));
+  srmhd_flux_diag r;
+  r.n_interfaces = wave_sum(c.n_interfaces);
+  r.n_not_converged = wave_sum(c.n_not_converged);
+  r.n_wave_order_bad = wave_sum(c.n_wave_order_bad);
+  r.n_bn_zero = wave_sum(c.n_bn_zero);
+  r.n_hlle_fallback = wave_sum(c.n_hlle_fallback);
+  r.sum_iters_converged = wave_sum(c.sum_iters_converged);
+  r.n_converged = wave_sum(c.n_converged);
+  r.max_iters = wave_max(c.max_iters);
+  r.c2p_mask = wave_or(c.c2p_mask);
+  r.absF_D_max = wave_max(c.absF_D_max);
+  r.n_c2p_redo = wave_sum(c.n_c2p_redo);
+  if (lane == 0) d[k] = r;
 }
 
 // FP64 recomputation of the faces the mixed-precision sweep flagged (its
@@ -672,10 +601,12 @@ struct Ws2 {
   C2PRec *recx, *recy, *recc;
   srmhd_flux_diag* diag;        // [0] x sweep, [1] y sweep
   unsigned long long* words;    // [0] cell c2p mask, [1..2] dt maxima
+  Cells32 cf{};                 // FP32 cell variables of the MIXED form (carved for it only)
+  srmhd_flux_diag* dsh;         // DIAG_SHARDS copies of diag[0], then of diag[1]
   int64_t bytes;
 };
 
-Ws2 carve2d(const G2& g, void* work) {
+Ws2 carve2d(const G2& g, void* work, bool cells32 = false) {
   Carver c{(char*)work};
   Ws2 w;
   const int64_t nfx = (int64_t)(g.nxt + 1) * g.nyt, nfy = (int64_t)g.nxt * (g.nyt + 1);
@@ -688,11 +619,16 @@ Ws2 carve2d(const G2& g, void* work) {
   w.recx = c.take<C2PRec>(nfx);
   w.recy = c.take<C2PRec>(nfy);
   w.recc = c.take<C2PRec>(ncell);
+  w.dsh = (srmhd_flux_diag*)c.take<char>(2 * DIAG_SHARDS * DIAG_SHARD_BYTES);
+  if (cells32)
+    for (int k = 0; k < 8; ++k) w.cf.p[k] = c.take<float>(ncell);
   w.bytes = c.off;
   return w;
 }
 
 constexpr int N_WORDS = (int)(2 * sizeof(srmhd_flux_diag) / 8);
+constexpr int DIAG_SHARD_WORDS = 2 * DIAG_SHARDS * DIAG_SHARD_BYTES / 8;
+constexpr int DIAG_ZERO_BLOCKS = (DIAG_SHARD_WORDS + 255) / 256;
 
 // the two sweeps of ct_stage for one (limiter, solver)
 template <int LIM, int SOL>
@@ -700,22 +636,27 @@ void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const W
             hipStream_t s) {
   const int bx = blocks_for((int64_t)(g.nxt + 1) * g.nyt);
   const int by = blocks_for((int64_t)g.nxt * (g.nyt + 1));
+  srmhd_flux_diag* shx = w.dsh;
+  srmhd_flux_diag* shy = (srmhd_flux_diag*)((char*)w.dsh + DIAG_SHARDS * DIAG_SHARD_BYTES);
   if (SOL == SOLVER_HLLD) {
     tbegin(SRMHD_K_SWEEP_X_A, s);
-    k_sweep_a<0, LIM><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0);
+    k_sweep_a<0, LIM><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, shx);
     tend(SRMHD_K_SWEEP_X_A, s);
+    k_fold_diag<<<1, 64, 0, s>>>(shx, w.diag + 0);   // the batch mask for pass B
   }
   tbegin(SRMHD_K_SWEEP_X_B, s);
-  k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, w.Fx, w.psx);
+  k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, shx, w.Fx, w.psx);
   tend(SRMHD_K_SWEEP_X_B, s);
   if (SOL == SOLVER_HLLD) {
     tbegin(SRMHD_K_SWEEP_Y_A, s);
-    k_sweep_a<1, LIM><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1);
+    k_sweep_a<1, LIM><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, shy);
     tend(SRMHD_K_SWEEP_Y_A, s);
+    k_fold_diag<<<1, 64, 0, s>>>(shy, w.diag + 1);
   }
   tbegin(SRMHD_K_SWEEP_Y_B, s);
-  k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, w.Fy, w.psy);
+  k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, shy, w.Fy, w.psy);
   tend(SRMHD_K_SWEEP_Y_B, s);
+  k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
 }
 
 // HLLD sweeps with each face's HLL-state con2prim solved locally (no pass A)
@@ -724,14 +665,17 @@ void sweeps_local(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, c
                   hipStream_t s) {
   const int bx = blocks_for((int64_t)(g.nxt + 1) * g.nyt);
   const int by = blocks_for((int64_t)g.nxt * (g.nyt + 1));
+  srmhd_flux_diag* shx = w.dsh;
+  srmhd_flux_diag* shy = (srmhd_flux_diag*)((char*)w.dsh + DIAG_SHARDS * DIAG_SHARD_BYTES);
   tbegin(SRMHD_K_SWEEP_X_B, s);
-  k_sweep_b<0, LIM, SOLVER_HLLD, true><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, w.Fx,
-                                                          w.psx);
+  k_sweep_b<0, LIM, SOLVER_HLLD, true><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, shx,
+                                                          w.Fx, w.psx);
   tend(SRMHD_K_SWEEP_X_B, s);
   tbegin(SRMHD_K_SWEEP_Y_B, s);
-  k_sweep_b<1, LIM, SOLVER_HLLD, true><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, w.Fy,
-                                                          w.psy);
+  k_sweep_b<1, LIM, SOLVER_HLLD, true><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, shy,
+                                                          w.Fy, w.psy);
   tend(SRMHD_K_SWEEP_Y_B, s);
+  k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
 }
 
 template <int LIM>
@@ -775,14 +719,18 @@ void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const P
   unsigned* fy = (unsigned*)w.recy;
   k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)fx, 1);
   k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)fy, 1);
+  if (!r64) cells_f32(g, S, P, w.cf, s);
+  srmhd_flux_diag* shx = w.dsh;
+  srmhd_flux_diag* shy = (srmhd_flux_diag*)((char*)w.dsh + DIAG_SHARDS * DIAG_SHARD_BYTES);
   tbegin(SRMHD_K_SWEEP_X_B, s);
-  sweep_mixed(0, lim, sol, r64, g, S, P, e, w.diag + 0, w.Fx, fx, s);
+  sweep_mixed(0, lim, sol, r64, g, S, P, w.cf, e, shx, w.Fx, fx, s);
   tend(SRMHD_K_SWEEP_X_B, s);
   if (sol == SOLVER_HLLD) fixup_dispatch<0>(lim, g, S, P, e, fx, w.Fx, s);
   tbegin(SRMHD_K_SWEEP_Y_B, s);
-  sweep_mixed(1, lim, sol, r64, g, S, P, e, w.diag + 1, w.Fy, fy, s);
+  sweep_mixed(1, lim, sol, r64, g, S, P, w.cf, e, shy, w.Fy, fy, s);
   tend(SRMHD_K_SWEEP_Y_B, s);
   if (sol == SOLVER_HLLD) fixup_dispatch<1>(lim, g, S, P, e, fy, w.Fy, s);
+  k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
 }
 
 void sweeps_dispatch(int lim, int sol, bool local, const G2& g, const St2& S, const Pr2& P,
@@ -1127,7 +1075,7 @@ int srmhd_p2c(int64_t n, const double* const* prim, double* const* cons, void* s
 
 int64_t srmhd2d_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s) {
   if (!g || !s || !grid_ok(g, s->limiter)) return -1;
-  return carve2d(make_g2(g), nullptr).bytes;
+  return carve2d(make_g2(g), nullptr, s->precision == SRMHD_PREC_MIXED).bytes;
 }
 
 static bool scheme_ok(const srmhd_scheme2d* s) {
@@ -1155,7 +1103,7 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
                !tape->FD_x || !tape->FD_y || !tape->thresh))
     return SRMHD_EARG;
   const G2 g = make_g2(gp);
-  Ws2 w = carve2d(g, work);
+  Ws2 w = carve2d(g, work, sp->precision == SRMHD_PREC_MIXED);
   if (work_bytes < w.bytes) return SRMHD_EARG;
   if (tape) {
     w.psx = tape->pstar_x;
@@ -1182,6 +1130,7 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   (void)pro;
   hipStream_t st = (hipStream_t)stream;
   k_zero_u64<<<1, 64, 0, st>>>((unsigned long long*)w.diag, N_WORDS);
+  k_zero_u64<<<DIAG_ZERO_BLOCKS, 256, 0, st>>>((unsigned long long*)w.dsh, DIAG_SHARD_WORDS);
   k_zero_u64<<<1, 64, 0, st>>>(w.words, 4);
   const bool mixed = sp->precision == SRMHD_PREC_MIXED || sp->precision == SRMHD_PREC_MIXED_R64;
   const bool local = sp->precision != SRMHD_PREC_FP64;   // per-element con2prim

@@ -9,14 +9,24 @@ inline int64_t n_faces_host(const G2& g, int dir) {
   return dir == 0 ? (int64_t)(g.nxt + 1) * g.nyt : (int64_t)g.nxt * (g.nyt + 1);
 }
 
+// The reconstructed variables of every cell (ghosts included) in FP32, one
+// array each: rho, p, W vx, W vy, W vz, cell-centred Bx, By, and Bz.
+struct Cells32 {
+  float* p[8];
+};
+
+// Fill C from the primitives P and the state S (face B) of one stage.
+void cells_f32(const G2& g, const St2& S, const Pr2& P, const Cells32& C, hipStream_t s);
+
 // One mixed-precision sweep along dir (0: x, 1: y) into the FP64 flux arrays
-// F, with the flux counters (and the |F_D| max of the upwind EMF) into *d,
+// F, with the flux counters (and the |F_D| max of the upwind EMF) into the
+// DIAG_SHARDS copies starting at d (folded by the caller),
 // and the faces to recompute in FP64 appended to fix (fix[0] = count, zeroed
 // by the caller; indices from fix[2]; room for every face).  r64: the
-// reconstruction in FP64 (else in FP32).
+// reconstruction in FP64 from P (else in FP32 from C, filled by cells_f32).
 void sweep_mixed(int dir, int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
-                 const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,
-                 hipStream_t s);
+                 const Cells32& C, const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F,
+                 unsigned* fix, hipStream_t s);
 
 // con2prim of the physical cells of S into P for the mixed forms: each cell
 // its own batch, mu_+ bracketed from FP32 (no primitive BCs; the caller

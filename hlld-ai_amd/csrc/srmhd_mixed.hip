@@ -5,11 +5,12 @@
 // (corner EMF, CT face-B update, RK accumulation, con2prim) is FP64.
 //
 // One kernel per sweep (no pass A: the FP32 HLL-state con2prim is per
-// element).  It reads the FP64 state, forms the reconstructed variables
-// (rho, p, W v with W from FP64, B) in FP64, reconstructs their deviations
-// from the centre cell in FP32, and writes the same seven FP64 flux arrays as
-// the FP64 sweeps, so the downstream kernels are shared.  Built with FP32 contraction and
-// approximate FP32 division / sqrt (this translation unit only).
+// element), writing the same seven FP64 flux arrays as the FP64 sweeps, so
+// the downstream kernels are shared.  MIXED reconstructs from FP32 copies of
+// each cell's reconstructed variables (rho, p, W v, cell-centred B), formed
+// once per stage by k_cells_f32; MIXED_R64 reads the FP64 primitives.
+// Built with FP32 contraction and approximate FP32 division / sqrt (this
+// translation unit only).
 #include "srmhd_mixed.h"
 
 #include "srmhd_c2p.h"
@@ -20,113 +21,109 @@
 namespace srmhd {
 namespace {
 
-// face_prims2d in FP32 (reconstruction.py:272-338 with vel_var "Wv", then
-// B_n := face value, sweep.py:82-89).  Every limiter is translation
-// equivariant, so each variable is reconstructed in FP32 as the deviation
-// from the stencil's centre cell (formed in FP64) and the centre value is
-// added back in FP64: the FP32 arithmetic then only touches the small
-// differences, and the face value carries one FP32 rounding.
-template <int N, int LIM, bool RECON64>
-__device__ void face_prims_f32(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e64,
-                               const f32::Eos& e, int i, int j, f32::Prim& L, f32::Prim& R,
-                               Prim& L64, Prim& R64) {
+// the reconstructed variables of every cell in FP32 (the W v and cell-centred
+// B of reconstruction.py:272-300 and sweep.py:82-89, formed in FP64 and
+// rounded once)
+__global__ __launch_bounds__(BLK) void k_cells_f32(G2 g, St2 S, Pr2 P, Cells32 C) {
+  const int64_t c = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (c >= (int64_t)g.nxt * g.nyt) return;
+  const int i = (int)(c / g.nyt), j = (int)(c - (int64_t)i * g.nyt);
+  const double vx = P.p[1][c], vy = P.p[2][c], vz = P.p[3][c];
+  const double W = 1.0 / __builtin_sqrt(fmax(1.0 - (vx * vx + vy * vy + vz * vz), 1e-14));
+  C.p[0][c] = (float)P.p[0][c];
+  C.p[1][c] = (float)P.p[4][c];
+  C.p[2][c] = (float)(W * vx);
+  C.p[3][c] = (float)(W * vy);
+  C.p[4][c] = (float)(W * vz);
+  C.p[5][c] = (float)bcx(g, S.p[6], i, j);
+  C.p[6][c] = (float)bcy(g, S.p[7], i, j);
+  C.p[7][c] = (float)S.p[5][c];
+}
+
+// face_prims2d (reconstruction.py:272-338 with vel_var "Wv", then B_n := face
+// value, sweep.py:82-89) in FP32 from the cell arrays C: rho, p, W v and the
+// two tangential B components reconstructed by the limiter, the PCM fallback
+// where a reconstructed rho or p is not positive.
+template <int N, int LIM>
+__device__ void face_prims_c32(const G2& g, const St2& S, const Cells32& C, const f32::Eos& e,
+                               int i, int j, f32::Prim& L, f32::Prim& R) {
   constexpr int w = stencil_halfwidth(LIM);
   constexpr int k0 = 3 - w, k1 = 4 + w;
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+  // C arrays of the recon variables rho, p, zx, zy, zz, B_t1, B_t2
+  constexpr int src[7] = {0, 1, 2, 3, 4, N == 0 ? 6 : 7, N == 0 ? 7 : 5};
   const int n = N == 0 ? g.nxt : g.nyt;
   const int f = N == 0 ? i : j;
-  auto cidx = [&](int k) -> int64_t {
-    const int c = clampi(f + k - 4, 0, n - 1);
-    return N == 0 ? (int64_t)c * g.nyt + j : (int64_t)i * g.nyt + c;
-  };
-  double a[8];
-  float q[8];
-  double ql[7], qr[7];   // rho, p, zx, zy, zz, B_t1, B_t2
-  // deviation reconstruction of a[k0..k1] about a[3]
-  auto recon = [&](double& l, double& r) {
-    if (RECON64) {   // the FP64 path's reconstruction
-      face_states<LIM>(a, f, n, l, r);
-      return;
-    }
-    const double base = a[3];
+  const int64_t stride = N == 0 ? g.nyt : 1;
+  const int64_t base = N == 0 ? (int64_t)j : (int64_t)i * g.nyt;
+  int64_t idx[8];
 #pragma unroll
-    for (int k = k0; k <= k1; ++k) q[k] = (float)(a[k] - base);
-    float fl, fr;
-    f32::face_states<LIM>(q, f, n, fl, fr);
-    l = base + (double)fl;
-    r = base + (double)fr;
-  };
+  for (int k = k0; k <= k1; ++k) idx[k] = base + (int64_t)clampi(f + k - 4, 0, n - 1) * stride;
+  float ql[7], qr[7], q[8];
 #pragma unroll
   for (int v = 0; v < 2; ++v) {
-    const double* src = P.p[v == 0 ? 0 : 4];
 #pragma unroll
-    for (int k = k0; k <= k1; ++k) a[k] = src[cidx(k)];
-    recon(ql[v], qr[v]);
+    for (int k = k0; k <= k1; ++k) q[k] = C.p[src[v]][idx[k]];
+    f32::face_states<LIM>(q, f, n, ql[v], qr[v]);
   }
-  const bool bad = (ql[0] <= 1e-12) || (qr[0] <= 1e-12) || (ql[1] <= 1e-14) || (qr[1] <= 1e-14);
+  const bool bad = (ql[0] <= 1e-12f) || (qr[0] <= 1e-12f) || (ql[1] <= 1e-14f) || (qr[1] <= 1e-14f);
   if (bad) {
 #pragma unroll
     for (int v = 0; v < 2; ++v) {
-      const double* src = P.p[v == 0 ? 0 : 4];
-      ql[v] = src[cidx(f == 0 ? 4 : 3)];
-      qr[v] = src[cidx(f == n ? 3 : 4)];
+      ql[v] = C.p[src[v]][idx[f == 0 ? 4 : 3]];
+      qr[v] = C.p[src[v]][idx[f == n ? 3 : 4]];
     }
-  }
-  double W[8];
-#pragma unroll
-  for (int k = k0; k <= k1; ++k) {
-    const int64_t c = cidx(k);
-    const double vx = P.p[1][c], vy = P.p[2][c], vz = P.p[3][c];
-    W[k] = 1.0 / __builtin_sqrt(fmax(1.0 - (vx * vx + vy * vy + vz * vz), 1e-14));
   }
 #pragma unroll
   for (int v = 2; v < 7; ++v) {
 #pragma unroll
-    for (int k = k0; k <= k1; ++k) {
-      const int64_t c = cidx(k);
-      if (v <= 4) {
-        a[k] = W[k] * P.p[v - 1][c];
-      } else {
-        const bool fromBz = (N == 0) ? (v == 6) : (v == 5);
-        const int ci = N == 0 ? clampi(f + k - 4, 0, n - 1) : i;
-        const int cj = N == 0 ? j : clampi(f + k - 4, 0, n - 1);
-        a[k] = fromBz ? S.p[5][c] : ((N == 0) ? bcy(g, S.p[7], ci, cj) : bcx(g, S.p[6], ci, cj));
-      }
-    }
+    for (int k = k0; k <= k1; ++k) q[k] = C.p[src[v]][idx[k]];
     if (bad) {
-      ql[v] = f == 0 ? a[4] : a[3];
-      qr[v] = f == n ? a[3] : a[4];
+      ql[v] = f == 0 ? q[4] : q[3];
+      qr[v] = f == n ? q[3] : q[4];
     } else {
-      recon(ql[v], qr[v]);
+      f32::face_states<LIM>(q, f, n, ql[v], qr[v]);
     }
   }
-  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
   const float Bn = (float)(N == 0 ? S.p[6][(int64_t)i * g.nyt + j]
                                   : S.p[7][(int64_t)i * (g.nyt + 1) + j]);
-  const double Bn64 = N == 0 ? S.p[6][(int64_t)i * g.nyt + j] : S.p[7][(int64_t)i * (g.nyt + 1) + j];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
-    const double* x = s ? qr : ql;
+    const float* x = s ? qr : ql;
     f32::Prim& Q = s ? R : L;
-    Prim& Q64 = s ? R64 : L64;
-    const double fac = 1.0 / __builtin_sqrt(1.0 + (x[2] * x[2] + x[3] * x[3] + x[4] * x[4]));
-    Q64.v[0] = x[2] * fac;
-    Q64.v[1] = x[3] * fac;
-    Q64.v[2] = x[4] * fac;
-    Q64.rho = fmax(x[0], 1e-12);
-    Q64.p = fmax(x[1], 1e-14);
-    Q64.eps = eos_eps(e64, Q64.p, Q64.rho);
-    Q64.B[N] = Bn64;
-    Q64.B[T1] = x[5];
-    Q64.B[T2] = x[6];
+    const float fac = 1.f / f32::fsqrt(1.f + (x[2] * x[2] + x[3] * x[3] + x[4] * x[4]));
+    Q.v[0] = x[2] * fac;
+    Q.v[1] = x[3] * fac;
+    Q.v[2] = x[4] * fac;
+    Q.rho = fmaxf(x[0], 1e-12f);
+    Q.p = fmaxf(x[1], 1e-14f);
+    Q.eps = e.eps_of(Q.p, Q.rho);
+    Q.B[N] = Bn;
+    Q.B[T1] = x[5];
+    Q.B[T2] = x[6];
+  }
+}
+
+// face_prims2d in FP64 (the FP64 path's reconstruction) for MIXED_R64: the
+// face states in FP64 and their FP32 copies.
+template <int N, int LIM>
+__device__ void face_prims_r64(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e64,
+                               const f32::Eos& e, int i, int j, f32::Prim& L, f32::Prim& R,
+                               Prim& L64, Prim& R64) {
+  face_prims2d<N, LIM>(g, S, P, e64, i, j, L64, R64);
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const Prim& Q64 = s ? R64 : L64;
+    f32::Prim& Q = s ? R : L;
     Q.v[0] = (float)Q64.v[0];
     Q.v[1] = (float)Q64.v[1];
     Q.v[2] = (float)Q64.v[2];
-    Q.rho = fmaxf((float)x[0], 1e-12f);
-    Q.p = fmaxf((float)x[1], 1e-14f);
-    Q.eps = e.eps_of(Q.p, Q.rho);
-    Q.B[N] = Bn;
-    Q.B[T1] = (float)x[5];
-    Q.B[T2] = (float)x[6];
+    Q.rho = (float)Q64.rho;
+    Q.p = (float)Q64.p;
+    Q.eps = (float)Q64.eps;
+    Q.B[0] = (float)Q64.B[0];
+    Q.B[1] = (float)Q64.B[1];
+    Q.B[2] = (float)Q64.B[2];
   }
 }
 
@@ -187,7 +184,7 @@ __device__ int hlld_mixed(const f32::Prim& l, const f32::Prim& r, const f32::Eos
 // fix: count + indices of the faces to recompute in FP64 (fix[0] = count,
 // indices from fix[2]); capacity >= the number of faces.
 template <int N, int LIM, int SOL, bool R64>
-__global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, srmhd_eos e64,
+__global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, Cells32 C, srmhd_eos e64,
                                                    srmhd_flux_diag* d, Flx F, unsigned* fix) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = t < n_faces<N>(g);
@@ -199,7 +196,10 @@ __global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, srmhd_eos
     const f32::Eos e(e64);
     f32::Prim l, r;
     Prim l64, r64;
-    face_prims_f32<N, LIM, R64>(g, S, P, e64, e, i, j, l, r, l64, r64);
+    if (R64)
+      face_prims_r64<N, LIM>(g, S, P, e64, e, i, j, l, r, l64, r64);
+    else
+      face_prims_c32<N, LIM>(g, S, C, e, i, j, l, r);
     double f[8];
     if (SOL == SOLVER_HLLD && R64) {
       st = hlld_mixed<N>(l, r, e, l64, r64, e64, f);
@@ -229,34 +229,36 @@ __global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, srmhd_eos
     F.p[6][t] = N == 0 ? f[CB + 1] : f[CB + 0];
     fD = f[CD];
   }
-  diag_accumulate(d, active, st, 0, dbits(fabs(fD)));
+  diag_accumulate(diag_shard(d), active, st, 0, dbits(fabs(fD)));
 }
 
 template <int N, int LIM, bool R64>
-void launch_lim(int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
-                srmhd_flux_diag* d, const Flx& F, unsigned* fix, hipStream_t s) {
+void launch_lim(int sol, const G2& g, const St2& S, const Pr2& P, const Cells32& C,
+                const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,
+                hipStream_t s) {
   const int nb = blocks_for(n_faces_host(g, N));
   switch (sol) {
     case SOLVER_HLLE:
-      k_sweep_f32<N, LIM, SOLVER_HLLE, R64><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix);
+      k_sweep_f32<N, LIM, SOLVER_HLLE, R64><<<nb, BLK, 0, s>>>(g, S, P, C, e, d, F, fix);
       break;
     case SOLVER_HLLC:
-      k_sweep_f32<N, LIM, SOLVER_HLLC, R64><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix);
+      k_sweep_f32<N, LIM, SOLVER_HLLC, R64><<<nb, BLK, 0, s>>>(g, S, P, C, e, d, F, fix);
       break;
-    default: k_sweep_f32<N, LIM, SOLVER_HLLD, R64><<<nb, BLK, 0, s>>>(g, S, P, e, d, F, fix);
+    default: k_sweep_f32<N, LIM, SOLVER_HLLD, R64><<<nb, BLK, 0, s>>>(g, S, P, C, e, d, F, fix);
   }
 }
 
 template <int N, bool R64>
-void launch_dir(int lim, int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
-                srmhd_flux_diag* d, const Flx& F, unsigned* fix, hipStream_t s) {
+void launch_dir(int lim, int sol, const G2& g, const St2& S, const Pr2& P, const Cells32& C,
+                const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,
+                hipStream_t s) {
   switch (lim) {
-    case LIM_PCM: launch_lim<N, LIM_PCM, R64>(sol, g, S, P, e, d, F, fix, s); break;
-    case LIM_MC: launch_lim<N, LIM_MC, R64>(sol, g, S, P, e, d, F, fix, s); break;
-    case LIM_MINMOD: launch_lim<N, LIM_MINMOD, R64>(sol, g, S, P, e, d, F, fix, s); break;
-    case LIM_WENO5Z: launch_lim<N, LIM_WENO5Z, R64>(sol, g, S, P, e, d, F, fix, s); break;
-    case LIM_MP5: launch_lim<N, LIM_MP5, R64>(sol, g, S, P, e, d, F, fix, s); break;
-    default: launch_lim<N, LIM_MP7, R64>(sol, g, S, P, e, d, F, fix, s);
+    case LIM_PCM: launch_lim<N, LIM_PCM, R64>(sol, g, S, P, C, e, d, F, fix, s); break;
+    case LIM_MC: launch_lim<N, LIM_MC, R64>(sol, g, S, P, C, e, d, F, fix, s); break;
+    case LIM_MINMOD: launch_lim<N, LIM_MINMOD, R64>(sol, g, S, P, C, e, d, F, fix, s); break;
+    case LIM_WENO5Z: launch_lim<N, LIM_WENO5Z, R64>(sol, g, S, P, C, e, d, F, fix, s); break;
+    case LIM_MP5: launch_lim<N, LIM_MP5, R64>(sol, g, S, P, C, e, d, F, fix, s); break;
+    default: launch_lim<N, LIM_MP7, R64>(sol, g, S, P, C, e, d, F, fix, s);
   }
 }
 
@@ -306,15 +308,19 @@ void c2p_cells_mixed(const G2& g, const St2& S, const srmhd_eos& e, double atmo_
                                                                   redo_count);
 }
 
+void cells_f32(const G2& g, const St2& S, const Pr2& P, const Cells32& C, hipStream_t s) {
+  k_cells_f32<<<blocks_for((int64_t)g.nxt * g.nyt), BLK, 0, s>>>(g, S, P, C);
+}
+
 void sweep_mixed(int dir, int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
-                 const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,
-                 hipStream_t s) {
+                 const Cells32& C, const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F,
+                 unsigned* fix, hipStream_t s) {
   if (dir == 0) {
-    if (r64) launch_dir<0, true>(lim, sol, g, S, P, e, d, F, fix, s);
-    else launch_dir<0, false>(lim, sol, g, S, P, e, d, F, fix, s);
+    if (r64) launch_dir<0, true>(lim, sol, g, S, P, C, e, d, F, fix, s);
+    else launch_dir<0, false>(lim, sol, g, S, P, C, e, d, F, fix, s);
   } else {
-    if (r64) launch_dir<1, true>(lim, sol, g, S, P, e, d, F, fix, s);
-    else launch_dir<1, false>(lim, sol, g, S, P, e, d, F, fix, s);
+    if (r64) launch_dir<1, true>(lim, sol, g, S, P, C, e, d, F, fix, s);
+    else launch_dir<1, false>(lim, sol, g, S, P, C, e, d, F, fix, s);
   }
 }
 
