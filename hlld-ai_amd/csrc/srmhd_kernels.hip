@@ -1,9 +1,12 @@
 // Kernels and C-ABI entry points of the gfx950 SRMHD step (see include/srmhd.h).
 //
 // Launch shape: one thread per face / cell, 256-thread blocks (4 wave64s).
-// Reductions are wave-level (shuffles, ballots) followed by one atomic per
-// wave on 64-bit words; every reduced quantity is an integer, a bit mask or a
-// max, so results are deterministic.
+// Reductions are wave-level (shuffles, ballots), then block-level, then one
+// atomic per block on 64-bit words (the 2D sweeps' counters into one of 64
+// line-sized shards, folded after the sweep); every reduced quantity is an
+// integer, a bit mask or a max, so results are deterministic.
+#include <mutex>
+
 #include "srmhd_grid2d.h"
 #include "srmhd_mixed.h"
 #include "srmhd_recon.h"
@@ -693,10 +696,12 @@ void sweeps_solver(int sol, bool local, const G2& g, const St2& S, const Pr2& P,
 }
 
 // the mixed-precision sweep along dir and the FP64 fix-up of its flagged faces
+// (the count is on the device: a grid of 4 blocks per CU strides over it)
+constexpr int FIXUP_BLOCKS = 1024;
 template <int N, int LIM>
 void fixup_lim(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const unsigned* fix,
                const Flx& F, hipStream_t s) {
-  k_fixup64<N, LIM><<<64, BLK, 0, s>>>(g, S, P, e, fix, F);
+  k_fixup64<N, LIM><<<FIXUP_BLOCKS, BLK, 0, s>>>(g, S, P, e, fix, F);
 }
 
 template <int N>
@@ -712,6 +717,35 @@ void fixup_dispatch(int lim, const G2& g, const St2& S, const Pr2& P, const srmh
   }
 }
 
+// A second stream per device for work that can run beside the sweeps (the
+// mixed form's FP64 fix-up of the x faces, a few latency-bound waves, beside
+// the y sweep), forked and joined with events.  Created at first use, at the
+// default priority.
+struct AuxStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+constexpr int AUX_MAX_DEVICES = 64;
+AuxStream g_aux[AUX_MAX_DEVICES];
+std::mutex g_aux_mu;   // held while a stage enqueues its fork/join
+
+AuxStream* aux_stream() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= AUX_MAX_DEVICES) return nullptr;
+  AuxStream& a = g_aux[dev];
+  if (!a.s) {
+    hipStream_t st;
+    hipEvent_t f, j;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) return nullptr;
+    a.fork = f;
+    a.join = j;
+    a.s = st;
+  }
+  return &a;
+}
+
 void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
                   const srmhd_eos& e, const Ws2& w, hipStream_t s) {
   // the C2PRec areas are unused in this form: they hold the fix-up lists
@@ -725,11 +759,23 @@ void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const P
   tbegin(SRMHD_K_SWEEP_X_B, s);
   sweep_mixed(0, lim, sol, r64, g, S, P, w.cf, e, shx, w.Fx, fx, s);
   tend(SRMHD_K_SWEEP_X_B, s);
-  if (sol == SOLVER_HLLD) fixup_dispatch<0>(lim, g, S, P, e, fx, w.Fx, s);
+  // the x fix-up reads the primitives and the state and writes only Fx: it
+  // runs beside the y sweep, joined before anything reads Fx
+  std::lock_guard<std::mutex> lock(g_aux_mu);
+  AuxStream* ax = sol == SOLVER_HLLD ? aux_stream() : nullptr;
+  const bool forked = ax && hipEventRecord(ax->fork, s) == hipSuccess &&
+                      hipStreamWaitEvent(ax->s, ax->fork, 0) == hipSuccess;
+  if (sol == SOLVER_HLLD) fixup_dispatch<0>(lim, g, S, P, e, fx, w.Fx, forked ? ax->s : s);
+  const bool joinable = forked && hipEventRecord(ax->join, ax->s) == hipSuccess;
   tbegin(SRMHD_K_SWEEP_Y_B, s);
   sweep_mixed(1, lim, sol, r64, g, S, P, w.cf, e, shy, w.Fy, fy, s);
   tend(SRMHD_K_SWEEP_Y_B, s);
   if (sol == SOLVER_HLLD) fixup_dispatch<1>(lim, g, S, P, e, fy, w.Fy, s);
+  if (joinable) {
+    (void)hipStreamWaitEvent(s, ax->join, 0);
+  } else if (forked) {
+    (void)hipStreamSynchronize(ax->s);   // no join event: wait on the host
+  }
   k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
 }
 
