@@ -636,8 +636,8 @@ G2 make_g2a(const srmhd_grid2d* g) {
 
 bool grid_ok_a(const srmhd_grid2d* g, const srmhd_scheme2d* s) {
   if (!g || !s || g->nx < 1 || g->ny < 1 || g->ng < 1) return false;
-  if (s->precision != SRMHD_PREC_FP64 || s->solver < 0 || s->solver > 2 || s->limiter < 0 ||
-      s->limiter > 5)
+  if (s->precision != SRMHD_PREC_FP64 || s->emf_mode != SRMHD_EMF_SOLVER || s->solver < 0 ||
+      s->solver > 2 || s->limiter < 0 || s->limiter > 5)
     return false;
   const int need = s->limiter == SRMHD_PCM ? 0
                    : (s->limiter <= SRMHD_MINMOD ? 1 : (s->limiter == SRMHD_MP7 ? 3 : 2));

@@ -188,8 +188,8 @@ __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e
 template <int N, int LIM, int SOL, bool LOCAL = false>
 __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e,
                                                  const C2PRec* rec, const srmhd_flux_diag* d,
-                                                 srmhd_flux_diag* dsh, Flx F,
-                                                 double* pstar_out) {
+                                                 srmhd_flux_diag* dsh, Flx F, double* pstar_out,
+                                                 int emf_hll) {
   static_assert(!LOCAL || SOL == SOLVER_HLLD, "the local con2prim concerns HLLD only");
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = t < n_faces<N>(g);
@@ -222,7 +222,8 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
     F.p[3][t] = f[CS + 2];
     F.p[4][t] = f[CT];
     F.p[5][t] = f[CB + 2];
-    F.p[6][t] = N == 0 ? f[CB + 1] : f[CB + 0];   // F^x(By) / F^y(Bx): the face EMF
+    // F^x(By) / F^y(Bx): the face EMF
+    F.p[6][t] = emf_hll ? hll_emf_flux<N>(l, r, e) : (N == 0 ? f[CB + 1] : f[CB + 0]);
     fD = f[CD];
     if (pstar_out) pstar_out[t] = ps;
   }
@@ -258,7 +259,7 @@ __global__ void k_fold_diag(const srmhd_flux_diag* dsh, srmhd_flux_diag* d) {
 // with the HLL-state con2prim of the face as its own batch.
 template <int N, int LIM>
 __global__ __launch_bounds__(BLK) void k_fixup64(G2 g, St2 S, Pr2 P, srmhd_eos e,
-                                                 const unsigned* fix, Flx F) {
+                                                 const unsigned* fix, Flx F, int emf_hll) {
   const unsigned cnt = fix[0];
   for (unsigned q = blockIdx.x * BLK + threadIdx.x; q < cnt; q += gridDim.x * BLK) {
     const int64_t t = fix[2 + q];
@@ -281,7 +282,7 @@ __global__ __launch_bounds__(BLK) void k_fixup64(G2 g, St2 S, Pr2 P, srmhd_eos e
     F.p[3][t] = f[CS + 2];
     F.p[4][t] = f[CT];
     F.p[5][t] = f[CB + 2];
-    F.p[6][t] = N == 0 ? f[CB + 1] : f[CB + 0];
+    F.p[6][t] = emf_hll ? hll_emf_flux<N>(l, r, e) : (N == 0 ? f[CB + 1] : f[CB + 0]);
   }
 }
 
@@ -606,6 +607,7 @@ struct Ws2 {
   unsigned long long* words;    // [0] cell c2p mask, [1..2] dt maxima
   Cells32 cf{};                 // FP32 cell variables of the MIXED form (carved for it only)
   srmhd_flux_diag* dsh;         // DIAG_SHARDS copies of diag[0], then of diag[1]
+  int emf_hll = 0;              // the scheme's emf_mode is SRMHD_EMF_HLL
   int64_t bytes;
 };
 
@@ -648,7 +650,8 @@ void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const W
     k_fold_diag<<<1, 64, 0, s>>>(shx, w.diag + 0);   // the batch mask for pass B
   }
   tbegin(SRMHD_K_SWEEP_X_B, s);
-  k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, shx, w.Fx, w.psx);
+  k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, shx, w.Fx, w.psx,
+                                             w.emf_hll);
   tend(SRMHD_K_SWEEP_X_B, s);
   if (SOL == SOLVER_HLLD) {
     tbegin(SRMHD_K_SWEEP_Y_A, s);
@@ -657,7 +660,8 @@ void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const W
     k_fold_diag<<<1, 64, 0, s>>>(shy, w.diag + 1);
   }
   tbegin(SRMHD_K_SWEEP_Y_B, s);
-  k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, shy, w.Fy, w.psy);
+  k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, shy, w.Fy, w.psy,
+                                             w.emf_hll);
   tend(SRMHD_K_SWEEP_Y_B, s);
   k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
 }
@@ -672,11 +676,11 @@ void sweeps_local(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, c
   srmhd_flux_diag* shy = (srmhd_flux_diag*)((char*)w.dsh + DIAG_SHARDS * DIAG_SHARD_BYTES);
   tbegin(SRMHD_K_SWEEP_X_B, s);
   k_sweep_b<0, LIM, SOLVER_HLLD, true><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, shx,
-                                                          w.Fx, w.psx);
+                                                          w.Fx, w.psx, w.emf_hll);
   tend(SRMHD_K_SWEEP_X_B, s);
   tbegin(SRMHD_K_SWEEP_Y_B, s);
   k_sweep_b<1, LIM, SOLVER_HLLD, true><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, shy,
-                                                          w.Fy, w.psy);
+                                                          w.Fy, w.psy, w.emf_hll);
   tend(SRMHD_K_SWEEP_Y_B, s);
   k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
 }
@@ -700,20 +704,20 @@ void sweeps_solver(int sol, bool local, const G2& g, const St2& S, const Pr2& P,
 constexpr int FIXUP_BLOCKS = 1024;
 template <int N, int LIM>
 void fixup_lim(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const unsigned* fix,
-               const Flx& F, hipStream_t s) {
-  k_fixup64<N, LIM><<<FIXUP_BLOCKS, BLK, 0, s>>>(g, S, P, e, fix, F);
+               const Flx& F, int emf_hll, hipStream_t s) {
+  k_fixup64<N, LIM><<<FIXUP_BLOCKS, BLK, 0, s>>>(g, S, P, e, fix, F, emf_hll);
 }
 
 template <int N>
 void fixup_dispatch(int lim, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
-                    const unsigned* fix, const Flx& F, hipStream_t s) {
+                    const unsigned* fix, const Flx& F, int emf_hll, hipStream_t s) {
   switch (lim) {
-    case LIM_PCM: fixup_lim<N, LIM_PCM>(g, S, P, e, fix, F, s); break;
-    case LIM_MC: fixup_lim<N, LIM_MC>(g, S, P, e, fix, F, s); break;
-    case LIM_MINMOD: fixup_lim<N, LIM_MINMOD>(g, S, P, e, fix, F, s); break;
-    case LIM_WENO5Z: fixup_lim<N, LIM_WENO5Z>(g, S, P, e, fix, F, s); break;
-    case LIM_MP5: fixup_lim<N, LIM_MP5>(g, S, P, e, fix, F, s); break;
-    default: fixup_lim<N, LIM_MP7>(g, S, P, e, fix, F, s);
+    case LIM_PCM: fixup_lim<N, LIM_PCM>(g, S, P, e, fix, F, emf_hll, s); break;
+    case LIM_MC: fixup_lim<N, LIM_MC>(g, S, P, e, fix, F, emf_hll, s); break;
+    case LIM_MINMOD: fixup_lim<N, LIM_MINMOD>(g, S, P, e, fix, F, emf_hll, s); break;
+    case LIM_WENO5Z: fixup_lim<N, LIM_WENO5Z>(g, S, P, e, fix, F, emf_hll, s); break;
+    case LIM_MP5: fixup_lim<N, LIM_MP5>(g, S, P, e, fix, F, emf_hll, s); break;
+    default: fixup_lim<N, LIM_MP7>(g, S, P, e, fix, F, emf_hll, s);
   }
 }
 
@@ -757,7 +761,7 @@ void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const P
   srmhd_flux_diag* shx = w.dsh;
   srmhd_flux_diag* shy = (srmhd_flux_diag*)((char*)w.dsh + DIAG_SHARDS * DIAG_SHARD_BYTES);
   tbegin(SRMHD_K_SWEEP_X_B, s);
-  sweep_mixed(0, lim, sol, r64, g, S, P, w.cf, e, shx, w.Fx, fx, s);
+  sweep_mixed(0, lim, sol, r64, g, S, P, w.cf, e, shx, w.Fx, fx, w.emf_hll, s);
   tend(SRMHD_K_SWEEP_X_B, s);
   // the x fix-up reads the primitives and the state and writes only Fx: it
   // runs beside the y sweep, joined before anything reads Fx
@@ -765,12 +769,13 @@ void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const P
   AuxStream* ax = sol == SOLVER_HLLD ? aux_stream() : nullptr;
   const bool forked = ax && hipEventRecord(ax->fork, s) == hipSuccess &&
                       hipStreamWaitEvent(ax->s, ax->fork, 0) == hipSuccess;
-  if (sol == SOLVER_HLLD) fixup_dispatch<0>(lim, g, S, P, e, fx, w.Fx, forked ? ax->s : s);
+  if (sol == SOLVER_HLLD)
+    fixup_dispatch<0>(lim, g, S, P, e, fx, w.Fx, w.emf_hll, forked ? ax->s : s);
   const bool joinable = forked && hipEventRecord(ax->join, ax->s) == hipSuccess;
   tbegin(SRMHD_K_SWEEP_Y_B, s);
-  sweep_mixed(1, lim, sol, r64, g, S, P, w.cf, e, shy, w.Fy, fy, s);
+  sweep_mixed(1, lim, sol, r64, g, S, P, w.cf, e, shy, w.Fy, fy, w.emf_hll, s);
   tend(SRMHD_K_SWEEP_Y_B, s);
-  if (sol == SOLVER_HLLD) fixup_dispatch<1>(lim, g, S, P, e, fy, w.Fy, s);
+  if (sol == SOLVER_HLLD) fixup_dispatch<1>(lim, g, S, P, e, fy, w.Fy, w.emf_hll, s);
   if (joinable) {
     (void)hipStreamWaitEvent(s, ax->join, 0);
   } else if (forked) {
@@ -1127,7 +1132,8 @@ int64_t srmhd2d_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s) 
 static bool scheme_ok(const srmhd_scheme2d* s) {
   return s && s->solver >= 0 && s->solver <= 2 && s->limiter >= 0 && s->limiter <= 5 &&
          (s->precision == SRMHD_PREC_FP64 || s->precision == SRMHD_PREC_MIXED ||
-          s->precision == SRMHD_PREC_MIXED_R64 || s->precision == SRMHD_PREC_FP64_LOCAL);
+          s->precision == SRMHD_PREC_MIXED_R64 || s->precision == SRMHD_PREC_FP64_LOCAL) &&
+         (s->emf_mode == SRMHD_EMF_SOLVER || s->emf_mode == SRMHD_EMF_HLL);
 }
 
 // the deadband thresholds of the upwind EMF, as k_corner forms them
@@ -1151,6 +1157,7 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   const G2 g = make_g2(gp);
   Ws2 w = carve2d(g, work, sp->precision == SRMHD_PREC_MIXED);
   if (work_bytes < w.bytes) return SRMHD_EARG;
+  w.emf_hll = sp->emf_mode == SRMHD_EMF_HLL;
   if (tape) {
     w.psx = tape->pstar_x;
     w.psy = tape->pstar_y;

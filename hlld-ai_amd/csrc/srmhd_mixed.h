@@ -24,9 +24,11 @@ void cells_f32(const G2& g, const St2& S, const Pr2& P, const Cells32& C, hipStr
 // and the faces to recompute in FP64 appended to fix (fix[0] = count, zeroed
 // by the caller; indices from fix[2]; room for every face).  r64: the
 // reconstruction in FP64 from P (else in FP32 from C, filled by cells_f32).
+// emf_hll: the EMF slot of F holds the HLL average of the tangential-field
+// flux (emf_mode "hll") instead of the solver's.
 void sweep_mixed(int dir, int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
                  const Cells32& C, const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F,
-                 unsigned* fix, hipStream_t s);
+                 unsigned* fix, int emf_hll, hipStream_t s);
 
 // con2prim of the physical cells of S into P for the mixed forms: each cell
 // its own batch, mu_+ bracketed from FP32 (no primitive BCs; the caller

@@ -183,9 +183,20 @@ __device__ int hlld_mixed(const f32::Prim& l, const f32::Prim& r, const f32::Eos
 
 // fix: count + indices of the faces to recompute in FP64 (fix[0] = count,
 // indices from fix[2]); capacity >= the number of faces.
+// The face EMF of emf_mode "hll" in FP32: the HLL average of the
+// tangential-field flux from the FP32 side states (hll_emf_flux's FP32 form).
+template <int N>
+__device__ float hll_emf_flux32(const f32::Prim& L, const f32::Prim& R, const f32::Eos& e) {
+  float uL[8], uR[8], fL[8], fR[8], cmax, cmin;
+  f32::srmhd_fluxes<N>(L, R, e, uL, uR, fL, fR, cmax, cmin);
+  constexpr int c = N == 0 ? CB + 1 : CB + 0;
+  return hll_flux(fL[c], fR[c], uL[c], uR[c], cmin, cmax);
+}
+
 template <int N, int LIM, int SOL, bool R64>
 __global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, Cells32 C, srmhd_eos e64,
-                                                   srmhd_flux_diag* d, Flx F, unsigned* fix) {
+                                                   srmhd_flux_diag* d, Flx F, unsigned* fix,
+                                                   int emf_hll) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = t < n_faces<N>(g);
   int st = 0;
@@ -226,7 +237,9 @@ __global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, Cells32 C
     F.p[3][t] = f[CS + 2];
     F.p[4][t] = f[CT];
     F.p[5][t] = f[CB + 2];
-    F.p[6][t] = N == 0 ? f[CB + 1] : f[CB + 0];
+    double emf = N == 0 ? f[CB + 1] : f[CB + 0];
+    if (emf_hll) emf = R64 ? hll_emf_flux<N>(l64, r64, e64) : (double)hll_emf_flux32<N>(l, r, e);
+    F.p[6][t] = emf;
     fD = f[CD];
   }
   diag_accumulate(diag_shard(d), active, st, 0, dbits(fabs(fD)));
@@ -235,30 +248,30 @@ __global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, Cells32 C
 template <int N, int LIM, bool R64>
 void launch_lim(int sol, const G2& g, const St2& S, const Pr2& P, const Cells32& C,
                 const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,
-                hipStream_t s) {
+                int emf_hll, hipStream_t s) {
   const int nb = blocks_for(n_faces_host(g, N));
   switch (sol) {
     case SOLVER_HLLE:
-      k_sweep_f32<N, LIM, SOLVER_HLLE, R64><<<nb, BLK, 0, s>>>(g, S, P, C, e, d, F, fix);
+      k_sweep_f32<N, LIM, SOLVER_HLLE, R64><<<nb, BLK, 0, s>>>(g, S, P, C, e, d, F, fix, emf_hll);
       break;
     case SOLVER_HLLC:
-      k_sweep_f32<N, LIM, SOLVER_HLLC, R64><<<nb, BLK, 0, s>>>(g, S, P, C, e, d, F, fix);
+      k_sweep_f32<N, LIM, SOLVER_HLLC, R64><<<nb, BLK, 0, s>>>(g, S, P, C, e, d, F, fix, emf_hll);
       break;
-    default: k_sweep_f32<N, LIM, SOLVER_HLLD, R64><<<nb, BLK, 0, s>>>(g, S, P, C, e, d, F, fix);
+    default: k_sweep_f32<N, LIM, SOLVER_HLLD, R64><<<nb, BLK, 0, s>>>(g, S, P, C, e, d, F, fix, emf_hll);
   }
 }
 
 template <int N, bool R64>
 void launch_dir(int lim, int sol, const G2& g, const St2& S, const Pr2& P, const Cells32& C,
                 const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F, unsigned* fix,
-                hipStream_t s) {
+                int emf_hll, hipStream_t s) {
   switch (lim) {
-    case LIM_PCM: launch_lim<N, LIM_PCM, R64>(sol, g, S, P, C, e, d, F, fix, s); break;
-    case LIM_MC: launch_lim<N, LIM_MC, R64>(sol, g, S, P, C, e, d, F, fix, s); break;
-    case LIM_MINMOD: launch_lim<N, LIM_MINMOD, R64>(sol, g, S, P, C, e, d, F, fix, s); break;
-    case LIM_WENO5Z: launch_lim<N, LIM_WENO5Z, R64>(sol, g, S, P, C, e, d, F, fix, s); break;
-    case LIM_MP5: launch_lim<N, LIM_MP5, R64>(sol, g, S, P, C, e, d, F, fix, s); break;
-    default: launch_lim<N, LIM_MP7, R64>(sol, g, S, P, C, e, d, F, fix, s);
+    case LIM_PCM: launch_lim<N, LIM_PCM, R64>(sol, g, S, P, C, e, d, F, fix, emf_hll, s); break;
+    case LIM_MC: launch_lim<N, LIM_MC, R64>(sol, g, S, P, C, e, d, F, fix, emf_hll, s); break;
+    case LIM_MINMOD: launch_lim<N, LIM_MINMOD, R64>(sol, g, S, P, C, e, d, F, fix, emf_hll, s); break;
+    case LIM_WENO5Z: launch_lim<N, LIM_WENO5Z, R64>(sol, g, S, P, C, e, d, F, fix, emf_hll, s); break;
+    case LIM_MP5: launch_lim<N, LIM_MP5, R64>(sol, g, S, P, C, e, d, F, fix, emf_hll, s); break;
+    default: launch_lim<N, LIM_MP7, R64>(sol, g, S, P, C, e, d, F, fix, emf_hll, s);
   }
 }
 
@@ -314,13 +327,13 @@ void cells_f32(const G2& g, const St2& S, const Pr2& P, const Cells32& C, hipStr
 
 void sweep_mixed(int dir, int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
                  const Cells32& C, const srmhd_eos& e, srmhd_flux_diag* d, const Flx& F,
-                 unsigned* fix, hipStream_t s) {
+                 unsigned* fix, int emf_hll, hipStream_t s) {
   if (dir == 0) {
-    if (r64) launch_dir<0, true>(lim, sol, g, S, P, C, e, d, F, fix, s);
-    else launch_dir<0, false>(lim, sol, g, S, P, C, e, d, F, fix, s);
+    if (r64) launch_dir<0, true>(lim, sol, g, S, P, C, e, d, F, fix, emf_hll, s);
+    else launch_dir<0, false>(lim, sol, g, S, P, C, e, d, F, fix, emf_hll, s);
   } else {
-    if (r64) launch_dir<1, true>(lim, sol, g, S, P, C, e, d, F, fix, s);
-    else launch_dir<1, false>(lim, sol, g, S, P, C, e, d, F, fix, s);
+    if (r64) launch_dir<1, true>(lim, sol, g, S, P, C, e, d, F, fix, emf_hll, s);
+    else launch_dir<1, false>(lim, sol, g, S, P, C, e, d, F, fix, emf_hll, s);
   }
 }
 

@@ -250,6 +250,19 @@ struct HlldPrep {
   }
 };
 
+// The face EMF flux of emf_mode "hll" (ct.py:63-88): the HLL average of the
+// tangential-field flux the sweep keeps for the corner EMF (F^x(B_y) for
+// N = 0, F^y(B_x) for N = 1) from the face's side states and HLL speeds
+// (compute_srmhd_fluxes, sweep.py:106-118), whatever the solver.
+template <int N, class T>
+SRMHD_DEV T hll_emf_flux(const PrimT<T>& L, const PrimT<T>& R, const srmhd_eos& e) {
+  static_assert(N == 0 || N == 1, "the 2D sweeps");
+  T uL[8], uR[8], fL[8], fR[8], cmax, cmin;
+  srmhd_fluxes<N>(L, R, e, uL, uR, fL, fR, cmax, cmin);
+  constexpr int c = N == 0 ? CB + 1 : CB + 0;
+  return hll_flux(fL[c], fR[c], uL[c], uR[c], cmin, cmax);
+}
+
 // p_hll from the con2prim of the HLL state   hlld.py:1255-1263
 template <class T>
 SRMHD_DEV T p_hll_from_prim(const PrimT<T>& P, const srmhd_eos& e) {

@@ -17,6 +17,7 @@ SOLVER_IDS = {"hlle": 0, "hllc": 1, "hlld": 2}
 LIMITER_IDS = {"pcm": 0, "mc": 1, "minmod": 2, "weno5z": 3, "mp5": 4, "mp7": 5}
 BC_IDS = {"outflow": 0, "periodic": 1, "constant": 2}
 PRECISION_IDS = {"fp64": 0, "mixed": 1, "mixed_r64": 2, "fp64_local": 3}
+EMF_MODE_IDS = {"solver": 0, "hll": 1}
 
 EARG = 1
 EHIP = 1000
@@ -47,7 +48,8 @@ class Grid1D(ctypes.Structure):
 
 class Scheme2D(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int32), ("limiter", ctypes.c_int32), ("upwind", ctypes.c_int32),
-                ("precision", ctypes.c_int32), ("atmo_rho", ctypes.c_double),
+                ("precision", ctypes.c_int32), ("emf_mode", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("atmo_rho", ctypes.c_double),
                 ("atmo_eps", ctypes.c_double)]
 
 

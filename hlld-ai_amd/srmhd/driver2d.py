@@ -35,7 +35,7 @@ class Stepper2D:
 
     def __init__(self, grid, eos, *, solver="hlld", limiter="mc", bc_x="outflow",
                  bc_y="outflow", upwind=True, atmo_rho=1e-10, atmo_eps=1e-10, device=None,
-                 precision="fp64"):
+                 precision="fp64", emf_mode="solver"):
         self.grid = grid
         self.device = torch.device(device) if device is not None else grid.device
         if self.device.type != "cuda":
@@ -52,9 +52,13 @@ class Stepper2D:
         if precision not in _lib.PRECISION_IDS:
             raise ValueError(f"precision must be one of {tuple(_lib.PRECISION_IDS)}")
         self.precision = precision
+        if emf_mode not in _lib.EMF_MODE_IDS:
+            raise ValueError(f"unknown emf mode {emf_mode!r}")
+        self.emf_mode = emf_mode
         self.cs = _lib.Scheme2D(_lib.SOLVER_IDS[self.solver], _lib.LIMITER_IDS[limiter],
                                 1 if upwind else 0, _lib.PRECISION_IDS[precision],
-                                float(atmo_rho), float(atmo_eps))
+                                _lib.EMF_MODE_IDS[emf_mode], 0, float(atmo_rho),
+                                float(atmo_eps))
         nb = self.lib.srmhd2d_workspace_bytes(ctypes.byref(self.cg), ctypes.byref(self.cs))
         if nb < 0:
             raise ValueError("grid / scheme rejected by srmhd2d_workspace_bytes")
@@ -151,16 +155,17 @@ _STEPPERS = {}
 
 
 def _stepper(grid, eos, solver, limiter, bc_x, bc_y, upwind, atmo_rho, device,
-             precision="fp64"):
+             precision="fp64", emf_mode="solver"):
     key = (id(grid), grid.nx, grid.ny, grid.ng, float(grid.dx), float(grid.dy),
            float(eos.K), float(eos.gamma), getattr(eos, "gamma_th", None), solver, limiter,
-           bc_x, bc_y, bool(upwind), float(atmo_rho), str(device), precision)
+           bc_x, bc_y, bool(upwind), float(atmo_rho), str(device), precision, emf_mode)
     st = _STEPPERS.get(key)
     if st is None:
         if len(_STEPPERS) > 8:
             _STEPPERS.clear()
         st = Stepper2D(grid, eos, solver=solver, limiter=limiter, bc_x=bc_x, bc_y=bc_y,
-                       upwind=upwind, atmo_rho=atmo_rho, device=device, precision=precision)
+                       upwind=upwind, atmo_rho=atmo_rho, device=device, precision=precision,
+                       emf_mode=emf_mode)
         _STEPPERS[key] = st
     return st
 
@@ -184,8 +189,8 @@ def _finish_prims(state, P6, grid, bc_x, bc_y):
 def _check_options(vel_var, emf_mode, recorder):
     if vel_var != "Wv":
         raise ValueError("the HIP path reconstructs W v (vel_var='Wv')")
-    if emf_mode != "solver":
-        raise ValueError("the HIP path implements emf_mode='solver'")
+    if emf_mode not in ("solver", "hll"):
+        raise ValueError(f"unknown emf mode {emf_mode!r}")
     if recorder is not None:
         raise ValueError("interface recorders are not supported on the HIP path")
 
@@ -235,7 +240,7 @@ def rk_step_ct(state, grid, eos, dt, *, scheme="rk3", bc_x="outflow", bc_y="outf
     solver in FP32 and the rest of the stage in FP64."""
     _check_options(vel_var, emf_mode, recorder)
     st = _stepper(grid, eos, solver_name(flux_fn), limiter, bc_x, bc_y, upwind, atmo_rho,
-                  grid.device, precision)
+                  grid.device, precision, emf_mode)
     st.set_dt(dt)
     U0 = [t.contiguous() for t in state.state_list()]
     U, P = U0, [state.prims[k].contiguous() for k in PRIM6]

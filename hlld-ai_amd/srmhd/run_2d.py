@@ -94,7 +94,7 @@ def _parser():
     ap.add_argument("--tend", type=float, default=None)
     ap.add_argument("--cfl", type=float, default=0.25)
     ap.add_argument("--solver", default="hlld", choices=SOLVERS)
-    ap.add_argument("--emf-mode", default="solver", choices=["solver"])
+    ap.add_argument("--emf-mode", default="solver", choices=["solver", "hll"])
     ap.add_argument("--limiter", default="mc")
     ap.add_argument("--no-upwind-emf", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=tuple(_lib.PRECISION_IDS),
@@ -166,7 +166,8 @@ def main(argv=None):
     _write_meta(out, meta)
 
     S = D2.Stepper2D(g, eos, solver=a.solver, limiter=a.limiter, bc_x=bc_x, bc_y=bc_y,
-                     upwind=not a.no_upwind_emf, precision=a.precision)
+                     upwind=not a.no_upwind_emf, precision=a.precision,
+                     emf_mode=a.emf_mode)
     S.load(st.state_list(), st.prim_list())
 
     def current():

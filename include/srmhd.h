@@ -122,8 +122,17 @@ enum {
   SRMHD_PREC_FP64_LOCAL = 3
 };
 
+/* emf_mode: the face EMF fed to the corner EMF (ct.py:63-88, selected by
+ * run_2d.py --emf-mode): SRMHD_EMF_SOLVER, the solver's own flux of the
+ * tangential field (-F^x(B_y), F^y(B_x)); SRMHD_EMF_HLL, the HLL average of
+ * that flux from the side states and HLL speeds of the face
+ * (compute_srmhd_fluxes, sweep.py:106-118), whatever the solver.  The
+ * adjoint entry points take SRMHD_EMF_SOLVER only. */
+enum { SRMHD_EMF_SOLVER = 0, SRMHD_EMF_HLL = 1 };
+
 typedef struct srmhd_scheme2d {
   int32_t solver, limiter, upwind, precision;
+  int32_t emf_mode, reserved;   /* reserved: 0 */
   double atmo_rho, atmo_eps;
 } srmhd_scheme2d;
 

@@ -8,7 +8,7 @@ import torch
 
 from .c2p import cons_to_prim
 from .grid import Grid2D
-from .physics import prim_to_cons, sqrt
+from .physics import hll_flux, prim_to_cons, sqrt, srmhd_fluxes
 from .recon import reconstruct_prims
 from .riemann import LAST_DIAG, hlld_flux
 
@@ -52,9 +52,11 @@ def combine(states, weights):
     return State2D(cons, Bxf, Byf)
 
 
-def sweep(prims, eos, axis, flux_fn=hlld_flux, limiter="mc", Bnf=None):
+def sweep(prims, eos, axis, flux_fn=hlld_flux, limiter="mc", Bnf=None, want_hll_aux=False):
     """reconstruct -> B_n := face value -> flux_fn on flattened faces ->
-    zero the normal-B flux   sweep.py:30-119.  Returns (F, p_star, diag)."""
+    zero the normal-B flux   sweep.py:30-119.  Returns (F, p_star, diag), and
+    with want_hll_aux the side states and HLL speeds of compute_srmhd_fluxes
+    (sweep.py:106-118) as a fourth item."""
     L, R = reconstruct_prims(prims, axis, eos, limiter)
     shape = L["rho"].shape
     if Bnf is not None:
@@ -65,7 +67,34 @@ def sweep(prims, eos, axis, flux_fn=hlld_flux, limiter="mc", Bnf=None):
     F, _, ps = flux_fn(flat(L), flat(R), eos, idir=axis)
     F = unflat(F, shape)
     F[_BKEY[axis]] = torch.zeros_like(F[_BKEY[axis]])
-    return F, ps.reshape(shape), dict(LAST_DIAG)
+    if not want_hll_aux:
+        return F, ps.reshape(shape), dict(LAST_DIAG)
+    diag = dict(LAST_DIAG)
+    uL, uR, fL, fR, cmax, cmin = srmhd_fluxes(flat(L), flat(R), eos, axis)
+    hll = {"uL": unflat(uL, shape), "uR": unflat(uR, shape), "fL": unflat(fL, shape),
+           "fR": unflat(fR, shape), "cmax": cmax.reshape(shape), "cmin": cmin.reshape(shape)}
+    return F, ps.reshape(shape), diag, hll
+
+
+def face_emf_x(Fx, hll=None, mode="solver"):
+    """E^z on x-faces   ct.py:63-74: -F^x(B_y), or with mode "hll" minus the
+    HLL average of B_y's flux from the side states."""
+    if mode == "solver":
+        return -Fx["By"]
+    if mode == "hll":
+        return -hll_flux(hll["fL"]["By"], hll["fR"]["By"], hll["uL"]["By"], hll["uR"]["By"],
+                         hll["cmin"], hll["cmax"])
+    raise ValueError(f"unknown emf mode {mode!r}")
+
+
+def face_emf_y(Fy, hll=None, mode="solver"):
+    """E^z on y-faces   ct.py:77-88: F^y(B_x), or its HLL average."""
+    if mode == "solver":
+        return Fy["Bx"]
+    if mode == "hll":
+        return hll_flux(hll["fL"]["Bx"], hll["fR"]["Bx"], hll["uL"]["Bx"], hll["uR"]["Bx"],
+                        hll["cmin"], hll["cmax"])
+    raise ValueError(f"unknown emf mode {mode!r}")
 
 
 def flux_divergence(F, axis, delta, keys):
@@ -161,16 +190,22 @@ def cons_to_prims_2d(cons, grid, eos, atmo_rho=1e-10, return_status=False):
     return unflat(out, grid.shape)
 
 
-def ct_stage(state, grid, eos, flux_fn=hlld_flux, limiter="mc", upwind=True):
-    """driver2d.py:162-193 with emf_mode = "solver".  Returns
-    (rhs, dBxf, dByf, aux) where aux carries the face fluxes and diag."""
+def ct_stage(state, grid, eos, flux_fn=hlld_flux, limiter="mc", upwind=True, emf_mode="solver"):
+    """driver2d.py:162-193.  Returns (rhs, dBxf, dByf, aux) where aux carries
+    the face fluxes and diag."""
+    if emf_mode not in ("solver", "hll"):
+        raise ValueError(f"unknown emf mode {emf_mode!r}")
     Bcx, Bcy = cell_centred_B(state.Bxf, state.Byf)
     prims = dict(state.prims)
     prims["Bx"], prims["By"] = Bcx, Bcy
     Ec = Bcx * prims["vy"] - Bcy * prims["vx"]
-    Fx, psx, dgx = sweep(prims, eos, 0, flux_fn, limiter, Bnf=state.Bxf)
-    Fy, psy, dgy = sweep(prims, eos, 1, flux_fn, limiter, Bnf=state.Byf)
-    Ez = corner_emf(Ec, -Fx["By"], Fy["Bx"], Fx["D"], Fy["D"], upwind=upwind)
+    want = emf_mode == "hll"
+    ox = sweep(prims, eos, 0, flux_fn, limiter, Bnf=state.Bxf, want_hll_aux=want)
+    oy = sweep(prims, eos, 1, flux_fn, limiter, Bnf=state.Byf, want_hll_aux=want)
+    (Fx, psx, dgx), (Fy, psy, dgy) = ox[:3], oy[:3]
+    Efx = face_emf_x(Fx, ox[3] if want else None, emf_mode)
+    Efy = face_emf_y(Fy, oy[3] if want else None, emf_mode)
+    Ez = corner_emf(Ec, Efx, Efy, Fx["D"], Fy["D"], upwind=upwind)
     rx = flux_divergence(Fx, 0, grid.dx, EVOLVED_KEYS)
     ry = flux_divergence(Fy, 1, grid.dy, EVOLVED_KEYS)
     rhs = {k: rx[k] + ry[k] for k in EVOLVED_KEYS}
@@ -199,14 +234,14 @@ def sync_state(state, grid, eos, bc_x="outflow", bc_y="outflow", atmo_rho=1e-10)
 
 
 def rk_step_ct(state, grid, eos, dt, scheme="rk3", bc_x="outflow", bc_y="outflow",
-               flux_fn=hlld_flux, limiter="mc", upwind=True, atmo_rho=1e-10):
+               flux_fn=hlld_flux, limiter="mc", upwind=True, atmo_rho=1e-10, emf_mode="solver"):
     """driver2d.py:221-249."""
     S0 = State2D({k: state.cons[k].clone() for k in EVOLVED_KEYS}, state.Bxf.clone(),
                  state.Byf.clone(), state.prims)
     S = state
     aux = {}
     for a, b, c in RK[scheme]:
-        rhs, dBx, dBy, aux = ct_stage(S, grid, eos, flux_fn, limiter, upwind)
+        rhs, dBx, dBy, aux = ct_stage(S, grid, eos, flux_fn, limiter, upwind, emf_mode)
         S = combine([S0, S, State2D(rhs, dBx, dBy)], [a, b, c * dt])
         S = sync_state(S, grid, eos, bc_x, bc_y, atmo_rho)
     return S, aux

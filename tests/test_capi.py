@@ -43,7 +43,7 @@ def test_workspace_queries_without_gpu():
     assert lib.srmhd_flux_workspace_bytes(0, 1000) > 0
     assert lib.srmhd_c2p_workspace_bytes(-1) == -1
     g = _lib.Grid2D(64, 64, 3, 0, 0, 0, 1 / 64, 1 / 64)
-    s = _lib.Scheme2D(2, 3, 1, 0, 1e-10, 1e-10)
+    s = _lib.Scheme2D(2, 3, 1, 0, 0, 0, 1e-10, 1e-10)
     nb = lib.srmhd2d_workspace_bytes(ctypes.byref(g), ctypes.byref(s))
     assert nb > 70 * 70 * 8 * 14
     bad = _lib.Grid2D(64, 64, 3, 7, 0, 0, 1 / 64, 1 / 64)
@@ -68,9 +68,11 @@ def test_adjoint_workspace_query_without_gpu():
     g = _lib.Grid2D(64, 64, 4, 1, 1, 0, 0.1, 0.1)   # ng 4: room for MP7
     sizes = []
     for lim in range(6):
-        s = _lib.Scheme2D(2, lim, 1, 0, 1e-10, 1e-10)
+        s = _lib.Scheme2D(2, lim, 1, 0, 0, 0, 1e-10, 1e-10)
         sizes.append(lib.srmhd2d_adjoint_workspace_bytes(ctypes.byref(g), ctypes.byref(s)))
     assert all(b > 0 for b in sizes)
     assert sizes[0] < sizes[1] < sizes[3] < sizes[5]
-    mixed = _lib.Scheme2D(2, 1, 1, 1, 1e-10, 1e-10)
+    mixed = _lib.Scheme2D(2, 1, 1, 1, 0, 0, 1e-10, 1e-10)
     assert lib.srmhd2d_adjoint_workspace_bytes(ctypes.byref(g), ctypes.byref(mixed)) == -1
+    emf_hll = _lib.Scheme2D(2, 1, 1, 0, 1, 0, 1e-10, 1e-10)   # the adjoint is of the solver EMF
+    assert lib.srmhd2d_adjoint_workspace_bytes(ctypes.byref(g), ctypes.byref(emf_hll)) == -1

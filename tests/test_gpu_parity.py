@@ -149,7 +149,8 @@ CASES_2D = [
 ]
 
 
-def _gpu_run(problem, n, lim, solver, scheme, gamma, box, bc, steps=None, dts=None, cfl=0.4):
+def _gpu_run(problem, n, lim, solver, scheme, gamma, box, bc, steps=None, dts=None, cfl=0.4,
+             upwind=True, emf_mode="solver"):
     from srmhd import driver2d as D2, eos as E, grid as G, initial_data2d as I
     from srmhd.reconstruction import ghosts_needed
     dev = _dev()
@@ -163,12 +164,13 @@ def _gpu_run(problem, n, lim, solver, scheme, gamma, box, bc, steps=None, dts=No
         dt = D2.compute_dt_2d(st.prims, grid, eos, cfl)
         out_dts.append(dt)
         st, _ = D2.rk_step_ct(st, grid, eos, dt, scheme=scheme, bc_x=bc, bc_y=bc,
-                              flux_fn=solver, limiter=lim)
+                              flux_fn=solver, limiter=lim, upwind=upwind, emf_mode=emf_mode)
     torch.cuda.synchronize()
     return init, st, out_dts
 
 
-def _oracle_run(problem, n, lim, solver, scheme, gamma, box, bc, steps, cfl=0.4):
+def _oracle_run(problem, n, lim, solver, scheme, gamma, box, bc, steps, cfl=0.4, upwind=True,
+                emf_mode="solver"):
     import oracle.eos as OE
     import oracle.riemann as OR
     import oracle.step2d as OS
@@ -183,7 +185,8 @@ def _oracle_run(problem, n, lim, solver, scheme, gamma, box, bc, steps, cfl=0.4)
         dt = OS.compute_dt_2d(so.prims, go, eos, cfl)
         dts.append(dt)
         so, _ = OS.rk_step_ct(so, go, eos, dt, scheme=scheme, bc_x=bc, bc_y=bc,
-                              flux_fn=OR.SOLVERS[solver], limiter=lim)
+                              flux_fn=OR.SOLVERS[solver], limiter=lim, upwind=upwind,
+                              emf_mode=emf_mode)
     return so, dts
 
 
@@ -213,6 +216,57 @@ def test_2d_steps_bitwise_vs_oracle_and_close_to_reference(golden_dir, case):
     _close(st.Byf, g["fin_Byf"], "golden Byf")
     for k in PRIM9:
         _close(st.prims[k], g[f"fin_prim_{k}"], f"golden prim {k}")
+
+
+# the production-selectable variants the golden cases leave out: minmod, PCM
+# and MP7 in 2D, the averaged corner EMF (--no-upwind-emf) and
+# emf_mode="hll" (ct.py:63-88) with every solver
+VARIANTS_2D = [
+    ("rotor_tapered_minmod_hlld", "rotor_tapered", 16, "minmod", "hlld", "rk3", True, "solver"),
+    ("rotor_sharp_pcm_hlle", "rotor", 16, "pcm", "hlle", "rk2", True, "solver"),
+    ("rotor_tapered_mp7_hlld", "rotor_tapered", 16, "mp7", "hlld", "rk3", True, "solver"),
+    ("rotor_sharp_mc_hlld_no_upwind", "rotor", 16, "mc", "hlld", "rk3", False, "solver"),
+    ("rotor_sharp_mc_hlld_emf_hll", "rotor", 16, "mc", "hlld", "rk3", True, "hll"),
+    ("rotor_tapered_weno5z_hllc_emf_hll", "rotor_tapered", 16, "weno5z", "hllc", "rk2", True,
+     "hll"),
+    ("rotor_sharp_mp5_hlle_emf_hll_no_upwind", "rotor", 16, "mp5", "hlle", "rk3", False, "hll"),
+]
+
+
+@pytest.mark.parametrize("case", VARIANTS_2D, ids=[c[0] for c in VARIANTS_2D])
+def test_2d_variants_bitwise_vs_oracle(case):
+    name, problem, n, lim, solver, scheme, upwind, emf = case
+    _flush(f"2d variant {name}")
+    box, bc = (-0.5, 0.5), "outflow"
+    init, st, dts = _gpu_run(problem, n, lim, solver, scheme, 5 / 3, box, bc, steps=3,
+                             upwind=upwind, emf_mode=emf)
+    so, dts_o = _oracle_run(problem, n, lim, solver, scheme, 5 / 3, box, bc, 3, upwind=upwind,
+                            emf_mode=emf)
+    assert dts == dts_o
+    _state_eq(st, so, "oracle")
+
+
+def test_periodic_ot_emf_hll_bitwise_vs_oracle():
+    _flush("2d ot2pi mc hlld emf hll")
+    args = ("orszag_tang_2pi", 16, "mc", "hlld", "rk2", 4 / 3, (0.0, 2 * np.pi), "periodic")
+    init, st, dts = _gpu_run(*args, steps=3, emf_mode="hll")
+    so, dts_o = _oracle_run(*args, 3, emf_mode="hll")
+    assert dts == dts_o
+    _state_eq(st, so, "oracle")
+
+
+@pytest.mark.parametrize("emf_mode", ["solver", "hll"])
+def test_divb_roundoff_both_emf_modes(emf_mode):
+    """test_ct.py:262-289 on the device: a 32^2 rotor (MC, HLLD, ng = 2),
+    4 RK3 steps at CFL 0.25, |div B| / (max|B_face|/dx) < 1e-13."""
+    _flush(f"rotor 32^2 divB emf {emf_mode}")
+    init, st, dts = _gpu_run("rotor", 32, "mc", "hlld", "rk3", 5 / 3, (-0.5, 0.5), "outflow",
+                             steps=4, cfl=0.25, emf_mode=emf_mode)
+    from srmhd.run_2d import div_b
+    ng = (st.Bxf.shape[1] - 32) // 2
+    d = div_b(st.Bxf, st.Byf, 1.0 / 32, 1.0 / 32)[ng:ng + 32, ng:ng + 32]
+    bmax = max(float(st.Bxf.abs().max()), float(st.Byf.abs().max()))
+    assert float(d.abs().max()) / (bmax * 32) < 1e-13
 
 
 def test_rotor_64_weno5z_matches_oracle_three_steps():
