@@ -44,8 +44,7 @@ DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "swee
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
 # (form, kernel class, n)
-MEASURED_TRAFFIC = {("mixed", "sweep_x_b", 2048): 553.4e6, ("fp64", "sweep_x_a", 2048): 416.8e6,
-                    ("fp64", "c2p_a", 2048): 376.3e6, ("mixed", "c2p_a", 2048): 490.0e6}
+MEASURED_TRAFFIC = {("fp64", "sweep_x_a", 2048): 416.8e6, ("fp64", "c2p_a", 2048): 376.3e6}
 
 
 def algorithmic_bytes(kernel, n, ng, precision="fp64"):
@@ -55,12 +54,13 @@ def algorithmic_bytes(kernel, n, ng, precision="fp64"):
     nt = n + 2 * ng
     cells = nt * nt
     faces = (nt + 1) * nt
-    if precision == "fp64_local" and kernel in ("sweep_x_b", "sweep_y_b"):
-        # the FP64 sweep without a pass-A record
-        return 8 * (6 * cells + 2 * faces) + 56 * faces
-    if precision != "fp64" and kernel in ("sweep_x_b", "sweep_y_b"):
-        # the one FP32 sweep kernel: prims rho, p, vx, vy, vz + cons Bz +
-        # both face-B arrays in, 7 FP64 flux arrays out
+    if precision == "mixed" and kernel in ("sweep_x_b", "sweep_y_b"):
+        # the FP32 sweep: 7 FP32 cell arrays (rho, p, W v, two tangential B)
+        # and the FP64 normal face field in, 7 FP64 flux arrays out
+        return 4 * 7 * cells + 8 * faces + 56 * faces
+    if precision in ("fp64_local", "mixed_r64") and kernel in ("sweep_x_b", "sweep_y_b"):
+        # the FP64-reconstructing sweep without a pass-A record: prims rho,
+        # p, vx, vy, vz + cons Bz + both face-B arrays in, 7 flux arrays out
         return 8 * (6 * cells + 2 * faces) + 56 * faces
     if kernel in ("sweep_x_b", "sweep_y_b"):
         # prims rho, p, vx, vy, vz + cons Bz + both face-B arrays; the

@@ -200,6 +200,8 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
     face_ij<N>(g, t, i, j);
     Prim l, r;
     face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
+    // the HLL face EMF first, while only the face states are live
+    const double emf_h = emf_hll ? hll_emf_flux<N>(l, r, e) : 0.0;
     double f[8], u[8], ps;
     if (SOL == SOLVER_HLLE) {
       hlle<N>(l, r, e, f, u, ps);
@@ -223,7 +225,7 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
     F.p[4][t] = f[CT];
     F.p[5][t] = f[CB + 2];
     // F^x(By) / F^y(Bx): the face EMF
-    F.p[6][t] = emf_hll ? hll_emf_flux<N>(l, r, e) : (N == 0 ? f[CB + 1] : f[CB + 0]);
+    F.p[6][t] = emf_hll ? emf_h : (N == 0 ? f[CB + 1] : f[CB + 0]);
     fD = f[CD];
     if (pstar_out) pstar_out[t] = ps;
   }
@@ -267,6 +269,7 @@ __global__ __launch_bounds__(BLK) void k_fixup64(G2 g, St2 S, Pr2 P, srmhd_eos e
     face_ij<N>(g, t, i, j);
     Prim l, r;
     face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
+    const double emf_h = emf_hll ? hll_emf_flux<N>(l, r, e) : 0.0;
     HlldPrep<N> h;
     h.run(l, r, e);
     Kastaun k;
@@ -282,7 +285,7 @@ __global__ __launch_bounds__(BLK) void k_fixup64(G2 g, St2 S, Pr2 P, srmhd_eos e
     F.p[3][t] = f[CS + 2];
     F.p[4][t] = f[CT];
     F.p[5][t] = f[CB + 2];
-    F.p[6][t] = emf_hll ? hll_emf_flux<N>(l, r, e) : (N == 0 ? f[CB + 1] : f[CB + 0]);
+    F.p[6][t] = emf_hll ? emf_h : (N == 0 ? f[CB + 1] : f[CB + 0]);
   }
 }
 

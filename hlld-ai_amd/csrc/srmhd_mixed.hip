@@ -193,10 +193,12 @@ __device__ float hll_emf_flux32(const f32::Prim& L, const f32::Prim& R, const f3
   return hll_flux(fL[c], fR[c], uL[c], uR[c], cmin, cmax);
 }
 
+// at least two waves per SIMD: the MIXED_R64 instance otherwise spills two
+// registers into AGPRs and drops to one wave
 template <int N, int LIM, int SOL, bool R64>
-__global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, Cells32 C, srmhd_eos e64,
-                                                   srmhd_flux_diag* d, Flx F, unsigned* fix,
-                                                   int emf_hll) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(2))) void k_sweep_f32(
+    G2 g, St2 S, Pr2 P, Cells32 C, srmhd_eos e64, srmhd_flux_diag* d, Flx F, unsigned* fix,
+    int emf_hll) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = t < n_faces<N>(g);
   int st = 0;
@@ -211,6 +213,9 @@ __global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, Cells32 C
       face_prims_r64<N, LIM>(g, S, P, e64, e, i, j, l, r, l64, r64);
     else
       face_prims_c32<N, LIM>(g, S, C, e, i, j, l, r);
+    // the HLL face EMF first, while only the face states are live
+    double emf_h = 0.0;
+    if (emf_hll) emf_h = R64 ? hll_emf_flux<N>(l64, r64, e64) : (double)hll_emf_flux32<N>(l, r, e);
     double f[8];
     if (SOL == SOLVER_HLLD && R64) {
       st = hlld_mixed<N>(l, r, e, l64, r64, e64, f);
@@ -237,9 +242,7 @@ __global__ __launch_bounds__(BLK) void k_sweep_f32(G2 g, St2 S, Pr2 P, Cells32 C
     F.p[3][t] = f[CS + 2];
     F.p[4][t] = f[CT];
     F.p[5][t] = f[CB + 2];
-    double emf = N == 0 ? f[CB + 1] : f[CB + 0];
-    if (emf_hll) emf = R64 ? hll_emf_flux<N>(l64, r64, e64) : (double)hll_emf_flux32<N>(l, r, e);
-    F.p[6][t] = emf;
+    F.p[6][t] = emf_hll ? emf_h : (N == 0 ? f[CB + 1] : f[CB + 0]);
     fD = f[CD];
   }
   diag_accumulate(diag_shard(d), active, st, 0, dbits(fabs(fD)));
