@@ -41,10 +41,18 @@ FP64_FLOPS_PER_ELEMENT = {("fp64", "sweep_x_a"): 18878.0, ("fp64", "sweep_y_a"):
 DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "sweep_x_b",
                    "fp64_local": "sweep_x_b"}
 
+# wave-level VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU on the
+# 2048^2 rotor, profiles/rotor2048_counters.md) and the issue peak they are
+# held against: 256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles
+# (16 lanes), 2.4 GHz
+VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 2.93e8}
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
 # (form, kernel class, n)
-MEASURED_TRAFFIC = {("fp64", "sweep_x_a", 2048): 416.8e6, ("fp64", "c2p_a", 2048): 376.3e6}
+MEASURED_TRAFFIC = {("mixed", "sweep_x_b", 2048): 420.3e6, ("fp64", "sweep_x_a", 2048): 416.8e6,
+                    ("fp64", "c2p_a", 2048): 376.3e6, ("mixed", "c2p_a", 2048): 490.1e6}
 
 
 def algorithmic_bytes(kernel, n, ng, precision="fp64"):
@@ -252,6 +260,12 @@ def _rotor_form(a, precision, kernel, U_init, P_init):
         tf = flops * kelems / (avg_ms * 1e-3) / 1e12
         valu = {"kernel": kernel, "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": tf / FP64_PEAK_TFLOPS}
+    insts = VALU_INSTS_PER_LAUNCH.get((precision, kernel, n))
+    if insts is not None:
+        rate = insts / (avg_ms * 1e-3)
+        roof["valu_issue"] = {"insts_per_launch": insts, "achieved": rate,
+                              "peak": VALU_ISSUE_PEAK, "unit": "wave-instructions/s",
+                              "frac": rate / VALU_ISSUE_PEAK}
     return {"step": step, "zc": zc, "wall": wall, "roofline": roof, "fp64_valu": valu,
             "ms_per_step": 1e3 * wall / a.steps}
 
