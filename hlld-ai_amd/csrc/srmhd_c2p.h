@@ -118,6 +118,56 @@ struct KastaunT {
     return hi;
   }
 
+  // mu_plus(), bit for bit, evaluating fa only where the bisection's decision
+  // is not already fixed.  The decision at mid is fa(mid) >= 0 (fa(lo) < 0
+  // always; fa values are 0 or at least one ulp of 1 in magnitude, so the
+  // product with fa(lo) cannot underflow).  fa is strictly increasing on
+  // (0, 1]: mu^2 (1 + rbar^2) is a sum of products of positive increasing
+  // factors (mu, mu chi, mu (1 + chi)), with slope at least 2 mu, so
+  // fa' >= 1/sqrt(1 + rbar^2) >= 1/sqrt(1 + r^2) =: m0, and the root mu* lies
+  // in [m0, 1] (rbar^2 <= r^2).  rbar^2 is a sum of non-negative terms and
+  // the final "- 1" is exact near the root, so fa's rounding error is at most
+  // ~10 ulp of max(1, mu sqrt(1 + rbar^2)) <= 2.3e-15 (1 + r^2)^(1/2).  Hence
+  // wherever |mid - mu*| > 1e-13 (1 + r^2) the sign of fa(mid) -- the
+  // decision -- is the sign of mid - mu*, with a margin of over 35.  mu* comes
+  // from bracketed Newton steps on [m0, 1] (forward-mode derivative) to
+  // |fa| <= 1e-14; anything non-finite, no convergence within 40 steps or
+  // r^2 > 1e8 takes the full bisection.
+  SRMHD_DEV double mu_plus_fast() const {
+    if (!(r2 <= 1e8)) return mu_plus();
+    double a = 1.0 / csqrt(1.0 + r2), b = 1.0 / (1.0 + TINY), x = a;
+    bool ok = false;
+#pragma unroll 1
+    for (int it = 0; it < 40; ++it) {
+      const Dual<1> m = Dual<1>::seed(x, 0);
+      const Dual<1> f = m * csqrt(1.0 + rbar2(m, chi(m))) - 1.0;
+      if (!(f.v == f.v) || !(f.d[0] > 0.0) || !(f.d[0] < 1e300)) break;
+      // |fa(x)| <= 1e-14 puts x within (1e-14 + fa's rounding) / m0
+      // <= 1.3e-14 (1 + r^2) of mu*, an eighth of delta
+      if (fabs(f.v) <= 1e-14) {
+        ok = true;
+        break;
+      }
+      if (f.v < 0.0) a = x; else b = x;
+      double xn = x - f.v / f.d[0];
+      if (!(xn > a && xn < b)) xn = 0.5 * (a + b);
+      x = xn;
+    }
+    if (!ok) return mu_plus();
+    const double delta = 1e-13 * (1.0 + r2);
+    double lo = 0.0, hi = 1.0 / (1.0 + TINY);
+#pragma unroll 1
+    for (int it = 0; it < 50; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      const bool up = fabs(mid - x) > delta ? mid > x : fa(mid) >= 0.0;
+      if (up)
+        hi = mid;
+      else
+        lo = mid;
+    }
+    return hi;
+  }
+
   // one guarded secant step   c2p.py:219-232
   SRMHD_DEV void secant_step(double& x0, double& f0, double& x1, double& f1) const {
     const double df = f1 - f0;
@@ -189,7 +239,7 @@ struct C2PRec {
 
 // Pass A.  Returns the element's not-converged mask (bit k: after k steps).
 SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRec& rec) {
-  const double mup = k.mu_plus();
+  const double mup = k.mu_plus_fast();   // bit-identical to k.mu_plus()
   double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
   double f0 = k.master(x0), f1 = k.master(x1);
   const bool good = (f0 * f1) < 0.0;
@@ -283,7 +333,7 @@ SRMHD_DEV int c2p_pass_b(const Kastaun& k, const C2PRec& rec, int K, double atmo
 // the same root from it).
 SRMHD_DEV int c2p_single(const Kastaun& k, double atmo_rho, double atmo_eps, Prim& P,
                          double* mu_out = nullptr, double mup_in = -1.0) {
-  const double mup = mup_in >= 0.0 ? mup_in : k.mu_plus();
+  const double mup = mup_in >= 0.0 ? mup_in : k.mu_plus_fast();
   double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
   double f0 = k.master(x0), f1 = k.master(x1);
   const bool good = (f0 * f1) < 0.0;
