@@ -181,6 +181,12 @@ def bench_tubes(a):
         torch.cuda.synchronize()
     loop_wall = time.perf_counter() - t0
     loop_zc = int(lengths[:n_loop].sum()) * a.steps / loop_wall
+    # compulsory bytes per cell and RK stage, each phase reading and writing
+    # its arrays once: HLLD pass A 64 in (P, By, Bz) + 24 out (record); pass B
+    # 88 in + 56 out (7 fluxes); update 168 in (F, U0, U) + 56 out; cell
+    # con2prim pass A 56 in + 24 out; pass B 80 in + 48 out (P) -- 664 B; plus
+    # 64 B per cell and step for the CFL reduction
+    tube_bytes = float(zones) * a.steps * (3 * 664 + 64)
     out = {
         "metric": "zone-cycles/s, batched 1D Riemann tubes (4096 ragged tubes, MP5/HLLD/RK3)",
         "value": zc, "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -192,6 +198,12 @@ def bench_tubes(a):
                                "persistent launch",
                    "tubes": a.tubes, "cells": zones, "block": best},
         "kernel_ms": kernel_ms,
+        "roofline": {"bound": "hbm", "achieved": tube_bytes / (kernel_ms * 1e-3) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": tube_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_tubes (the persistent launch, every step)",
+                     "avg_launch_ms": kernel_ms, "launches": 1,
+                     "algorithmic_bytes_per_launch": tube_bytes},
         "per_tube_loop": {"value": loop_zc, "unit": "zone-cycles/s", "tubes_timed": n_loop,
                           "speedup_of_batch": zc / loop_zc},
         "autotune_ms": tune,
