@@ -30,11 +30,11 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 # FP64 VALU work per element of the dominant kernels, from rocprofv3 --pmc
-# SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 on the 1024^2 rotor (profiles/
-# sweep_x_a_rotor_counters.md): flops = 64 lanes x (add + mul + 2 fma + trans)
-# per wave, divided by the elements of the launch.
-FP64_FLOPS_PER_ELEMENT = {("fp64", "sweep_x_a"): 18878.0, ("fp64", "sweep_y_a"): 18878.0,
-                          ("fp64", "c2p_a"): 15487.0}
+# SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 on the 2048^2 rotor
+# (profiles/rotor2048_counters.md): flops = 64 lanes x (add + mul + 2 fma +
+# trans) per wave, divided by the elements of the launch.
+FP64_FLOPS_PER_ELEMENT = {("fp64", "sweep_x_a"): 16980.0, ("fp64", "sweep_y_a"): 16980.0,
+                          ("fp64", "c2p_a"): 13584.0}
 
 # the kernel class each form spends most time in (rocprofv3 --stats of the
 # 2048^2 rotor: profiles/rotor2048_weno5z_hlld_kernel_stats.md)
@@ -45,7 +45,7 @@ DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "swee
 # 2048^2 rotor, profiles/rotor2048_counters.md) and the issue peak they are
 # held against: 256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles
 # (16 lanes), 2.4 GHz
-VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 2.93e8}
+VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 2.93e8, ("fp64", "sweep_x_a", 2048): 1.394e9}
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
