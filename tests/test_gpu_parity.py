@@ -246,6 +246,45 @@ def test_2d_variants_bitwise_vs_oracle(case):
     _state_eq(st, so, "oracle")
 
 
+@pytest.mark.parametrize("solver,emf", [("hlld", "solver"), ("hlld", "hll"), ("hllc", "solver"),
+                                        ("hlle", "solver")])
+def test_2d_sweep_diag_equals_oracle(solver, emf):
+    """The per-sweep counters of the last stage (the reference's LAST_DIAG
+    of each sweep, returned by rk_step_ct as aux["x"/"y"]["diag"]) equal the
+    oracle's: the device accumulates them in 64 per-line shards per sweep and
+    folds them after the sweep."""
+    from srmhd import driver2d as D2, eos as E, grid as G, initial_data2d as I
+    import oracle.eos as OE
+    import oracle.riemann as OR
+    import oracle.step2d as OS
+    _flush(f"2d sweep diag {solver} {emf}")
+    dev = _dev()
+    n, lim = 24, "mc"
+    eos = E.hybrid_eos(0.0, 5 / 3, 5 / 3)
+    grid = G.Grid2D(-0.5, 0.5, n, -0.5, 0.5, n, ng=2, device=dev)
+    P, Az = I.PROBLEMS["rotor"](grid, eos)
+    st = I.init_state(grid, eos, P, Az, limiter=lim)
+    oe = OE.ideal(5 / 3)
+    go = OS.make_grid(n, (-0.5, 0.5), lim)
+    Po, Azo = OS.rotor_sharp(go, oe)
+    so = OS.init_state(go, oe, Po, Azo)
+    for _ in range(2):
+        dt = D2.compute_dt_2d(st.prims, grid, eos, 0.4)
+        st, diag = D2.rk_step_ct(st, grid, eos, dt, scheme="rk3", flux_fn=solver, limiter=lim,
+                                 emf_mode=emf)
+        dto = OS.compute_dt_2d(so.prims, go, oe, 0.4)
+        so, aux = OS.rk_step_ct(so, go, oe, dto, scheme="rk3", flux_fn=OR.SOLVERS[solver],
+                                limiter=lim, emf_mode=emf)
+    for ax in ("x", "y"):
+        d, do = dict(diag[ax]["diag"]), dict(aux[ax]["diag"])
+        assert d["n_interfaces"] == do["n_interfaces"] == (n + 5) * (n + 4)
+        if solver == "hlld":
+            for key in ("n_not_converged", "n_wave_order_bad", "n_bn_zero", "n_hlle_fallback",
+                        "max_iters"):
+                assert d[key] == do[key], (ax, key)
+            assert d["mean_iters"] == do["mean_iters"], ax
+
+
 def test_periodic_ot_emf_hll_bitwise_vs_oracle():
     _flush("2d ot2pi mc hlld emf hll")
     args = ("orszag_tang_2pi", 16, "mc", "hlld", "rk2", 4 / 3, (0.0, 2 * np.pi), "periodic")
