@@ -500,8 +500,9 @@ def main():
                          "FETCH/WRITE counters)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the domain-decomposed design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (the step would "
+              "shard as x-slabs with halo exchange; only the one-GPU kernel is built)",
+              file=sys.stderr)
         sys.exit(2)
     if a.workload == "tubes":
         if a.steps is None:

@@ -6,8 +6,8 @@ Run in a container that holds the reference checkout (default
     python tests/golden/make_golden.py
 
 It imports the reference's own functions for exactly the parts that no
-closed form or invariant pins (SURVEY of the reference: hlld_flux/hllc_flux
-branch decisions and values, the batch-coupled Kastaun con2prim,
+closed form or invariant pins (hlld_flux/hllc_flux branch decisions and
+values, the batch-coupled Kastaun con2prim,
 reconstruct_prims with its PCM fallback, rk_step_ct, driver.run on
 Balsara 1, run_tubes), feeds them inputs generated here from fixed seeds,
 and stores inputs and outputs as .npz data.  No reference source is copied;
