@@ -564,10 +564,10 @@ F32_DEV float star_residual(const StarR& R, float p) {
 // the bracket with its expansions and the clamped secant (hlld.py:1255-1356).
 // Returns whether |residual| < PSTAR_TOL was reached; ps, p_hll out.
 template <int N>
-F32_DEV bool hlld_pstar(const Prim& L, const Prim& Rp, const Eos& e, float& ps, float& p_hll) {
+F32_DEV bool hlld_pstar_sides(const float uL[8], const float uR[8], const float fL[8],
+                              const float fR[8], float cmax, float cmin, const Eos& e, float& ps,
+                              float& p_hll) {
   constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
-  float uL[8], uR[8], fL[8], fR[8], cmax, cmin;
-  srmhd_fluxes<N>(L, Rp, e, uL, uR, fL, fR, cmax, cmin);
   float fH[8], uH[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
@@ -642,6 +642,14 @@ F32_DEV bool hlld_pstar(const Prim& L, const Prim& Rp, const Eos& e, float& ps, 
   return conv;
 }
 
+// the same from the two face states
+template <int N>
+F32_DEV bool hlld_pstar(const Prim& L, const Prim& Rp, const Eos& e, float& ps, float& p_hll) {
+  float uL[8], uR[8], fL[8], fR[8], cmax, cmin;
+  srmhd_fluxes<N>(L, Rp, e, uL, uR, fL, fR, cmax, cmin);
+  return hlld_pstar_sides<N>(uL, uR, fL, fR, cmax, cmin, e, ps, p_hll);
+}
+
 // HLLD flux (hlld.py:1202-1543) in FP32; returns the face status bits of the
 // FP64 path (not converged, wave order, B_n = 0, fallback).
 template <int N>
@@ -656,10 +664,10 @@ F32_DEV int hlld(const Prim& L, const Prim& Rp, const Eos& e, float F[8]) {
     for (int k = 0; k < 8; ++k) F[k] = hll_flux(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
     return FS_BN_ZERO | FS_FALLBACK;
   }
-  float ps, p_hll;
-  const bool conv = hlld_pstar<N>(L, Rp, e, ps, p_hll);
   float uL[8], uR[8], fL[8], fR[8], cmax, cmin;
   srmhd_fluxes<N>(L, Rp, e, uL, uR, fL, fR, cmax, cmin);
+  float ps, p_hll;
+  const bool conv = hlld_pstar_sides<N>(uL, uR, fL, fR, cmax, cmin, e, ps, p_hll);
   float fH[8], uH[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
