@@ -45,7 +45,7 @@ DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "swee
 # 2048^2 rotor, profiles/rotor2048_counters.md) and the issue peak they are
 # held against: 256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles
 # (16 lanes), 2.4 GHz
-VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 2.93e8, ("fp64", "sweep_x_a", 2048): 1.394e9}
+VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 2.761e8, ("fp64", "sweep_x_a", 2048): 1.394e9}
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
