@@ -731,26 +731,38 @@ void fixup_dispatch(int lim, const G2& g, const St2& S, const Pr2& P, const srmh
 struct AuxStream {
   hipStream_t s = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
+  bool failed = false;   // creation failed once: this device stays on one stream
 };
 constexpr int AUX_MAX_DEVICES = 64;
 AuxStream g_aux[AUX_MAX_DEVICES];
 std::mutex g_aux_mu;   // held while a stage enqueues its fork/join
 
+// The current device's auxiliary stream, created at first use (nullptr when
+// unavailable: the caller then stays on its own stream).  Caller holds
+// g_aux_mu.
 AuxStream* aux_stream() {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= AUX_MAX_DEVICES) return nullptr;
   AuxStream& a = g_aux[dev];
-  if (!a.s) {
-    hipStream_t st;
-    hipEvent_t f, j;
-    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return nullptr;
-    if (hipEventCreateWithFlags(&f, hipEventDisableTiming) != hipSuccess) return nullptr;
-    if (hipEventCreateWithFlags(&j, hipEventDisableTiming) != hipSuccess) return nullptr;
+  if (!a.s && !a.failed) {
+    hipStream_t st = nullptr;
+    hipEvent_t f = nullptr, j = nullptr;
+    const bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+                    hipEventCreateWithFlags(&f, hipEventDisableTiming) == hipSuccess &&
+                    hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      if (j) (void)hipEventDestroy(j);
+      if (f) (void)hipEventDestroy(f);
+      if (st) (void)hipStreamDestroy(st);
+      (void)hipGetLastError();   // the failure is handled here, not by the next launch check
+      a.failed = true;
+      return nullptr;
+    }
     a.fork = f;
     a.join = j;
     a.s = st;
   }
-  return &a;
+  return a.s ? &a : nullptr;
 }
 
 void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
