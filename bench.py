@@ -45,13 +45,13 @@ DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "swee
 # 2048^2 rotor, profiles/rotor2048_counters.md) and the issue peak they are
 # held against: 256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles
 # (16 lanes), 2.4 GHz
-VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 2.761e8, ("fp64", "sweep_x_a", 2048): 1.394e9}
+VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 1.836e8, ("fp64", "sweep_x_a", 2048): 1.394e9}
 VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
 # (form, kernel class, n)
-MEASURED_TRAFFIC = {("mixed", "sweep_x_b", 2048): 420.3e6, ("fp64", "sweep_x_a", 2048): 416.8e6,
+MEASURED_TRAFFIC = {("mixed", "sweep_x_b", 2048): 420.1e6, ("fp64", "sweep_x_a", 2048): 416.8e6,
                     ("fp64", "c2p_a", 2048): 376.3e6, ("mixed", "c2p_a", 2048): 490.1e6}
 
 

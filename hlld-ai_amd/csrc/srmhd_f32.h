@@ -41,11 +41,38 @@ constexpr int FS_DOUBT = 32;
 #define F32_DEV __host__ __device__ __forceinline__
 
 F32_DEV float sq(float x) { return x * x; }
-F32_DEV float fsqrt(float x) { return __builtin_sqrtf(x); }
-F32_DEV float sgn(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
-F32_DEV float sdiv(float n, float d) {
-  return n / (fabsf(d) < TINY ? (d < 0.f ? -TINY : TINY) : d);
+// sqrt(x) as the hardware square root (1 ulp), without the default
+// lowering's rescaling of inputs below 2^-96
+F32_DEV float fsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return __builtin_sqrtf(x);
+#endif
 }
+F32_DEV float sgn(float x) { return x > 0.f ? 1.f : (x < 0.f ? -1.f : 0.f); }
+// n / d as the hardware reciprocal (1 ulp) times n.  The default FP32
+// lowering wraps the reciprocal in exponent scaling (frexp, rcp, ldexp: 7
+// instructions a division) that matters only near the denormal and overflow
+// ranges; the physical magnitudes divided here stay far from both.
+F32_DEV float fdiv(float n, float d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return n * __builtin_amdgcn_rcpf(d);
+#else
+  return n / d;
+#endif
+}
+// 1 / sqrt(x) in one instruction (1 ulp)
+F32_DEV float frsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rsqf(x);
+#else
+  return 1.f / __builtin_sqrtf(x);
+#endif
+}
+// n / d with |d| raised to at least TINY, keeping its sign (d = -0 counts as
+// negative)
+F32_DEV float sdiv(float n, float d) { return fdiv(n, copysignf(fmaxf(fabsf(d), TINY), d)); }
 
 struct Prim {
   float rho, v[3], p, eps, B[3];
@@ -60,7 +87,7 @@ struct Eos {
         k0(e.K == 0.0) {}
   F32_DEV void cold_of(float rho, float& pc, float& ec) const {
     pc = k0 ? 0.f : K * powf(rho, g);
-    ec = k0 ? 0.f : pc / (rho * (g - 1.f));
+    ec = k0 ? 0.f : fdiv(pc, rho * (g - 1.f));
   }
   F32_DEV float press(float eps, float rho) const {
     float pc, ec;
@@ -72,7 +99,7 @@ struct Eos {
     float pc, ec;
     cold_of(rho, pc, ec);
     if (cold) return ec;
-    return ec + (p - pc) / (rho * (gth - 1.f));
+    return ec + fdiv(p - pc, rho * (gth - 1.f));
   }
   F32_DEV float eps_lo(float rho) const {
     float pc, ec;
@@ -81,10 +108,10 @@ struct Eos {
   }
   F32_DEV void press_cs2(float eps, float rho, float& p, float& cs2) const {
     p = press(eps, rho);
-    const float h = 1.f + eps + p / rho;
+    const float h = 1.f + eps + fdiv(p, rho);
     float pc = 0.f, ec;
     if (!k0) cold_of(rho, pc, ec);
-    cs2 = cold ? g * p / (rho * h) : (g * pc + gth * (p - pc)) / (rho * h);
+    cs2 = fdiv(cold ? g * p : g * pc + gth * (p - pc), rho * h);
   }
 };
 
@@ -108,10 +135,10 @@ F32_DEV float weno5z(float vm2, float vm1, float v0, float vp1, float vp2) {
   // tau5 / (beta + eps) can exceed sqrt(FLT_MAX) where a beta vanishes; the
   // ratio is capped at 1e18 (the weights are then one-hot to FP32 precision,
   // as the FP64 weights are)
-  const float a0 = 0.1f * (1.f + sq(fminf(t5 / (b0 + 1e-30f), 1e18f)));
-  const float a1 = 0.6f * (1.f + sq(fminf(t5 / (b1 + 1e-30f), 1e18f)));
-  const float a2 = 0.3f * (1.f + sq(fminf(t5 / (b2 + 1e-30f), 1e18f)));
-  return (a0 * p0 + a1 * p1 + a2 * p2) / (a0 + a1 + a2);
+  const float a0 = 0.1f * (1.f + sq(fminf(fdiv(t5, b0 + 1e-30f), 1e18f)));
+  const float a1 = 0.6f * (1.f + sq(fminf(fdiv(t5, b1 + 1e-30f), 1e18f)));
+  const float a2 = 0.3f * (1.f + sq(fminf(fdiv(t5, b2 + 1e-30f), 1e18f)));
+  return fdiv(a0 * p0 + a1 * p1 + a2 * p2, a0 + a1 + a2);
 }
 F32_DEV float mp_limit(float vl, float vm2, float vm1, float v0, float vp1, float vp2) {
   const float alpha = 4.f;
@@ -171,12 +198,12 @@ F32_DEV void face_states(const float q[8], int f, int n, float& ql, float& qr) {
 
 // ── states and fluxes ───────────────────────────────────────────────────────
 F32_DEV float lorentz_W(const float v[3]) {
-  return 1.f / fsqrt(1.f - fminf(sq(v[0]) + sq(v[1]) + sq(v[2]), V2CAP));
+  return frsqrt(1.f - fminf(sq(v[0]) + sq(v[1]) + sq(v[2]), V2CAP));
 }
 F32_DEV float b_squared(const float v[3], const float B[3], float W) {
   const float B2 = sq(B[0]) + sq(B[1]) + sq(B[2]);
   const float vdB = v[0] * B[0] + v[1] * B[1] + v[2] * B[2];
-  return B2 / (W * W) + sq(vdB);
+  return fdiv(B2, W * W) + sq(vdB);
 }
 F32_DEV void char_speeds(float v02, float u0, float vd, float& cp, float& cm) {
   const float u0sq = u0 * u0;
@@ -184,8 +211,8 @@ F32_DEV void char_speeds(float v02, float u0, float vd, float& cp, float& cm) {
   const float b = -2.f * u0sq * vd * (1.f - v02);
   const float c = u0sq * sq(vd) * (1.f - v02) - v02;
   const float det = fsqrt(fmaxf(b * b - 4.f * a * c, 0.f));
-  const float r1 = 0.5f * (det - b) / a;
-  const float r2 = -0.5f * (det + b) / a;
+  const float r1 = fdiv(0.5f * (det - b), a);
+  const float r2 = fdiv(-0.5f * (det + b), a);
   cp = fmaxf(r1, r2);
   cm = fminf(r1, r2);
 }
@@ -202,9 +229,9 @@ F32_DEV void side_state(const Prim& P, const Eos& e, float U[8], float F[8], flo
   const float bt = W * vdB;
   float b[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) b[i] = P.B[i] / W + vdB * P.v[i] * W;
-  const float h = 1.f + P.eps + p / P.rho;
-  const float v02 = cs2 + (b2 / (P.rho * h + b2)) * (1.f - cs2);
+  for (int i = 0; i < 3; ++i) b[i] = fdiv(P.B[i], W) + vdB * P.v[i] * W;
+  const float h = 1.f + P.eps + fdiv(p, P.rho);
+  const float v02 = cs2 + fdiv(b2, P.rho * h + b2) * (1.f - cs2);
   char_speeds(v02, W, vn, cp, cm);
   const float d = P.rho * W;
   const float rhb2 = P.rho * h + b2;
@@ -228,10 +255,10 @@ F32_DEV void side_state(const Prim& P, const Eos& e, float U[8], float F[8], flo
 }
 
 F32_DEV float hll_flux(float fl, float fr, float ul, float ur, float cm, float cx) {
-  return (cx * fl + cm * fr - cx * cm * (ur - ul)) / (cx + cm);
+  return fdiv(cx * fl + cm * fr - cx * cm * (ur - ul), cx + cm);
 }
 F32_DEV float hll_state(float fl, float fr, float ul, float ur, float cm, float cx) {
-  return (cx * ur + cm * ul + fl - fr) / (cx + cm);
+  return fdiv(cx * ur + cm * ul + fl - fr, cx + cm);
 }
 
 template <int N>
@@ -290,9 +317,9 @@ F32_DEV void hllc(const Prim& L, const Prim& R, const Eos& e, float F[8]) {
   const float vt1 = hasB ? sdiv(uH[CB + T1] * vc - fH[CB + T1], Bn) : 0.f;
   const float vt2 = hasB ? sdiv(uH[CB + T2] * vc - fH[CB + T2], Bn) : 0.f;
   const float v2 = sq(vc) + sq(vt1) + sq(vt2);
-  const float g = 1.f / fsqrt(1.f - fminf(v2, V2CAP));
+  const float g = frsqrt(1.f - fminf(v2, V2CAP));
   const float vB = hasB ? vc * uH[CB + N] + vt1 * uH[CB + T1] + vt2 * uH[CB + T2] : 0.f;
-  const float pB = fH[CS + N] + sq(Bn / g) - vc * (fH[CT] + fH[CD] - Bn * vB);
+  const float pB = fH[CS + N] + sq(fdiv(Bn, g)) - vc * (fH[CT] + fH[CD] - Bn * vB);
   const float p0 = fH[CS + N] - vc * (fH[CT] + fH[CD]);
   const float ps = hasB ? pB : p0;
   float ucL[8], ucR[8];
@@ -310,8 +337,8 @@ F32_DEV void hllc(const Prim& L, const Prim& R, const Eos& e, float F[8]) {
     s[CD] = sdiv(u[CD] * (S - vd), den);
     s[CT] = sdiv(S * u[CT] - f[CT] + ps * vc - vB * Bn, den);
     s[CS + N] = (s[CT] + s[CD] + ps) * vc - vB * Bn;
-    const float w1 = sdiv(-Bn * (uH[CB + T1] / (g * g) + vB * vt1) - f[CS + T1] + S * u[CS + T1], den);
-    const float w2 = sdiv(-Bn * (uH[CB + T2] / (g * g) + vB * vt2) - f[CS + T2] + S * u[CS + T2], den);
+    const float w1 = sdiv(-Bn * (fdiv(uH[CB + T1], g * g) + vB * vt1) - f[CS + T1] + S * u[CS + T1], den);
+    const float w2 = sdiv(-Bn * (fdiv(uH[CB + T2], g * g) + vB * vt2) - f[CS + T2] + S * u[CS + T2], den);
     s[CS + T1] = hasB ? w1 : sdiv(u[CS + T1] * (S - vd), den);
     s[CS + T2] = hasB ? w2 : sdiv(u[CS + T2] * (S - vd), den);
   }
@@ -342,26 +369,27 @@ struct Kastaun {
     float S2 = sq(S[0]) + sq(S[1]) + sq(S[2]);
     const float Sn = fsqrt(S2);
     const float Smax = D + tau;
-    const float fac = (Sn > Smax) ? 0.9999f * Smax / (Sn + TINY) : 1.f;
+    const float fac = (Sn > Smax) ? fdiv(0.9999f * Smax, Sn + TINY) : 1.f;
 #pragma unroll
     for (int i = 0; i < 3; ++i) S[i] *= fac;
     S2 = sq(S[0]) + sq(S[1]) + sq(S[2]);
     const float B2 = sq(B[0]) + sq(B[1]) + sq(B[2]);
     const float BS = B[0] * S[0] + B[1] * S[1] + B[2] * S[2];
-    const float iD = 1.f / (D + TINY);
+    const float iD = fdiv(1.f, D + TINY);
     q = tau * iD;
 #pragma unroll
     for (int i = 0; i < 3; ++i) rv[i] = S[i] * iD;
-    r2 = S2 / (D * D + TINY);
+    r2 = fdiv(S2, D * D + TINY);
     const float sD = fsqrt(D + TINY);
+    const float isD = fdiv(1.f, sD);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) btv[i] = B[i] / sD;
+    for (int i = 0; i < 3; ++i) btv[i] = B[i] * isD;
     bt2 = B2 * iD;
-    br = BS / (D * sD + TINY);
+    br = fdiv(BS, D * sD + TINY);
     bperp2 = r2 * bt2 - br * br;
-    v02 = r2 / (r2 + 1.f);
+    v02 = fdiv(r2, r2 + 1.f);
   }
-  F32_DEV float chi(float mu) const { return 1.f / (1.f + mu * bt2); }
+  F32_DEV float chi(float mu) const { return fdiv(1.f, 1.f + mu * bt2); }
   F32_DEV float rbar2(float mu, float c) const {
     return r2 * (c * c) + mu * c * (1.f + c) * (br * br);
   }
@@ -372,13 +400,14 @@ struct Kastaun {
     const float qb = q - 0.5f * bt2 - 0.5f * (mc * mc) * bperp2;
     const float rb2 = rbar2(mu, c);
     const float vh2 = fminf(mu * mu * rb2, v02);
-    const float Wh = 1.f / fsqrt(fmaxf(1.f - vh2, 1.2e-7f));
-    const float rho = fminf(fmaxf(D / Wh, 1e-12f), 1e15f);
-    const float eps0 = Wh * (qb - mu * rb2) + vh2 * (Wh * Wh) / (1.f + Wh);
+    const float iWh = fsqrt(fmaxf(1.f - vh2, 1.2e-7f));
+    const float Wh = fdiv(1.f, iWh);
+    const float rho = fminf(fmaxf(D * iWh, 1e-12f), 1e15f);
+    const float eps0 = Wh * (qb - mu * rb2) + fdiv(vh2 * (Wh * Wh), 1.f + Wh);
     const float eps = fminf(fmaxf(eps0, e.eps_lo(rho)), 1e5f);
     const float p = e.press(eps, rho);
-    const float a = p / (rho * (1.f + eps) + TINY);
-    const float nu = fmaxf((1.f + a) * (1.f + eps) / Wh, (1.f + a) * (1.f + qb - mu * rb2));
+    const float a = fdiv(p, rho * (1.f + eps) + TINY);
+    const float nu = fmaxf((1.f + a) * (1.f + eps) * iWh, (1.f + a) * (1.f + qb - mu * rb2));
     if (rho_out) *rho_out = rho;
     if (eps_out) *eps_out = eps;
     return mu - sdiv(1.f, nu + mu * rb2);
@@ -419,7 +448,7 @@ struct Kastaun {
       }
       const float df = f1 - f0;
       const float mid = 0.5f * (x0 + x1);
-      float x2 = (fabsf(df) > 1e-30f) ? x1 - f1 * (x1 - x0) / df : mid;
+      float x2 = (fabsf(df) > 1e-30f) ? x1 - fdiv(f1 * (x1 - x0), df) : mid;
       x2 = ((x2 - x0) * (x2 - x1) < 0.f) ? x2 : mid;
       const float f2 = master(x2);
       if (!((f0 * f2) <= 0.f)) {
@@ -457,7 +486,7 @@ struct Kastaun {
     const float v2 = sq(v[0]) + sq(v[1]) + sq(v[2]);
     const float lim = V2CAP;
     if (v2 > lim) {
-      const float nrm = fsqrt(lim / (v2 + TINY));
+      const float nrm = fsqrt(fdiv(lim, v2 + TINY));
 #pragma unroll
       for (int i = 0; i < 3; ++i) v[i] *= nrm;
     }
@@ -582,12 +611,12 @@ F32_DEV bool hlld_pstar_sides(const float uL[8], const float uR[8], const float 
   const float disc = fmaxf(b * b - 4.f * a * c, 0.f);
   const bool big = fabsf(a) > 1e-14f;
   // with |a| small the reference divides -c by torch.where(big, b, 1) = 1
-  float vs = big ? (-b - fsqrt(disc)) / (2.f * a) : -c;
+  float vs = big ? fdiv(-b - fsqrt(disc), 2.f * a) : -c;
   vs = fminf(fmaxf(vs, -V2CAP), V2CAP);
   const float p0 = fmaxf(fH[CS + N] - (fH[CT] + fH[CD]) * vs, 1e-30f);
 
   const float BnL = uL[CB + N], BnR = uR[CB + N];
-  const bool use_p0 = (0.5f * (BnL * BnL + BnR * BnR) / p_hll) < 0.1f;
+  const bool use_p0 = fdiv(0.5f * (BnL * BnL + BnR * BnR), p_hll) < 0.1f;
   float plo = use_p0 ? p0 * 0.99f : p_hll * 0.99f;
   float phi = use_p0 ? p0 * 1.01f : p_hll * 1.01f;
 
@@ -629,7 +658,7 @@ F32_DEV bool hlld_pstar_sides(const float uL[8], const float uR[8], const float 
       break;
     }
     const float df = f1 - f0;
-    const float dx = (fabsf(df) > 1e-30f * fabsf(f1)) ? f1 * (x1 - x0) / df : 0.f;
+    const float dx = (fabsf(df) > 1e-30f * fabsf(f1)) ? fdiv(f1 * (x1 - x0), df) : 0.f;
     const float x2 = fmaxf(x1 - dx, 1e-30f);
     const float f2 = star_residual(R, x2);
     x0 = x1;

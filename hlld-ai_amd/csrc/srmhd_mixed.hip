@@ -91,7 +91,7 @@ __device__ void face_prims_c32(const G2& g, const St2& S, const Cells32& C, cons
   for (int s = 0; s < 2; ++s) {
     const float* x = s ? qr : ql;
     f32::Prim& Q = s ? R : L;
-    const float fac = 1.f / f32::fsqrt(1.f + (x[2] * x[2] + x[3] * x[3] + x[4] * x[4]));
+    const float fac = f32::frsqrt(1.f + (x[2] * x[2] + x[3] * x[3] + x[4] * x[4]));
     Q.v[0] = x[2] * fac;
     Q.v[1] = x[3] * fac;
     Q.v[2] = x[4] * fac;
