@@ -40,6 +40,19 @@ __global__ void k_zero_u64(unsigned long long* w, int n) {
   if (i < n) w[i] = 0ull;
 }
 
+// up to five regions cleared by one launch (a grid covering the longest);
+// a null region is skipped
+struct ZeroSet {
+  unsigned long long* p[5];
+  int n[5];
+};
+__global__ void k_zero_set(ZeroSet z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    if (z.p[k] && i < z.n[k]) z.p[k][i] = 0ull;
+}
+
 // ── flat flux (flux_fn) ─────────────────────────────────────────────────────
 
 template <int N>
@@ -637,6 +650,7 @@ Ws2 carve2d(const G2& g, void* work, bool cells32 = false) {
 constexpr int N_WORDS = (int)(2 * sizeof(srmhd_flux_diag) / 8);
 constexpr int DIAG_SHARD_WORDS = 2 * DIAG_SHARDS * DIAG_SHARD_BYTES / 8;
 constexpr int DIAG_ZERO_BLOCKS = (DIAG_SHARD_WORDS + 255) / 256;
+static_assert(N_WORDS <= DIAG_ZERO_BLOCKS * 256, "the stage's zeroing grid covers the diagnostics");
 
 // the two sweeps of ct_stage for one (limiter, solver)
 template <int LIM, int SOL>
@@ -768,10 +782,9 @@ AuxStream* aux_stream() {
 void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
                   const srmhd_eos& e, const Ws2& w, hipStream_t s) {
   // the C2PRec areas are unused in this form: they hold the fix-up lists
+  // (their counts, the first word of each, are cleared by the stage's zeroing launch)
   unsigned* fx = (unsigned*)w.recx;
   unsigned* fy = (unsigned*)w.recy;
-  k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)fx, 1);
-  k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)fy, 1);
   if (!r64) cells_f32(g, S, P, w.cf, s);
   srmhd_flux_diag* shx = w.dsh;
   srmhd_flux_diag* shy = (srmhd_flux_diag*)((char*)w.dsh + DIAG_SHARDS * DIAG_SHARD_BYTES);
@@ -1197,10 +1210,23 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   }
   (void)pro;
   hipStream_t st = (hipStream_t)stream;
-  k_zero_u64<<<1, 64, 0, st>>>((unsigned long long*)w.diag, N_WORDS);
-  k_zero_u64<<<DIAG_ZERO_BLOCKS, 256, 0, st>>>((unsigned long long*)w.dsh, DIAG_SHARD_WORDS);
-  k_zero_u64<<<1, 64, 0, st>>>(w.words, 4);
   const bool mixed = sp->precision == SRMHD_PREC_MIXED || sp->precision == SRMHD_PREC_MIXED_R64;
+  {
+    // the folded diagnostics, the sweeps' diagnostic shards, the words and
+    // (mixed forms) the two fix-up list counts, in one launch
+    ZeroSet z;
+    z.p[0] = (unsigned long long*)w.diag;
+    z.n[0] = N_WORDS;
+    z.p[1] = (unsigned long long*)w.dsh;
+    z.n[1] = DIAG_SHARD_WORDS;
+    z.p[2] = w.words;
+    z.n[2] = 4;
+    z.p[3] = mixed ? (unsigned long long*)w.recx : nullptr;
+    z.n[3] = 1;
+    z.p[4] = mixed ? (unsigned long long*)w.recy : nullptr;
+    z.n[4] = 1;
+    k_zero_set<<<DIAG_ZERO_BLOCKS, 256, 0, st>>>(z);
+  }
   const bool local = sp->precision != SRMHD_PREC_FP64;   // per-element con2prim
   if (mixed) {
     sweeps_mixed(sp->limiter, sp->solver, sp->precision == SRMHD_PREC_MIXED_R64, g, s1, pr, *eos,
