@@ -51,7 +51,7 @@ VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
 # (form, kernel class, n)
-MEASURED_TRAFFIC = {("mixed", "sweep_x_b", 2048): 420.1e6, ("fp64", "sweep_x_a", 2048): 416.8e6,
+MEASURED_TRAFFIC = {("mixed", "sweep_x_b", 2048): 420.0e6, ("fp64", "sweep_x_a", 2048): 416.8e6,
                     ("fp64", "c2p_a", 2048): 376.3e6, ("mixed", "c2p_a", 2048): 490.1e6}
 
 

@@ -8,7 +8,7 @@
 //   * the con2prim of the HLL state only seeds the p* bracket (p_lo, p_hi =
 //     0.99, 1.01 p_hll), so it runs per element (no batch coupling) to a
 //     tolerance of 1e-5, its secant bracketed by [0, 1) instead of mu_+;
-//     mu_+ (26-step bisection, FP32 resolution) serves the FP64 cell
+//     mu_+ (14-step bisection: its upper end within 2^-14 above mu_+) serves the FP64 cell
 //     con2prim of the mixed forms as its bracket end;
 //   * the p* secant stops at |residual| < 1e-6 (its FP32 noise floor is
 //     about 1e-7).  A face whose p* does not get there, or whose wave-order
@@ -29,7 +29,10 @@ constexpr float TINY = 1e-30f;
 constexpr float V2CAP = 1.0f - 1.2e-7f;
 constexpr float C2P_TOL = 1e-5f;
 constexpr int C2P_SECANT = 30;
-constexpr int MU_BISECT = 26;
+// 14 steps: fewer leave the FP64 secant a wider bracket that costs more than
+// the steps saved (measured on the 2048^2 rotor: 12-16 steps equal, 10 and
+// fewer slower)
+constexpr int MU_BISECT = 14;
 constexpr int C2P_BISECT = 40;
 constexpr float PSTAR_TOL = 1e-6f;
 constexpr int PSTAR_MAXIT = 30;
@@ -432,7 +435,7 @@ struct Kastaun {
   // The bracket's upper end is 1/h_min = 1 rather than mu_+: the root lies
   // below mu_+ <= 1 and the master function is defined on all of (0, 1]
   // (v^2 clamped), so the secant converges from [0, 1) too, in ~3
-  // evaluations instead of mu_+'s 26 bisection steps (measured on rotor
+  // evaluations instead of mu_+'s 14+ bisection steps (measured on rotor
   // states: every element converges, at most 14 evaluations).
   F32_DEV float total_pressure() const {
     const float mup = 1.f;

@@ -280,9 +280,11 @@ void launch_dir(int lim, int sol, const G2& g, const St2& S, const Pr2& P, const
 
 // con2prim of the physical cells, each cell its own batch, with mu_+ from
 // the FP32 bisection: mu_+ only closes the secant's bracket, which any value
-// at or above it does, so the FP32 root inflated by 4e-6 relative (its error
-// is ~2e-7) plus 4e-7 stands in for the 50 FP64 bisection steps; the FP64
-// secant then converges to the same root, |f| < 1e-15.
+// at or above it does.  The bisection's upper end lies above mu_+ (within
+// 2^-14) except where FP32 rounding flips a decision next to the root (an
+// error of ~2e-7); inflated by 4e-6 relative plus 4e-7 it stands in for the
+// 50 FP64 bisection steps, and the FP64 secant converges to the same root,
+// |f| < 1e-15.
 __global__ __launch_bounds__(BLK) void k_c2p2d_mixed(G2 g, St2 S, srmhd_eos e, double atmo_rho,
                                                      double atmo_eps, Pr2Out P,
                                                      unsigned long long* redo_count) {
