@@ -272,11 +272,25 @@ __global__ void k_fold_diag(const srmhd_flux_diag* dsh, srmhd_flux_diag* d) {
 // FP64 recomputation of the faces the mixed-precision sweep flagged (its
 // list: fix[0] = count, indices from fix[2]): the FP64 face states and HLLD,
 // with the HLL-state con2prim of the face as its own batch.
+// The list is short (tens of faces on a 2048^2 rotor stage) and each face is
+// a long serial FP64 computation whose iteration counts differ from face to
+// face, and the x fix-up runs beside the y sweep, which fills every CU.  So:
+// one-wave workgroups, one face per wave while the list fits the grid (lanes
+// on different faces would serialise each other's branches), at most 256
+// registers (the compiler's own choice is 282, which fits beside only one of
+// the sweep's 118-register waves on a SIMD; 256 fits beside two), and raised
+// issue priority once resident.
+constexpr int FIXUP_BLK = 64;
 template <int N, int LIM>
-__global__ __launch_bounds__(BLK) void k_fixup64(G2 g, St2 S, Pr2 P, srmhd_eos e,
-                                                 const unsigned* fix, Flx F, int emf_hll) {
+__global__ __launch_bounds__(FIXUP_BLK) __attribute__((amdgpu_waves_per_eu(2)))
+void k_fixup64(G2 g, St2 S, Pr2 P, srmhd_eos e, const unsigned* fix, Flx F, int emf_hll) {
   const unsigned cnt = fix[0];
-  for (unsigned q = blockIdx.x * BLK + threadIdx.x; q < cnt; q += gridDim.x * BLK) {
+  const unsigned waves = gridDim.x;
+  const unsigned per = (cnt + waves - 1) / waves;   // faces per wave
+  const unsigned lanes = per < FIXUP_BLK ? per : FIXUP_BLK;
+  if (threadIdx.x >= lanes) return;
+  __builtin_amdgcn_s_setprio(3);
+  for (unsigned q = blockIdx.x * lanes + threadIdx.x; q < cnt; q += waves * lanes) {
     const int64_t t = fix[2 + q];
     int i, j;
     face_ij<N>(g, t, i, j);
@@ -717,12 +731,12 @@ void sweeps_solver(int sol, bool local, const G2& g, const St2& S, const Pr2& P,
 }
 
 // the mixed-precision sweep along dir and the FP64 fix-up of its flagged faces
-// (the count is on the device: a grid of 4 blocks per CU strides over it)
+// (the count is on the device: a grid of 1024 one-wave blocks strides over it)
 constexpr int FIXUP_BLOCKS = 1024;
 template <int N, int LIM>
 void fixup_lim(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const unsigned* fix,
                const Flx& F, int emf_hll, hipStream_t s) {
-  k_fixup64<N, LIM><<<FIXUP_BLOCKS, BLK, 0, s>>>(g, S, P, e, fix, F, emf_hll);
+  k_fixup64<N, LIM><<<FIXUP_BLOCKS, FIXUP_BLK, 0, s>>>(g, S, P, e, fix, F, emf_hll);
 }
 
 template <int N>
