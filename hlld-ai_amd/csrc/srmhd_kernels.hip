@@ -316,30 +316,47 @@ void k_fixup64(G2 g, St2 S, Pr2 P, srmhd_eos e, const unsigned* fix, Flx F, int 
   }
 }
 
-// Gardiner-Stone corner EMF (ct.py:119-177) with the ring of zeros
+// Gardiner-Stone corner EMF (ct.py:119-177) with the ring of zeros.  A block
+// takes a tile of CT_TI x CT_TJ corners (j fastest); the cell EMFs
+// E_c = Bx vy - By vx of the (CT_TI + 1) x (CT_TJ + 1) cells around it are
+// formed once into LDS and each corner reads its four from there.
+constexpr int CT_TI = 8, CT_TJ = 32;
+static_assert(CT_TI * CT_TJ == BLK, "one corner per thread");
+constexpr int CT_LJ = CT_TJ + 1, CT_LN = (CT_TI + 1) * CT_LJ;
 __global__ __launch_bounds__(BLK) void k_corner(G2 g, St2 S, Pr2 P, const double* FxD,
                                                 const double* FxB, const double* FyD,
                                                 const double* FyB, const srmhd_flux_diag* dg,
                                                 int upwind, double* Ez) {
-  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  const int nc = g.nyt + 1;
-  if (t >= (int64_t)(g.nxt + 1) * nc) return;
-  const int i = (int)(t / nc), j = (int)(t - (int64_t)i * nc);
+  __shared__ double L[CT_LN];
+  const int i0 = blockIdx.y * CT_TI, j0 = blockIdx.x * CT_TJ;
+  const double* Bxf = S.p[6];
+  const double* Byf = S.p[7];
+  // cells (i0 - 1 + e / CT_LJ, j0 - 1 + e % CT_LJ); those outside [0, nxt) x
+  // [0, nyt) only neighbour ring corners, which do not read them
+  for (int e = threadIdx.x; e < CT_LN; e += BLK) {
+    const int ci = i0 - 1 + e / CT_LJ, cj = j0 - 1 + e % CT_LJ;
+    double v = 0.0;
+    if (ci >= 0 && ci < g.nxt && cj >= 0 && cj < g.nyt) {
+      const int64_t c = (int64_t)ci * g.nyt + cj;
+      v = bcx(g, Bxf, ci, cj) * P.p[2][c] - bcy(g, Byf, ci, cj) * P.p[1][c];
+    }
+    L[e] = v;
+  }
+  __syncthreads();
+  const int ti = threadIdx.x / CT_TJ, tj = threadIdx.x % CT_TJ;
+  const int i = i0 + ti, j = j0 + tj;
+  if (i > g.nxt || j > g.nyt) return;
+  const int64_t t = (int64_t)i * (g.nyt + 1) + j;
   if (i == 0 || j == 0 || i == g.nxt || j == g.nyt) {
     Ez[t] = 0.0;
     return;
   }
-  const double* Bxf = S.p[6];
-  const double* Byf = S.p[7];
-  auto Ec = [&](int ci, int cj) {
-    const int64_t c = (int64_t)ci * g.nyt + cj;
-    return bcx(g, Bxf, ci, cj) * P.p[2][c] - bcy(g, Byf, ci, cj) * P.p[1][c];
-  };
   const int64_t yf = (int64_t)i * (g.nyt + 1) + j, yfm = (int64_t)(i - 1) * (g.nyt + 1) + j;
   const int64_t xf = (int64_t)i * g.nyt + j, xfm = (int64_t)i * g.nyt + j - 1;
   const double eyf = FyB[yf], eyfm = FyB[yfm];
   const double exf = -FxB[xf], exfm = -FxB[xfm];
-  const double NE = Ec(i, j), NW = Ec(i - 1, j), SE = Ec(i, j - 1), SW = Ec(i - 1, j - 1);
+  const double NE = L[(ti + 1) * CT_LJ + tj + 1], NW = L[ti * CT_LJ + tj + 1];
+  const double SE = L[(ti + 1) * CT_LJ + tj], SW = L[ti * CT_LJ + tj];
   const double avg = 0.25 * ((eyf + eyfm) + (exf + exfm));
   if (!upwind) {
     Ez[t] = avg;
@@ -1259,8 +1276,9 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   }
   const int64_t ncor = (int64_t)(g.nxt + 1) * (g.nyt + 1);
   tbegin(SRMHD_K_CORNER, st);
-  k_corner<<<blocks_for(ncor), BLK, 0, st>>>(g, s1, pr, w.Fx.p[0], w.Fx.p[6], w.Fy.p[0],
-                                             w.Fy.p[6], w.diag, sp->upwind, w.Ez);
+  const dim3 cgrid((g.nyt + 1 + CT_TJ - 1) / CT_TJ, (g.nxt + 1 + CT_TI - 1) / CT_TI);
+  k_corner<<<cgrid, BLK, 0, st>>>(g, s1, pr, w.Fx.p[0], w.Fx.p[6], w.Fy.p[0], w.Fy.p[6], w.diag,
+                                  sp->upwind, w.Ez);
   tend(SRMHD_K_CORNER, st);
   tbegin(SRMHD_K_UPDATE, st);
   k_update<<<blocks_for(ncor), BLK, 0, st>>>(g, s0, s1, w.Fx, w.Fy, w.Ez, a, b, c, dt_dev, o);
