@@ -2,16 +2,24 @@
 
 Workload (the headline configuration): the tapered relativistic magnetic
 rotor on [-0.5, 0.5]^2 at 2048^2, Gamma = 5/3, WENO5-Z + HLLD + upwind
-constrained transport + Kastaun con2prim + SSP-RK3, CFL 0.4, outflow BCs,
-float64 throughout.  A step is one full RK3 step including the CFL
-reduction; one zone-cycle is one physical cell advanced one step.
+constrained transport + Kastaun con2prim + SSP-RK3, CFL 0.4, outflow BCs.
+A step is one full RK3 step including the CFL reduction; one zone-cycle is
+one physical cell advanced one step.  Each precision form (float64, and the
+FP32-flux forms) is timed over K steps from the same initial state and then
+continued, untimed, to step 200, where its L1(rho, p, |B|) distance from the
+float64 form and div B are taken.  ``value`` is the fastest form whose
+L1(rho) there is <= 1e-6 (the float64 form always qualifies).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Prints ONE JSON line.  ``roofline`` is for the dominant kernel (measured
-live with HIP events around its launches inside the timed region);
-``cpu_baseline`` times the CPU oracle (torch, float64) on a bounded sample of
-the same problem.
+Prints ONE JSON line.  ``roofline`` is for the headline form's dominant
+kernel (measured live with HIP events around its launches inside the timed
+region); ``cpu_baseline`` times the CPU oracle (torch, float64) on a bounded
+sample of the same problem.  The same line carries the other two
+configurations as keys: ``tubes`` (4096 ragged Riemann tubes, MP5/HLLD/RK3,
+500 steps in one persistent launch) and ``adjoint`` (the gradient of the
+differentiable 512^2 Orszag-Tang rollout, 200 steps, checked against central
+differences), each with its own roofline and CPU baseline.
 """
 
 import argparse
@@ -89,26 +97,51 @@ def algorithmic_bytes(kernel, n, ng, precision="fp64"):
     raise ValueError(kernel)
 
 
-def cpu_baseline(n_sample=256, limiter="weno5z"):
-    """The oracle (torch CPU float64) on the same problem at n_sample^2: one
-    RK3 step after the initial sync, timed."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n_sample=256, limiter="weno5z", steps=2, threads=None):
+    """The oracle (torch CPU float64, the reference's algorithm and op order)
+    on the same problem at n_sample^2: one untimed RK3 step, then ``steps``
+    timed RK3 steps, at each thread count of ``threads`` (default: this
+    process's share, torch.get_num_threads(), and the reference's production
+    default of 8).  The fastest thread count is the reported value."""
     import torch
     import oracle.eos as OE
     import oracle.step2d as OS
-    threads = torch.get_num_threads()
+    share = torch.get_num_threads()
+    if threads is None:
+        threads = sorted({share, 8})
     eos = OE.ideal(5.0 / 3.0)
     g = OS.make_grid(n_sample, (-0.5, 0.5), limiter)
     P, Az = OS.rotor_tapered(g, eos)
-    st = OS.init_state(g, eos, P, Az)
-    dt = OS.compute_dt_2d(st.prims, g, eos, 0.4)
-    t0 = time.perf_counter()
-    OS.rk_step_ct(st, g, eos, dt, limiter=limiter)
-    wall = time.perf_counter() - t0
-    return {"value": n_sample * n_sample / wall, "unit": "zone-cycles/s", "cores": threads,
-            "kind": "port",
-            "sample": (f"CPU oracle (torch {torch.__version__} float64, {threads} threads): one RK3 "
-                       f"step of the same tapered rotor, WENO5-Z/HLLD/CT, at {n_sample}^2 "
-                       f"({wall:.1f} s)")}
+    st0 = OS.init_state(g, eos, P, Az)
+    dt = OS.compute_dt_2d(st0.prims, g, eos, 0.4)
+    runs = {}
+    for T in threads:
+        torch.set_num_threads(T)
+        st, _ = OS.rk_step_ct(st0, g, eos, dt, limiter=limiter)   # warm-up
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st, _ = OS.rk_step_ct(st, g, eos, dt, limiter=limiter)
+        wall = time.perf_counter() - t0
+        runs[T] = {"zone_cycles_per_s": n_sample * n_sample * steps / wall, "wall_s": wall}
+    torch.set_num_threads(share)
+    best = max(runs, key=lambda T: runs[T]["zone_cycles_per_s"])
+    return {"value": runs[best]["zone_cycles_per_s"], "unit": "zone-cycles/s", "cores": best,
+            "kind": "port", "by_threads": {str(T): r for T, r in runs.items()},
+            "host": {"cpu_model": _cpu_model(), "os_cpu_count": os.cpu_count(),
+                     "torch_threads_share": share, "torch": torch.__version__},
+            "sample": (f"CPU oracle (torch {torch.__version__} float64, OMP/MKL threads as set, "
+                       f"torch threads {threads}): the same tapered rotor, WENO5-Z/HLLD/CT, at "
+                       f"{n_sample}^2, one warm-up then {steps} timed RK3 steps per thread count")}
 
 
 def cpu_baseline_tubes(lengths, left, right, Bn, steps=2, limiter="mp5"):
@@ -133,7 +166,7 @@ def cpu_baseline_tubes(lengths, left, right, Bn, steps=2, limiter="mp5"):
 
 def bench_tubes(a):
     """4096 ragged Riemann tubes, MP5 + HLLD + SSP-RK3, per-tube dt, all
-    steps in one persistent graph-captured launch."""
+    steps in one persistent graph-captured launch.  Returns the result dict."""
     import torch
     from srmhd import eos as E, tubes as T
     dev = torch.device("cuda:0")
@@ -217,7 +250,7 @@ def bench_tubes(a):
             out["cpu_baseline"] = cpu_baseline_tubes(lengths, left, right, Bn, limiter=a.limiter)
         except Exception as exc:  # the baseline never hides the GPU result
             out["cpu_baseline"] = {"error": repr(exc)}
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def _rotor_form(a, precision, kernel, U_init, P_init):
@@ -294,11 +327,11 @@ def _rotor_l1_floor(a, U_init, P_init):
     U = [t.clone() for t in U_init]
     U[0] = torch.nextafter(U[0], torch.full_like(U[0], float("inf")))
     step.load(U, [t.clone() for t in P_init])
-    for _ in range(a.steps):
+    for _ in range(a.gate_at):
         step.compute_dt(step.U, step.P, a.cfl)
         step.step("rk3")
     torch.cuda.synchronize()
-    print(f"fp64 (D + 1 ulp): {a.steps} steps done", file=sys.stderr, flush=True)
+    print(f"fp64 (D + 1 ulp): {a.gate_at} steps done", file=sys.stderr, flush=True)
     return step
 
 
@@ -463,28 +496,153 @@ def bench_adjoint(a):
             out["cpu_baseline"] = cpu_baseline_adjoint()
         except Exception as exc:  # the baseline never hides the GPU result
             out["cpu_baseline"] = {"error": repr(exc)}
-    print(json.dumps(out), flush=True)
+    return out
+
+
+GATE_STEPS = 200   # the configuration's L1 / div B comparison point
+L1_GATE = 1e-6
+
+
+def _continue_to(a, step, done, total):
+    """Advance a form's stepper from ``done`` to ``total`` RK3 steps (untimed)."""
+    import torch
+    for _ in range(max(total - done, 0)):
+        step.compute_dt(step.U, step.P, a.cfl)
+        step.step("rk3")
+    torch.cuda.synchronize()
+
+
+def bench_rotor(a):
+    """The headline: the tapered rotor at a.n^2 in each precision form, timed
+    over a.steps RK3 steps; L1 distances and div B after GATE_STEPS steps."""
+    import torch
+    from srmhd import eos as E, grid as G, initial_data2d as I
+    from srmhd.reconstruction import ghosts_needed
+
+    dev = torch.device("cuda:0")
+    eos = E.hybrid_eos(0.0, 5.0 / 3.0, 5.0 / 3.0)
+    a.ng = ng = ghosts_needed(a.limiter)
+    a.grid = G.Grid2D(-0.5, 0.5, a.n, -0.5, 0.5, a.n, ng=ng, device=dev)
+    P, Az = I.rotor_tapered(a.grid, eos)
+    st = I.init_state(a.grid, eos, P, Az, limiter=a.limiter)
+    U_init, P_init = st.state_list(), st.prim_list()
+    del P, Az
+    forms = tuple(f.strip() for f in a.precision.split(",") if f.strip())
+    for f in forms:
+        if f not in DOMINANT_KERNEL:
+            raise SystemExit(f"unknown precision form {f!r}")
+    if "fp64" not in forms:
+        raise SystemExit("the fp64 form is the reference every other form is measured against")
+    res = {}
+    a.gate_at = max(GATE_STEPS, a.steps)
+    for prec in forms:
+        kernel = a.kernel or DOMINANT_KERNEL[prec]
+        res[prec] = _rotor_form(a, prec, kernel, U_init, P_init)
+        # the comparison point of the configuration, whatever was timed
+        _continue_to(a, res[prec]["step"], a.steps, a.gate_at)
+        print(f"{prec}: continued to {a.gate_at} steps", file=sys.stderr, flush=True)
+
+    if a.copy_calibration:
+        # an elementwise kernel of known traffic (torch routes copy_ through
+        # the DMA engines, which the counters do not see): 512 MiB read,
+        # 512 MiB written per launch
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+
+    fields = {p: _rotor_fields(r["step"], a.n, ng) for p, r in res.items()}
+    finite = all(bool(torch.isfinite(fields[p][0]).all().item()) for p in fields)
+    ra, pa, ba, _ = fields["fp64"]
+
+    def l1_of_fields(fb):
+        rb, pb, bb, _ = fb
+        return {"rho": _l1(rb, ra), "p": _l1(pb, pa), "absB": _l1(bb, ba)}
+
+    l1 = {f: l1_of_fields(fields[f]) for f in forms if f != "fp64"}
+    meets = {f: l1[f]["rho"] <= L1_GATE for f in l1}
+    # the headline is the fastest form that meets the gate; FP64 always does
+    head = max([f for f in forms if f == "fp64" or meets[f]], key=lambda f: res[f]["zc"])
+    h = res[head]
+    bytes_per_zc = 752.0  # RK3 2D algorithmic bytes per zone-cycle
+    form_desc = {"fp64": "float64 throughout (plain form, bit-faithful to the reference)",
+                 "mixed": "reconstruction + Riemann-solver root finding in FP32; HLLD flux "
+                          "assembly, con2prim, CT and RK in FP64",
+                 "mixed_r64": "Riemann-solver root finding in FP32; reconstruction, HLLD flux "
+                              "assembly, con2prim, CT and RK in FP64",
+                 "fp64_local": "float64 throughout; every con2prim solved per element to its "
+                               "own convergence instead of batch-coupled"}
+    f64 = res["fp64"]
+    out = {
+        "metric": "zone-cycles/s, 2D rotor 2048^2 (FP64 vs FP32-flux @ L1<=1e-6); HBM GB/s",
+        "value": h["zc"], "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head in ("fp64", "fp64_local") else "f32+f64 (mixed)",
+        "data": "synthetic",
+        "config": {"workload": f"tapered relativistic rotor {a.n}^2, {a.limiter.upper()} + "
+                               f"{a.solver.upper()} + CT + Kastaun C2P + SSP-RK3, CFL {a.cfl}, "
+                               f"{a.steps} timed steps, {form_desc[head]}",
+                   "n": a.n, "ng": ng, "limiter": a.limiter, "solver": a.solver, "scheme": "rk3",
+                   "precision": head},
+        "headline_form": head,
+        "headline_rule": (f"the fastest form whose L1(rho) vs the FP64 form after {a.gate_at} "
+                          f"steps is <= {L1_GATE:g} (the FP64 form itself qualifies)"),
+        "roofline": h["roofline"],
+        "fp64_valu": h["fp64_valu"],
+        "step_hbm": {"bytes_per_zone_cycle": bytes_per_zc,
+                     "achieved_GBs": h["zc"] * bytes_per_zc / 1e9,
+                     "frac": h["zc"] * bytes_per_zc / 1e9 / HBM_PEAK_GBS},
+        "gate_steps": a.gate_at,
+        "div_b": fields[head][3],
+        "state_finite": finite,
+        "fp64_form": {"value": f64["zc"], "ms_per_step": f64["ms_per_step"],
+                      "roofline": f64["roofline"], "fp64_valu": f64["fp64_valu"],
+                      "div_b": fields["fp64"][3]},
+        "other_forms": {
+            form: {"value": res[form]["zc"], "ms_per_step": res[form]["ms_per_step"],
+                   "speedup_vs_fp64": res[form]["zc"] / f64["zc"],
+                   f"l1_vs_fp64_after_{a.gate_at}_steps": l1[form],
+                   "l1_rho_gate_1e-6_met": meets[form], "div_b": fields[form][3],
+                   "roofline": res[form]["roofline"], "description": form_desc[form]}
+            for form in forms if form != "fp64"},
+    }
+    if not a.no_floor:
+        fl = _rotor_l1_floor(a, U_init, P_init)
+        out["l1_floor"] = {
+            f"l1_vs_fp64_after_{a.gate_at}_steps": l1_of_fields(_rotor_fields(fl, a.n, ng)),
+            "description": "the FP64 form from the initial state with D raised by one ulp in "
+                           "every cell: the distance round-off alone reaches on this problem"}
+        del fl
+    for r in res.values():
+        del r["step"]
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="rotor", choices=("rotor", "tubes", "adjoint"),
-                    help="rotor: the 2048^2 rotor (headline); tubes: 4096 batched tubes; "
-                         "adjoint: the differentiable 512^2 Orszag-Tang rollout")
+    ap.add_argument("--workload", default="all", choices=("all", "rotor", "tubes", "adjoint"),
+                    help="all (default): the 2048^2 rotor headline, with the batched tubes and "
+                         "the differentiable Orszag-Tang rollout as keys of the same line; "
+                         "rotor / tubes / adjoint: one workload alone")
     ap.add_argument("--checkpoint", type=int, default=20,
                     help="adjoint workload: steps between checkpoints")
     ap.add_argument("--precision", default="fp64,mixed,mixed_r64",
-                    help="comma-separated rotor forms to time (fp64, mixed, mixed_r64); the "
-                         "first mixed form is the headline value, the others sit beside it "
-                         "with their L1 distance from FP64 after the timed steps")
+                    help="comma-separated rotor forms to time (fp64 required; mixed, "
+                         "mixed_r64, fp64_local); each is compared with fp64 after 200 steps")
     ap.add_argument("--tubes", type=int, default=4096)
-    ap.add_argument("--loop-tubes", type=int, default=16,
-                    help="tubes timed one launch each for the per-tube-loop comparison")
-    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--loop-tubes", type=int, default=None,
+                    help="tubes timed one launch each for the per-tube-loop comparison "
+                         "(default 16 alone, 8 with --workload all)")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps of the workload (rotor default 200; with --workload "
+                         "all the tubes and adjoint keys keep their configurations' 500 and 200)")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--limiter", default="weno5z")
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--limiter", default=None)
     ap.add_argument("--solver", default="hlld")
     ap.add_argument("--cfl", type=float, default=0.4)
     ap.add_argument("--kernel", default=None,
@@ -504,123 +662,34 @@ def main():
               "shard as x-slabs with halo exchange; only the one-GPU kernel is built)",
               file=sys.stderr)
         sys.exit(2)
+
+    def sub(workload, steps, n, limiter, loop_tubes):
+        b = argparse.Namespace(**vars(a))
+        b.steps = steps
+        b.n = n
+        b.limiter = limiter
+        b.loop_tubes = loop_tubes
+        return b
+
+    lim = a.limiter
     if a.workload == "tubes":
-        if a.steps is None:
-            a.steps = 500
-        if a.limiter == "weno5z":
-            a.limiter = "mp5"
-        bench_tubes(a)
-        return
-    if a.workload == "adjoint":
-        if a.steps is None:
-            a.steps = 200
-        if a.n == 2048:
-            a.n = 512
-        bench_adjoint(a)
-        return
-    if a.steps is None:
-        a.steps = 200
-
-    import torch
-    from srmhd import eos as E, grid as G, initial_data2d as I
-    from srmhd.reconstruction import ghosts_needed
-
-    dev = torch.device("cuda:0")
-    eos = E.hybrid_eos(0.0, 5.0 / 3.0, 5.0 / 3.0)
-    a.ng = ng = ghosts_needed(a.limiter)
-    a.grid = G.Grid2D(-0.5, 0.5, a.n, -0.5, 0.5, a.n, ng=ng, device=dev)
-    P, Az = I.rotor_tapered(a.grid, eos)
-    st = I.init_state(a.grid, eos, P, Az, limiter=a.limiter)
-    U_init, P_init = st.state_list(), st.prim_list()
-    del P, Az
-    forms = tuple(f.strip() for f in a.precision.split(",") if f.strip())
-    for f in forms:
-        if f not in DOMINANT_KERNEL:
-            raise SystemExit(f"unknown precision form {f!r}")
-    res = {}
-    for prec in forms:
-        kernel = a.kernel or DOMINANT_KERNEL[prec]
-        res[prec] = _rotor_form(a, prec, kernel, U_init, P_init)
-
-    if a.copy_calibration:
-        # an elementwise kernel of known traffic (torch routes copy_ through
-        # the DMA engines, which the counters do not see): 512 MiB read,
-        # 512 MiB written per launch
-        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.mul(src, 1.0, out=dst)
-        torch.cuda.synchronize()
-        del src, dst
-
-    mixed_forms = [f for f in forms if f != "fp64"]
-    head = mixed_forms[0] if mixed_forms else "fp64"
-    h = res[head]
-    fields = {p: _rotor_fields(r["step"], a.n, ng) for p, r in res.items()}
-    finite = all(bool(torch.isfinite(fields[p][0]).all().item()) for p in fields)
-    bytes_per_zc = 752.0  # RK3 2D algorithmic bytes per zone-cycle
-    form_desc = {"fp64": "float64 throughout (plain form)",
-                 "mixed": "reconstruction + Riemann-solver root finding in FP32; HLLD flux "
-                          "assembly, con2prim, CT and RK in FP64",
-                 "mixed_r64": "Riemann-solver root finding in FP32; reconstruction, HLLD flux "
-                              "assembly, con2prim, CT and RK in FP64",
-                 "fp64_local": "float64 throughout; every con2prim solved per element to its "
-                               "own convergence instead of batch-coupled"}
-    out = {
-        "metric": "zone-cycles/s, 2D rotor 2048^2 (FP64 vs FP32-flux @ L1<=1e-6); HBM GB/s",
-        "value": h["zc"], "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": h["ms_per_step"], "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64 (mixed)" if head != "fp64" else "f64", "data": "synthetic",
-        "config": {"workload": f"tapered relativistic rotor {a.n}^2, {a.limiter.upper()} + "
-                               f"{a.solver.upper()} + CT + Kastaun C2P + SSP-RK3, CFL {a.cfl}, "
-                               f"{a.steps} steps, {form_desc[head]}",
-                   "n": a.n, "ng": ng, "limiter": a.limiter, "solver": a.solver, "scheme": "rk3",
-                   "precision": head},
-        "roofline": h["roofline"],
-        "fp64_valu": h["fp64_valu"],
-        "step_hbm": {"bytes_per_zone_cycle": bytes_per_zc,
-                     "achieved_GBs": h["zc"] * bytes_per_zc / 1e9,
-                     "frac": h["zc"] * bytes_per_zc / 1e9 / HBM_PEAK_GBS},
-        "div_b": fields[head][3],
-        "state_finite": finite,
-    }
-    if "fp64" in res and head != "fp64":
-        f = res["fp64"]
-        ra, pa, ba, _ = fields["fp64"]
-
-        def l1_of_fields(fb):
-            rb, pb, bb, _ = fb
-            return {"rho": _l1(rb, ra), "p": _l1(pb, pa), "absB": _l1(bb, ba)}
-
-        def l1_of(form):
-            return l1_of_fields(fields[form])
-
-        l1 = l1_of(head)
-        out["fp64_form"] = {"value": f["zc"], "ms_per_step": f["ms_per_step"],
-                            "roofline": f["roofline"], "fp64_valu": f["fp64_valu"],
-                            "div_b": fields["fp64"][3]}
-        out["speedup_vs_fp64"] = h["zc"] / f["zc"]
-        out["l1_vs_fp64"] = l1
-        out["l1_rho_gate_1e-6_met"] = l1["rho"] <= 1e-6
-        if not a.no_floor:
-            fl = _rotor_l1_floor(a, U_init, P_init)
-            out["l1_floor"] = {
-                "l1_vs_fp64": l1_of_fields(_rotor_fields(fl, a.n, ng)),
-                "description": "the FP64 form from the initial state with D raised by one ulp "
-                               "in every cell, after the same steps: the distance round-off "
-                               "alone reaches on this problem"}
-            del fl
-        out["other_forms"] = {
-            form: {"value": res[form]["zc"], "ms_per_step": res[form]["ms_per_step"],
-                   "speedup_vs_fp64": res[form]["zc"] / f["zc"], "l1_vs_fp64": l1_of(form),
-                   "div_b": fields[form][3], "description": form_desc[form]}
-            for form in mixed_forms[1:]}
-    if not a.no_cpu:
-        try:
-            out["cpu_baseline"] = cpu_baseline(a.cpu_sample, a.limiter)
-        except Exception as exc:  # the baseline never hides the GPU result
-            out["cpu_baseline"] = {"error": repr(exc)}
+        out = bench_tubes(sub("tubes", a.steps or 500, a.n, lim or "mp5", a.loop_tubes or 16))
+    elif a.workload == "adjoint":
+        out = bench_adjoint(sub("adjoint", a.steps or 200, a.n or 512, lim, 0))
+    else:
+        out = bench_rotor(sub("rotor", a.steps or GATE_STEPS, a.n or 2048, lim or "weno5z", 0))
+        if not a.no_cpu:
+            try:
+                out["cpu_baseline"] = cpu_baseline(a.cpu_sample, lim or "weno5z")
+            except Exception as exc:  # the baseline never hides the GPU result
+                out["cpu_baseline"] = {"error": repr(exc)}
+        if a.workload == "all":
+            # the other two configurations, each with its own roofline and CPU
+            # baseline, at their own step counts
+            out["tubes"] = bench_tubes(sub("tubes", 500, None, "mp5", a.loop_tubes or 8))
+            print("tubes done", file=sys.stderr, flush=True)
+            out["adjoint"] = bench_adjoint(sub("adjoint", 200, 512, None, 0))
+            print("adjoint done", file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 

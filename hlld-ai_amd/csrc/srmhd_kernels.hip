@@ -810,8 +810,8 @@ AuxStream* aux_stream() {
   return a.s ? &a : nullptr;
 }
 
-void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
-                  const srmhd_eos& e, const Ws2& w, hipStream_t s) {
+int sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
+                 const srmhd_eos& e, const Ws2& w, hipStream_t s) {
   // the C2PRec areas are unused in this form: they hold the fix-up lists
   // (their counts, the first word of each, are cleared by the stage's zeroing launch)
   unsigned* fx = (unsigned*)w.recx;
@@ -822,6 +822,9 @@ void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const P
   tbegin(SRMHD_K_SWEEP_X_B, s);
   sweep_mixed(0, lim, sol, r64, g, S, P, w.cf, e, shx, w.Fx, fx, w.emf_hll, s);
   tend(SRMHD_K_SWEEP_X_B, s);
+  // a launch error so far is reported now: creating the auxiliary stream
+  // may clear the error state when it fails
+  if (const int rc = launch_status()) return rc;
   // the x fix-up reads the primitives and the state and writes only Fx: it
   // runs beside the y sweep, joined before anything reads Fx
   std::lock_guard<std::mutex> lock(g_aux_mu);
@@ -841,6 +844,7 @@ void sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const P
     (void)hipStreamSynchronize(ax->s);   // no join event: wait on the host
   }
   k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
+  return 0;
 }
 
 void sweeps_dispatch(int lim, int sol, bool local, const G2& g, const St2& S, const Pr2& P,
@@ -1260,8 +1264,9 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   }
   const bool local = sp->precision != SRMHD_PREC_FP64;   // per-element con2prim
   if (mixed) {
-    sweeps_mixed(sp->limiter, sp->solver, sp->precision == SRMHD_PREC_MIXED_R64, g, s1, pr, *eos,
-                 w, st);
+    const int rc = sweeps_mixed(sp->limiter, sp->solver, sp->precision == SRMHD_PREC_MIXED_R64,
+                                g, s1, pr, *eos, w, st);
+    if (rc) return rc;
   } else {
     sweeps_dispatch(sp->limiter, sp->solver, local, g, s1, pr, *eos, w, st);
   }

@@ -178,14 +178,17 @@ def workspace(nbytes, device):
     return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
 
 
-def diag_to_dict(words, solver, idir):
+def diag_to_dict(words, solver, idir, iters_counted=True, pstar_method="secant"):
     """FluxDiag words (a host int64 tensor) -> the reference's LAST_DIAG keys
-    (hlld.py:1516-1529)."""
+    (hlld.py:1516-1529).  iters_counted False (the FP32-flux forms, whose
+    root finder is not the reference's): mean_iters is NaN and max_iters -1,
+    and the fallback counts are those of the FP32 solver's decisions."""
     w = [int(x) & 0xFFFFFFFFFFFFFFFF for x in words.tolist()]
     n, nnc, nwo, nbz, nfb, si, ncv, mx = w[:8]
+    mean = (si / ncv) if (ncv and iters_counted) else float("nan")
     return dict(solver=solver, idir=idir, n_interfaces=n, n_not_converged=nnc,
                 n_wave_order_bad=nwo, n_bn_zero=nbz, n_hlle_fallback=nfb,
                 frac_hlle_fallback=nfb / max(n, 1),
-                mean_iters=(si / ncv) if ncv else float("nan"),
-                pstar_method="secant", pstar_cost=None, max_iters=mx,
+                mean_iters=mean, pstar_method=pstar_method, pstar_cost=None,
+                max_iters=mx if iters_counted else -1,
                 n_c2p_redo=w[10] if len(w) > 10 else 0)

@@ -121,8 +121,11 @@ class Stepper2D:
     def diag_dicts(self):
         dx, dy = _LazyDiag(), _LazyDiag()
         w = self.diag.clone()
-        dx._set_pending(w[:_lib.DIAG_WORDS], self.solver, 0)
-        dy._set_pending(w[_lib.DIAG_WORDS:], self.solver, 1)
+        # the FP32-flux forms count their own solver's decisions, not the
+        # reference's secant iterations
+        counted = self.precision in ("fp64", "fp64_local")
+        dx._set_pending(w[:_lib.DIAG_WORDS], self.solver, 0, iters_counted=counted)
+        dy._set_pending(w[_lib.DIAG_WORDS:], self.solver, 1, iters_counted=counted)
         return {"x": {"diag": dx}, "y": {"diag": dy}}
 
     # ── resident step loop ─────────────────────────────────────────────────
@@ -251,7 +254,8 @@ def rk_step_ct(state, grid, eos, dt, *, scheme="rk3", bc_x="outflow", bc_y="outf
     S = State2D({k: U[i] for i, k in enumerate(EVOLVED_KEYS)}, U[6], U[7])
     _finish_prims(S, P, grid, bc_x, bc_y)
     diag = st.diag_dicts()
-    LAST_DIAG._set_pending(st.diag.clone()[_lib.DIAG_WORDS:], st.solver, 1)
+    LAST_DIAG._set_pending(st.diag.clone()[_lib.DIAG_WORDS:], st.solver, 1,
+                           iters_counted=st.precision in ("fp64", "fp64_local"))
     return S, diag
 
 

@@ -29,15 +29,15 @@ class _LazyDiag(dict):
         super().__init__()
         self._pending = None
 
-    def _set_pending(self, words, solver, idir):
-        self._pending = (words, solver, idir)
+    def _set_pending(self, words, solver, idir, **kw):
+        self._pending = (words, solver, idir, kw)
 
     def _fill(self):
         if self._pending is not None:
-            words, solver, idir = self._pending
+            words, solver, idir, kw = self._pending
             self._pending = None
             dict.clear(self)
-            dict.update(self, _lib.diag_to_dict(words.cpu(), solver, idir))
+            dict.update(self, _lib.diag_to_dict(words.cpu(), solver, idir, **kw))
 
     def __getitem__(self, k):
         self._fill()

@@ -4,8 +4,15 @@
  * Every entry point takes plain device pointers (float64 structure-of-arrays,
  * contiguous), sizes, and a HIP stream (hipStream_t passed as void*; NULL =
  * the null stream).  The caller owns every buffer, including the workspace
- * whose size the matching *_workspace_bytes() returns; nothing is allocated
- * inside.  All calls are asynchronous on the stream.  Return value: 0 on
+ * whose size the matching *_workspace_bytes() returns; no device memory is
+ * allocated inside.  One exception holds host-side HIP objects: the first
+ * mixed-precision srmhd2d_stage (SRMHD_PREC_MIXED / _MIXED_R64, HLLD) on a
+ * device creates one non-blocking stream and two events for that device
+ * (default priority), kept until the process ends, to run the FP64 fix-up
+ * of the x faces beside the y sweep; if their creation fails the stage runs
+ * on the caller's stream alone.  All calls are asynchronous on the stream
+ * (the mixed stage's second stream is forked from and joined back into it
+ * with events).  Return value: 0 on
  * success, SRMHD_EARG for a bad argument, otherwise SRMHD_EHIP + the
  * hipError_t of the failing launch.  Calls that share a workspace must not
  * overlap.

@@ -15,6 +15,15 @@ from .physics import (BK, CONS_KEYS, SK, VEL, TINY, b_squared, energy_form,
 LAST_DIAG: dict = {}
 PSTAR_TOL = 1.0e-10       # hlld.py:1080
 PSTAR_MAXIT = 30          # hlld.py:1327
+# The reference's call-time switches of hlld_flux (module globals, read on
+# every call): the p* root finder (hlld.py:1088, RMHD_PSTAR_METHOD), the
+# bracket source (hlld.py:1065) and the relative finite-difference step of
+# the newton_fd / newton_fd1 derivatives (hlld.py:1091-1092, None = the
+# method's own default).
+PSTAR_METHOD = "secant"
+PSTAR_BRACKET = "legacy"
+PSTAR_FD_H = None
+PSTAR_METHODS = ("secant", "newton", "newton_fd", "newton_fd1", "newton_safe")
 
 
 def _p_hll_sides(sL, sR, eos):
@@ -166,22 +175,25 @@ class StarStates:
         v2L = sdiv(QL * RS2L + RB2L * (CL + BnL * ((-cm) * RSnL - RTL)), XL)
         v2R = sdiv(QR * RS2R + RB2R * (CR + BnR * (cx * RSnR - RTR)), XR)
 
-        dL_, dR_ = -cm - vaL, cx - vaR
-        B1L = sdiv(RB1L - BnL * v1L, dL_)
-        B1R = sdiv(RB1R - BnR * v1R, dR_)
-        B2L = sdiv(RB2L - BnL * v2L, dL_)
-        B2R = sdiv(RB2R - BnR * v2R, dR_)
+        # the Alfven denominators -cm - va, cx - va are formed afresh at every
+        # use, as the reference writes them: the values are the same either
+        # way, but torch autograd (the exact-derivative Newton p* root)
+        # accumulates their gradients use by use
+        B1L = sdiv(RB1L - BnL * v1L, -cm - vaL)
+        B1R = sdiv(RB1R - BnR * v1R, cx - vaR)
+        B2L = sdiv(RB2L - BnL * v2L, -cm - vaL)
+        B2R = sdiv(RB2R - BnR * v2R, cx - vaR)
 
         vRSL = vaL * RSnL + v1L * RS1L + v2L * RS2L
         vRSR = vaR * RSnR + v1R * RS1R + v2R * RS2R
-        waL = p + sdiv(RTL - vRSL, dL_)
-        waR = p + sdiv(RTR - vRSR, dR_)
-        DaL = sdiv(self.RDL, dL_)
-        DaR = sdiv(self.RDR, dR_)
+        waL = p + sdiv(RTL - vRSL, -cm - vaL)
+        waR = p + sdiv(RTR - vRSR, cx - vaR)
+        DaL = sdiv(self.RDL, -cm - vaL)
+        DaR = sdiv(self.RDR, cx - vaR)
         vBL = vaL * BnL + v1L * B1L + v2L * B2L
         vBR = vaR * BnR + v1R * B1R + v2R * B2R
-        TaL = sdiv(RTL + p * vaL - vBL * BnL, dL_) - DaL
-        TaR = sdiv(RTR + p * vaR - vBR * BnR, dR_) - DaR
+        TaL = sdiv(RTL + p * vaL - vBL * BnL, -cm - vaL) - DaL
+        TaR = sdiv(RTR + p * vaR - vBR * BnR, cx - vaR) - DaR
         SaL = (TaL + DaL + p) * vaL - vBL * BnL
         SaR = (TaR + DaR + p) * vaR - vBR * BnR
         S1L = (TaL + DaL + p) * v1L - vBL * B1L
@@ -257,6 +269,137 @@ def clamped_secant(func, x0, x1, tol, max_iter):
     return x1, err, used
 
 
+def newton_pstar(func, x0, tol=1e-12, max_iter=30, derivative="ad", fd_h=1e-6,
+                 max_halvings=6, bracket=None):
+    """Batched safeguarded Newton for p* (hlld.py:751-904): (root, err,
+    iters_used, cost).  derivative "ad": exact f' by reverse-mode autograd
+    through the residual; "fd": central difference, two extra residuals;
+    "fd1": forward difference from f(p).  Lanes with a non-finite or
+    vanishing derivative freeze; a step that does not reduce |f| is halved up
+    to max_halvings times (unguarded lanes only); with a sign-changing
+    bracket the step that would leave it bisects instead and the bracket
+    shrinks around the sign change.  Converged lanes freeze."""
+    x = x0.detach().clone()
+    cost = dict(fevals=0, backward=0, halvings=0)
+
+    def f_only(p):
+        cost["fevals"] += 1
+        with torch.no_grad():
+            return func(p)
+
+    def f_and_df(p):
+        if derivative == "ad":
+            q = p.detach().clone().requires_grad_(True)
+            with torch.enable_grad():
+                f = func(q)
+                g = (torch.autograd.grad(f.sum(), q, allow_unused=True)[0]
+                     if f.requires_grad else None)
+            cost["fevals"] += 1
+            cost["backward"] += 1
+            if g is None:
+                g = torch.zeros_like(q)
+            return f.detach(), g.detach()
+        h = fd_h * p.abs().clamp(min=1e-30)
+        if derivative == "fd1":
+            f0 = f_only(p)
+            ph = p + h
+            return f0, (f_only(ph) - f0) / (ph - p)
+        fp, fm = f_only(p + h), f_only((p - h).clamp(min=1e-30))
+        return f_only(p), (fp - fm) / (2.0 * h)
+
+    f, df = f_and_df(x)
+    n = x.shape
+    if bracket is not None:
+        lo, hi, flo, fhi = (b.detach().clone().reshape(n) for b in bracket)
+        lo, hi, flo, fhi = (torch.where(lo <= hi, lo, hi), torch.where(lo <= hi, hi, lo),
+                            torch.where(lo <= hi, flo, fhi), torch.where(lo <= hi, fhi, flo))
+        guarded = (flo * fhi) < 0
+    else:
+        guarded = torch.zeros(n, dtype=torch.bool)
+    err = torch.full(n, max_iter, dtype=torch.int32)
+    iters_used = torch.full(n, max_iter, dtype=torch.int32)
+    done = torch.zeros(n, dtype=torch.bool)
+    dead = torch.zeros(n, dtype=torch.bool)
+    it = 0
+    for _ in range(max_iter):
+        it += 1
+        conv = f.abs() < tol
+        newly = conv & ~done
+        iters_used = torch.where(newly, torch.full_like(iters_used, it - 1), iters_used)
+        done = done | conv
+        dead = dead | ~torch.isfinite(df) | (df.abs() <= 1e-300) | ~torch.isfinite(f)
+        active = ~done & ~dead
+        if not bool(active.any()):
+            break
+        step = torch.where(active, f / torch.where(active, df, torch.ones_like(df)),
+                           torch.zeros_like(f))
+        lam = torch.ones_like(x)
+        x_new = torch.clamp(x - step, min=1e-30)
+        if bracket is not None:
+            out = guarded & ((x_new <= lo) | (x_new >= hi))
+            x_new = torch.where(out, 0.5 * (lo + hi), x_new)
+        f_new = f_only(x_new)
+        worse = active & ~guarded & ~(f_new.abs() < f.abs())
+        for _h in range(max_halvings):
+            if not bool(worse.any()):
+                break
+            cost["halvings"] += 1
+            lam = torch.where(worse, lam * 0.5, lam)
+            x_try = torch.clamp(x - lam * step, min=1e-30)
+            f_try = f_only(x_try)
+            x_new = torch.where(worse, x_try, x_new)
+            f_new = torch.where(worse, f_try, f_new)
+            worse = worse & ~(f_new.abs() < f.abs())
+        x = torch.where(active, x_new, x)
+        if bracket is not None:
+            left = guarded & active & ((f_new * flo) > 0)
+            right = guarded & active & ~left
+            lo = torch.where(left, x_new, lo)
+            flo = torch.where(left, f_new, flo)
+            hi = torch.where(right, x_new, hi)
+            fhi = torch.where(right, f_new, fhi)
+        f_next, df_next = f_and_df(x)
+        f = torch.where(active, f_next, f)
+        df = torch.where(active, df_next, df)
+    conv = f.abs() < tol
+    err = torch.where(conv, torch.zeros_like(err), err)
+    iters_used = torch.where(conv & ~done, torch.full_like(iters_used, it), iters_used)
+    return x, err, iters_used, cost
+
+
+def hllc_pstar(fH, uH, idir=0):
+    """The HLLC star total pressure from the HLL flux and state, the "hllc"
+    bracket source of hlld_flux (hlld.py:1104-1199); returns (p*, Bt1, Bt2)."""
+    n, t1, t2 = idir, (idir + 1) % 3, (idir + 2) % 3
+    Bn = uH[BK[n]]
+    hasB = Bn.abs() > 1e-15
+    Bt1, Bt2 = uH[BK[t1]], uH[BK[t2]]
+    fB1, fB2 = fH[BK[t1]], fH[BK[t2]]
+    z = torch.zeros_like(Bn)
+    cross = torch.where(hasB, Bt1 * fB1 + Bt2 * fB2, z)
+    bt2 = torch.where(hasB, Bt1 ** 2 + Bt2 ** 2, z)
+    fbt2 = torch.where(hasB, fB1 ** 2 + fB2 ** 2, z)
+    E = uH["tau"] + uH["D"]
+    fE = fH["tau"] + fH["D"]
+    qa = fE - cross
+    qb = -fH[SK[n]] - E + bt2 + fbt2
+    qc = uH[SK[n]] - cross
+    disc = sqrt(torch.clamp(qb ** 2 - 4.0 * qa * qc, min=0.0))
+    vq = sdiv(-0.5 * (qb + disc), qa)
+    vl = sdiv(-qc, qb)
+    vc = torch.where(qa.abs() < 1e-15, torch.where(qb.abs() < 1e-15, torch.zeros_like(qa), vl),
+                     vq)
+    vt1 = torch.where(hasB, sdiv(Bt1 * vc - fB1, Bn), torch.zeros_like(vc))
+    vt2 = torch.where(hasB, sdiv(Bt2 * vc - fB2, Bn), torch.zeros_like(vc))
+    v2 = vc ** 2 + vt1 ** 2 + vt2 ** 2
+    g = 1.0 / sqrt(torch.clamp(1.0 - v2, min=1e-10))
+    vB = torch.where(hasB, vc * Bn + vt1 * Bt1 + vt2 * Bt2, torch.zeros_like(vc))
+    pB = fH[SK[n]] + (Bn / g) ** 2 - vc * (fE - Bn * vB)
+    p0 = fH[SK[n]] - vc * fE
+    ps = torch.clamp(torch.where(hasB, pB, p0), min=1e-30)
+    return ps, Bt1, Bt2
+
+
 def hlld_flux(sL, sR, eos, idir=0):
     """Mignone-Ugliano-Bodo relativistic HLLD as the reference implements it
     (hlld.py:1202-1543): legacy p_hll bracket from a full con2prim of the HLL
@@ -269,12 +412,17 @@ def hlld_flux(sL, sR, eos, idir=0):
     EL, GL = energy_form(uL, fL)
     ER, GR = energy_form(uR, fR)
 
-    # bracket source: total pressure of the HLL state   hlld.py:1253-1263
-    ph = cons_to_prim(uH, eos)
-    p_hll, _ = eos.press_and_cs2(ph["eps"], ph["rho"])
-    wh, _ = lorentz(ph)
-    p_hll = p_hll + 0.5 * b_squared((ph["vx"], ph["vy"], ph["vz"]),
-                                     (ph["Bx"], ph["By"], ph["Bz"]), wh)
+    if PSTAR_BRACKET == "hllc":
+        # the HLLC star pressure of the HLL state   hlld.py:1246-1251
+        p_hll, _, _ = hllc_pstar(fH, uH, idir)
+        p_hll = torch.clamp(p_hll, min=1e-30)
+    else:
+        # bracket source: total pressure of the HLL state   hlld.py:1253-1263
+        ph = cons_to_prim(uH, eos)
+        p_hll, _ = eos.press_and_cs2(ph["eps"], ph["rho"])
+        wh, _ = lorentz(ph)
+        p_hll = p_hll + 0.5 * b_squared((ph["vx"], ph["vy"], ph["vz"]),
+                                         (ph["Bx"], ph["By"], ph["Bz"]), wh)
 
     # p0 from the Bn = 0 quadratic   hlld.py:1265-1287
     a = fH["tau"] + fH["D"]
@@ -304,7 +452,19 @@ def hlld_flux(sL, sR, eos, idir=0):
         p_hi = torch.where(bad, p_hi * 2.0, p_hi)
         f_lo = torch.where(bad, calc.residual(p_lo), f_lo)
         f_hi = torch.where(bad, calc.residual(p_hi), f_hi)
-    ps, err, used = clamped_secant(calc.residual, p_lo, p_hi, PSTAR_TOL, PSTAR_MAXIT)
+    method, cost = PSTAR_METHOD, None
+    if method == "secant":
+        ps, err, used = clamped_secant(calc.residual, p_lo, p_hi, PSTAR_TOL, PSTAR_MAXIT)
+    elif method in PSTAR_METHODS:
+        # one start: the bracket end with the smaller residual   hlld.py:1330-1343
+        start = torch.where(f_lo.abs() <= f_hi.abs(), p_lo, p_hi)
+        deriv = {"newton_fd": "fd", "newton_fd1": "fd1"}.get(method, "ad")
+        fd_h = PSTAR_FD_H or (1.5e-8 if deriv == "fd1" else 1e-6)
+        ps, err, used, cost = newton_pstar(
+            calc.residual, start, tol=PSTAR_TOL, max_iter=PSTAR_MAXIT, derivative=deriv,
+            fd_h=fd_h, bracket=((p_lo, p_hi, f_lo, f_hi) if method == "newton_safe" else None))
+    else:
+        raise ValueError("PSTAR_METHOD=%r: expected one of %s" % (method, PSTAR_METHODS))
     calc.solve(ps)
 
     t1, t2 = calc.t1, calc.t2
@@ -359,7 +519,7 @@ def hlld_flux(sL, sR, eos, idir=0):
         n_bn_zero=int(bn_zero.sum()), n_hlle_fallback=int(failed.sum()),
         frac_hlle_fallback=float(failed.sum()) / max(nf, 1),
         mean_iters=(float(used[conv].double().mean()) if bool(conv.any()) else float("nan")),
-        pstar_method="secant", pstar_cost=None, max_iters=int(used.max()) if nf else 0)
+        pstar_method=method, pstar_cost=cost, max_iters=int(used.max()) if nf else 0)
     ps = torch.where(failed, -torch.abs(p_hll) - 1e-10, ps)
     for k in CONS_KEYS:
         F[k] = torch.where(failed, fH[k], F[k])

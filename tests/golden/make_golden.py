@@ -100,6 +100,43 @@ def gen_flux():
     np.savez_compressed(HERE / "flux_random.npz", **out)
 
 
+def gen_pstar():
+    """hlld_flux with the non-default p* root finders and bracket
+    (hlld.py:751-905, 1046-1098, 1104-1199, 1323-1343), selected through the
+    module globals the reference reads at call time, on the flux_random
+    interface pairs."""
+    g = np.load(HERE / "flux_random.npz")
+    eos = hybrid_eos(K=0.0, gamma=5.0 / 3.0, gamma_th=5.0 / 3.0)
+    variants = [("newton", "legacy", 1e-10), ("newton_fd", "legacy", 1e-10),
+                ("newton_fd1", "legacy", 1e-10), ("newton_safe", "legacy", 1e-10),
+                ("secant", "hllc", 1e-10), ("newton_safe", "hllc", 1e-10),
+                ("secant", "legacy", 1e-13)]
+    out = {}
+    saved = (R_hlld._PSTAR_METHOD, R_hlld._PSTAR_BRACKET, R_hlld._PSTAR_TOL)
+    try:
+        for method, bracket, tol in variants:
+            R_hlld._PSTAR_METHOD, R_hlld._PSTAR_BRACKET, R_hlld._PSTAR_TOL = method, bracket, tol
+            for tag in ("a", "b"):
+                sL = {k: torch.from_numpy(g[f"{tag}_L_{k}"].copy()) for k in PRIM9}
+                sR = {k: torch.from_numpy(g[f"{tag}_R_{k}"].copy()) for k in PRIM9}
+                for idir in (0, 1, 2):
+                    F, U, ps = R_hlld.hlld_flux(sL, sR, eos, idir=idir)
+                    pre = f"{method}_{bracket}_{tol:g}_{tag}{idir}"
+                    for k in CONS8:
+                        out[f"{pre}_F_{k}"] = _np(F[k])
+                        if idir == 0:        # the state comes from the same assembly
+                            out[f"{pre}_U_{k}"] = _np(U[k])
+                    out[f"{pre}_pstar"] = _np(ps)
+                    d = R_hlld.LAST_DIAG
+                    out[f"{pre}_diag"] = np.array(
+                        [d["n_not_converged"], d["n_wave_order_bad"], d["n_bn_zero"],
+                         d["n_hlle_fallback"], d["max_iters"]], dtype=np.int64)
+                    out[f"{pre}_mean_iters"] = np.array(d["mean_iters"])
+    finally:
+        R_hlld._PSTAR_METHOD, R_hlld._PSTAR_BRACKET, R_hlld._PSTAR_TOL = saved
+    np.savez_compressed(HERE / "pstar_methods.npz", **out)
+
+
 def gen_c2p():
     eos = hybrid_eos(K=0.0, gamma=5.0 / 3.0, gamma_th=5.0 / 3.0)
     rng = np.random.default_rng(7)
@@ -191,7 +228,7 @@ def _state_arrays(prefix, st, out):
         out[f"{prefix}prim_{k}"] = _np(st.prims[k])
 
 
-def _run2d(name, problem, n, limiter, solver, scheme, steps, cfl, gamma, box, bc):
+def _run2d(name, problem, n, limiter, solver, scheme, steps, cfl, gamma, box, bc, first=False):
     eos = hybrid_eos(K=0.0, gamma=gamma, gamma_th=gamma)
     g = Grid2D(box[0], box[1], n, box[0], box[1], n, ng=ghosts_needed(limiter))
     if problem == "rotor":
@@ -208,11 +245,13 @@ def _run2d(name, problem, n, limiter, solver, scheme, steps, cfl, gamma, box, bc
     _state_arrays("init_", st, out)
     flux = {"hlld": R_hlld.hlld_flux, "hllc": R_hlld.hllc_flux, "hlle": R_hlld.hlle_flux}[solver]
     dts = []
-    for _ in range(steps):
+    for i in range(steps):
         dt = compute_dt_2d(st.prims, g, eos, cfl)
         dts.append(dt)
         st, _ = rk_step_ct(st, g, eos, dt, scheme=scheme, bc_x=bc[0], bc_y=bc[1],
                            flux_fn=flux, limiter=limiter)
+        if first and i == 0:
+            _state_arrays("s1_", st, out)
     out["dts"] = np.array(dts)
     _state_arrays("fin_", st, out)
     np.savez_compressed(HERE / f"{name}.npz", **out)
@@ -228,6 +267,165 @@ def gen_2d():
     _run2d("rotor_sharp_mc_hlle", "rotor", 16, "mc", "hlle", "rk2", 1, 0.4, 5.0 / 3.0, r, oo)
     _run2d("ot2pi_mc_hlld", "ot", 16, "mc", "hlld", "rk2", 2, 0.4, 4.0 / 3.0,
            (0.0, 2.0 * np.pi), ("periodic", "periodic"))
+
+
+def gen_2d_long():
+    """100 steps at 32^2: the north star's multi-step criterion (L1 in rho
+    after 100 steps) needs the reference's own trajectory, not one step."""
+    r = (-0.5, 0.5)
+    _run2d("rotor_tapered_weno5z_hlld_100", "tapered", 32, "weno5z", "hlld", "rk3", 100, 0.4,
+           5.0 / 3.0, r, ("outflow", "outflow"), first=True)
+    _run2d("ot2pi_mc_hlld_100", "ot", 32, "mc", "hlld", "rk2", 100, 0.4, 4.0 / 3.0,
+           (0.0, 2.0 * np.pi), ("periodic", "periodic"), first=True)
+
+
+def gen_dt1d():
+    """driver.compute_dt on random 1D primitives, with the fast-speed bound
+    (speed_of_light=False) and the legacy c = 1 form; the 1D driver's own
+    runs only ever take the latter."""
+    from src.physics.driver import compute_dt
+    out = {}
+    for tag, gamma, seed, vmax, bsc in (("a", 5.0 / 3.0, 3, 0.6, 1.0), ("b", 4.0 / 3.0, 4, 0.95, 4.0),
+                                        ("c", 2.0, 5, 0.3, 0.0)):
+        eos = hybrid_eos(K=0.0, gamma=gamma, gamma_th=gamma)
+        rng = np.random.default_rng(seed)
+        n, ng = 61, 3
+        nt = n + 2 * ng
+        v = rng.uniform(-1.0, 1.0, (3, nt))
+        v = v / np.linalg.norm(v, axis=0) * rng.uniform(0.0, vmax, nt)
+        P = {"rho": 10.0 ** rng.uniform(-2, 1, nt), "p": 10.0 ** rng.uniform(-2, 1, nt),
+             "vx": v[0], "vy": v[1], "vz": v[2], "Bx": bsc * rng.uniform(-1, 1, nt),
+             "By": bsc * rng.uniform(-1, 1, nt), "Bz": bsc * rng.uniform(-1, 1, nt)}
+        P = {k: torch.from_numpy(x.copy()) for k, x in P.items()}
+        P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
+        for k in PRIM9:
+            out[f"{tag}_{k}"] = _np(P[k])
+        out[f"{tag}_gamma"] = np.array(gamma)
+        dx = 1.0 / n
+        for sol in (False, True):
+            out[f"{tag}_dt_{int(sol)}"] = _np(compute_dt(P, eos, dx, 0.4, ng, n,
+                                                         speed_of_light=sol))
+    np.savez_compressed(HERE / "dt1d.npz", **out)
+
+
+def gen_tubes_pertube():
+    """The per-tube-dt SSP-RK form of the batched tubes, assembled from the
+    reference's own pieces: tube_seed's _rhs/_bc and con2prim sync
+    (tube_seed.py:72-90,111-117), driver.compute_dt with the fast-speed
+    bound (driver.py:77-127), the RK weights of driver2d._RK2/_RK3 applied by
+    state.combine (state.py:77-103).  Tubes drawn as the batched workload
+    draws them (seeded CPU generator), shortened so the CPU run is quick."""
+    from src.physics.driver import compute_dt
+    from src.physics.driver2d import _RK2, _RK3
+    from src.physics.state import combine
+    from src.physics.tube_seed import _bc, _rhs
+    eos = hybrid_eos(K=0.0, gamma=5.0 / 3.0, gamma_th=5.0 / 3.0)
+    keys = ("D", "Sx", "Sy", "Sz", "tau")
+    g = torch.Generator().manual_seed(0)
+    nt_ = 3
+
+    def u(lo, hi):
+        return lo + (hi - lo) * torch.rand((nt_,), generator=g, dtype=torch.float64)
+
+    lengths = torch.randint(24, 65, (nt_,), generator=g)
+    sides = []
+    for _ in range(2):
+        rho = 10.0 ** u(-1.0, 1.0)
+        p = 10.0 ** u(-2.0, 1.0)
+        ct = u(-1.0, 1.0)
+        ph = u(0.0, 2.0 * np.pi)
+        sp = u(0.0, 0.9)
+        st = torch.sqrt(1.0 - ct * ct)
+        sides.append({"rho": rho, "p": p, "vx": sp * st * torch.cos(ph),
+                      "vy": sp * st * torch.sin(ph), "vz": sp * ct,
+                      "By": u(-2.0, 2.0), "Bz": u(-2.0, 2.0)})
+    Bn = u(-1.0, 1.0)
+    out = {"lengths": _np(lengths), "Bn": _np(Bn)}
+    for s, side in zip(("L", "R"), sides):
+        for k, v in side.items():
+            out[f"{s}_{k}"] = _np(v)
+    for lim, flux, scheme, steps, ntubes in (("mp5", "hlld", "rk3", 20, nt_),
+                                             ("mc", "hllc", "rk2", 15, 2)):
+        ng = ghosts_needed(lim)
+        rk = {"rk2": _RK2, "rk3": _RK3}[scheme]
+        for i in range(ntubes):
+            n = int(lengths[i])
+            dx = 1.0 / n
+            x = (torch.arange(n + 2 * ng, dtype=torch.float64) - ng + 0.5) * dx
+            m = x < 0.5
+            P = {k: torch.where(m, torch.full_like(x, float(sides[0][k][i])),
+                                torch.full_like(x, float(sides[1][k][i])))
+                 for k in ("rho", "p", "vx", "vy", "vz", "By", "Bz")}
+            P["Bx"] = torch.full_like(x, float(Bn[i]))
+            P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
+            c = R_hlld.primitive_to_conserved(P)
+            cons = {k: c[k].clone() for k in keys}
+            cons["By"], cons["Bz"] = P["By"].clone(), P["Bz"].clone()
+            Bx = P["Bx"]
+
+            def sync(cc):
+                full = {k: cc[k] for k in keys}
+                full["Bx"], full["By"], full["Bz"] = Bx, cc["By"], cc["Bz"]
+                p = conservative_to_primitive(full, eos)
+                p["Bx"], p["By"], p["Bz"] = Bx, cc["By"], cc["Bz"]
+                _bc(p, ng)
+                return p
+
+            z = torch.zeros(1, dtype=torch.float64)
+            t, dts = 0.0, []
+            for _ in range(steps):
+                dt = float(compute_dt(P, eos, dx, 0.4, ng, n, speed_of_light=False))
+                dts.append(dt)
+                S0 = State2D(cons={k: v for k, v in cons.items()}, Bxf=z, Byf=z)
+                S = S0
+                for a, b, cw in rk:
+                    L = _rhs(P, eos, dx, ng, lim, flux)
+                    S = combine([S0, S, State2D(cons=L, Bxf=z, Byf=z)], [a, b, cw * dt])
+                    P = sync(S.cons)
+                cons = S.cons
+                t += dt
+            pre = f"{lim}_{flux}_{scheme}_{i}"
+            out[f"{pre}_dts"] = np.array(dts)
+            for k in PRIM9:
+                out[f"{pre}_{k}"] = _np(P[k])
+        out[f"{lim}_{flux}_{scheme}_steps"] = np.array(steps)
+        out[f"{lim}_{flux}_{scheme}_ntubes"] = np.array(ntubes)
+    np.savez_compressed(HERE / "tubes_pertube.npz", **out)
+
+
+def gen_run2d():
+    """scripts/run_2d.py itself on a 32^2 sharp rotor (diag.csv every 2 steps,
+    the final snapshot), and its non-finite guard at CFL 40
+    (tests/test_high_order_2d.py:196-215)."""
+    import subprocess
+    import tempfile
+    out = {}
+    base = ["--problem", "rotor", "--n", "32", "--tend", "0.02", "--cfl", "0.4",
+            "--log-every", "2", "--nsnap", "2", "--solver", "hlld"]
+    env = dict(os.environ, TORCH_THREADS="4")
+    for lim in ("mc", "weno5z"):
+        with tempfile.TemporaryDirectory() as d:
+            r = subprocess.run([sys.executable, "scripts/run_2d.py", *base, "--limiter", lim,
+                                "--out", d], cwd=REF, capture_output=True, text=True,
+                               timeout=1800, env=env)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+            out[f"{lim}_diag"] = np.array(open(os.path.join(d, "diag.csv")).read().splitlines())
+            f = np.load(os.path.join(d, "snap_fin.npz"))
+            for k in ("t", "rho", "p", "vx", "vy", "Bx", "By", "divB"):
+                out[f"{lim}_fin_{k}"] = f[k]
+    with tempfile.TemporaryDirectory() as d:
+        boom = os.path.join(d, "boom")
+        r = subprocess.run([sys.executable, "scripts/run_2d.py", "--problem", "rotor", "--n", "32",
+                            "--solver", "hlld", "--limiter", "mc", "--cfl", "40.0", "--nsnap",
+                            "2", "--out", boom], cwd=REF, capture_output=True, text=True,
+                           timeout=1800, env=env)
+        msg = (r.stdout + r.stderr)
+        line = [ln for ln in msg.splitlines() if "run failed" in ln]
+        out["cfl40_returncode"] = np.array(r.returncode)
+        out["cfl40_message"] = np.array(line[-1] if line else "")
+        out["cfl40_snap_fin"] = np.array(os.path.exists(os.path.join(boom, "snap_fin.npz")))
+    np.savez_compressed(HERE / "run2d_rotor32.npz", **out)
 
 
 def gen_1d():
@@ -304,7 +502,8 @@ def gen_host():
 
 if __name__ == "__main__":
     torch.set_num_threads(4)
-    which = sys.argv[1:] or ["flux", "c2p", "recon", "2d", "1d", "tubes", "host", "init2d"]
+    which = sys.argv[1:] or ["flux", "c2p", "recon", "2d", "1d", "tubes", "host", "init2d",
+                             "2d_long", "dt1d", "tubes_pertube", "run2d", "pstar"]
     for w in which:
         print("generating", w, flush=True)
         globals()[f"gen_{w}"]()

@@ -141,3 +141,26 @@ def test_mixed_orszag_tang_periodic():
         out.append(S.P[0][2:66, 2:66].clone())
     assert torch.isfinite(out[1]).all()
     assert _l1(out[0], out[1]) < 1e-4
+
+
+@pytest.mark.xfail(strict=True, reason=(
+    "the mixed form misses L1(rho) <= 1e-6 at the configuration's 2048^2 after 200 steps "
+    "(measured 1.1e-5): HLLD's p* is pinned only to its 1e-10 residual tolerance "
+    "(hlld.py:1067-1080), and any change of the root's inputs moves the stopping iterate "
+    "within that freedom; the FP64 form run from different but equally valid roots lands "
+    "at the same order"))
+def test_mixed_meets_the_l1_gate_at_2048_after_200_steps():
+    """The configuration's own gate on the configuration's own run: tapered
+    rotor 2048^2, WENO5-Z + HLLD + CT + RK3, 200 steps, L1(rho) of the mixed
+    form vs the FP64 form <= 1e-6.  Strict: a change that meets it shows up
+    as an unexpected pass."""
+    _dev()
+    print("mixed vs fp64 at 2048^2 x 200 steps (the L1 gate)", flush=True)
+    A, g, ng = _run(2048, "weno5z", "hlld", "fp64", 200)
+    ra = _phys(A, g, ng)[0].clone()
+    del A
+    B, _, _ = _run(2048, "weno5z", "hlld", "mixed", 200)
+    rb = _phys(B, g, ng)[0]
+    l1 = float((rb - ra).abs().mean() / ra.abs().mean())
+    print(f"L1(rho) = {l1:.3e}", flush=True)
+    assert l1 <= 1e-6, l1

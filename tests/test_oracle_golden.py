@@ -151,6 +151,10 @@ def test_2d_steps_bitwise(golden_dir, case):
 @pytest.mark.slow
 @pytest.mark.parametrize("solver", ["hlld", "hllc", "hlle"])
 def test_balsara1_bitwise(golden_dir, solver):
+    """driver.run on Balsara 1: HLLC and HLLE to t = 0.05 (50 steps); HLLD
+    to t = 0.4 (400 steps, ~5 minutes of CPU: HLLD_ORACLE_FULL=1 runs it)."""
+    if solver == "hlld" and os.environ.get("HLLD_ORACLE_FULL") != "1":
+        pytest.skip("400 HLLD steps take minutes on the CPU; set HLLD_ORACLE_FULL=1")
     g = _load(golden_dir, "balsara1.npz")
     cfg = S1.balsara1_cfg(solver=solver)
     cfg["run"]["t_end"] = float(g[f"{solver}_t"]) if solver != "hlld" else 0.4
@@ -171,3 +175,127 @@ def test_tubes_shared_bitwise(golden_dir, lim):
     assert t == float(g[f"{lim}_t"])
     for k, v in prof.items():
         _eq(v.numpy(), g[f"{lim}_{k}"], f"tubes {lim} {k}")
+
+
+@pytest.mark.parametrize("tag", ["a", "b", "c"])
+def test_compute_dt_1d_bitwise(golden_dir, tag):
+    """driver.compute_dt with the fast-speed bound and with c = 1."""
+    g = _load(golden_dir, "dt1d.npz")
+    eos = E.ideal(float(g[f"{tag}_gamma"]))
+    P = {k: torch.from_numpy(g[f"{tag}_{k}"].copy()) for k in PRIM9}
+    n, ng = 61, 3
+    for sol in (False, True):
+        dt = S1.compute_dt(P, eos, 1.0 / n, 0.4, ng, n, speed_of_light=sol)
+        _eq(dt.numpy(), g[f"{tag}_dt_{int(sol)}"], f"dt speed_of_light={sol}")
+
+
+TUBE_CASES = [("mp5", "hlld", "rk3"), ("mc", "hllc", "rk2")]
+
+
+@pytest.mark.parametrize("lim,flux,scheme", TUBE_CASES)
+def test_tubes_per_tube_dt_bitwise(golden_dir, lim, flux, scheme):
+    """The per-tube-dt SSP-RK composition (oracle.tubes.run_tube) against
+    the same composition built from the reference's own tube_seed, driver,
+    driver2d and state pieces."""
+    g = _load(golden_dir, "tubes_pertube.npz")
+    eos = E.ideal(5.0 / 3.0)
+    pre = f"{lim}_{flux}_{scheme}"
+    steps = int(g[f"{pre}_steps"])
+    ng = RC.ghosts_needed(lim)
+    for i in range(int(g[f"{pre}_ntubes"])):
+        n = int(g["lengths"][i])
+        left = {k: float(g[f"L_{k}"][i]) for k in ("rho", "p", "vx", "vy", "vz", "By", "Bz")}
+        right = {k: float(g[f"R_{k}"][i]) for k in ("rho", "p", "vx", "vy", "vz", "By", "Bz")}
+        P0 = T.tube_initial(n, ng, eos, left, right, float(g["Bn"][i]))
+        P, t = T.run_tube(P0, eos, n, ng, steps, cfl=0.4, limiter=lim, scheme=scheme, flux=flux)
+        t_ref = 0.0
+        for dt in g[f"{pre}_{i}_dts"]:
+            t_ref += float(dt)
+        assert t == t_ref, (i, t, t_ref)
+        for k in PRIM9:
+            _eq(P[k].numpy(), g[f"{pre}_{i}_{k}"], f"tube {i} {k}")
+
+
+LONG_2D = [
+    ("rotor_tapered_weno5z_hlld_100", "tapered", 32, "weno5z", "hlld", "rk3", 5 / 3, (-0.5, 0.5),
+     "outflow"),
+    ("ot2pi_mc_hlld_100", "ot", 32, "mc", "hlld", "rk2", 4 / 3, (0.0, 2 * np.pi), "periodic"),
+]
+
+
+def _run_case(g, case, steps, prefix):
+    name, problem, n, lim, solver, scheme, gamma, box, bc = case
+    eos = E.ideal(gamma)
+    grid = S2.make_grid(n, box, lim)
+    init = {"rotor": S2.rotor_sharp, "tapered": S2.rotor_tapered,
+            "ot": S2.orszag_tang_2pi}[problem]
+    P, Az = init(grid, eos)
+    st = S2.init_state(grid, eos, P, Az, (bc, bc))
+    for dt_ref in g["dts"][:steps]:
+        dt = S2.compute_dt_2d(st.prims, grid, eos, 0.4)
+        assert dt == dt_ref
+        st, _ = S2.rk_step_ct(st, grid, eos, dt, scheme=scheme, bc_x=bc, bc_y=bc,
+                              flux_fn=R.SOLVERS[solver], limiter=lim)
+    ref = _state_from(g, prefix)
+    for k in S2.EVOLVED_KEYS:
+        _eq(st.cons[k].numpy(), ref.cons[k].numpy(), f"{prefix} cons {k}")
+    _eq(st.Bxf.numpy(), ref.Bxf.numpy(), f"{prefix} Bxf")
+    _eq(st.Byf.numpy(), ref.Byf.numpy(), f"{prefix} Byf")
+    for k in PRIM9:
+        _eq(st.prims[k].numpy(), ref.prims[k].numpy(), f"{prefix} prim {k}")
+
+
+@pytest.mark.parametrize("case", LONG_2D, ids=[c[0] for c in LONG_2D])
+def test_2d_long_fixture_first_step_bitwise(golden_dir, case):
+    """The 100-step fixtures' first step (the full 100 steps of the oracle
+    take minutes on the CPU: HLLD_ORACLE_FULL=1 runs them)."""
+    g = _load(golden_dir, f"{case[0]}.npz")
+    _run_case(g, case, 1, "s1_")
+
+
+@pytest.mark.slow
+@pytest.mark.skipif(os.environ.get("HLLD_ORACLE_FULL") != "1",
+                    reason="minutes of CPU; set HLLD_ORACLE_FULL=1")
+@pytest.mark.parametrize("case", LONG_2D, ids=[c[0] for c in LONG_2D])
+def test_2d_long_fixture_100_steps_bitwise(golden_dir, case):
+    g = _load(golden_dir, f"{case[0]}.npz")
+    _run_case(g, case, len(g["dts"]), "fin_")
+
+
+PSTAR_VARIANTS = [("newton", "legacy", 1e-10), ("newton_fd", "legacy", 1e-10),
+                  ("newton_fd1", "legacy", 1e-10), ("newton_safe", "legacy", 1e-10),
+                  ("secant", "hllc", 1e-10), ("newton_safe", "hllc", 1e-10),
+                  ("secant", "legacy", 1e-13)]
+
+
+@pytest.mark.parametrize("method,bracket,tol", PSTAR_VARIANTS,
+                         ids=[f"{m}-{b}-{t:g}" for m, b, t in PSTAR_VARIANTS])
+@pytest.mark.parametrize("tag", ["a", "b"])
+def test_hlld_pstar_variants_bitwise(golden_dir, method, bracket, tol, tag):
+    """hlld_flux with the reference's call-time p* switches: the root finder
+    (_PSTAR_METHOD), the bracket source (_PSTAR_BRACKET) and the tolerance
+    (_PSTAR_TOL), on the flux_random pairs in every direction."""
+    g = _load(golden_dir, "pstar_methods.npz")
+    f = _load(golden_dir, "flux_random.npz")
+    eos = E.ideal(5.0 / 3.0)
+    sL = {k: torch.from_numpy(f[f"{tag}_L_{k}"]) for k in PRIM9}
+    sR = {k: torch.from_numpy(f[f"{tag}_R_{k}"]) for k in PRIM9}
+    saved = (R.PSTAR_METHOD, R.PSTAR_BRACKET, R.PSTAR_TOL)
+    try:
+        R.PSTAR_METHOD, R.PSTAR_BRACKET, R.PSTAR_TOL = method, bracket, tol
+        for idir in (0, 1, 2):
+            F, U, ps = R.hlld_flux(sL, sR, eos, idir=idir)
+            pre = f"{method}_{bracket}_{tol:g}_{tag}{idir}"
+            for k in CONS8:
+                _eq(F[k].numpy(), g[f"{pre}_F_{k}"], f"{pre} F[{k}]")
+                if idir == 0:
+                    _eq(U[k].numpy(), g[f"{pre}_U_{k}"], f"{pre} U[{k}]")
+            _eq(ps.numpy(), g[f"{pre}_pstar"], f"{pre} p*")
+            d = R.LAST_DIAG
+            got = [d["n_not_converged"], d["n_wave_order_bad"], d["n_bn_zero"],
+                   d["n_hlle_fallback"], d["max_iters"]]
+            assert got == list(g[f"{pre}_diag"]), pre
+            _eq(d["mean_iters"], g[f"{pre}_mean_iters"], "mean_iters")
+            assert d["pstar_method"] == method
+    finally:
+        R.PSTAR_METHOD, R.PSTAR_BRACKET, R.PSTAR_TOL = saved
