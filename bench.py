@@ -609,6 +609,29 @@ def bench_rotor(a):
                    "roofline": res[form]["roofline"], "description": form_desc[form]}
             for form in forms if form != "fp64"},
     }
+    if a.tol_study:
+        # the FP64 form with HLLD's p* tolerance (the reference's _PSTAR_TOL,
+        # hlld.py:1080) tightened: how far the reference's own stopping rule
+        # leaves the answer free
+        from srmhd import driver2d as D2, hlld as H
+        study = {}
+        for tol in (1e-11, 1e-12, 1e-13):
+            H._PSTAR_TOL = tol
+            try:
+                stp = D2.Stepper2D(a.grid, E.hybrid_eos(0.0, 5.0 / 3.0, 5.0 / 3.0),
+                                   solver=a.solver, limiter=a.limiter, precision="fp64")
+                stp.load([t.clone() for t in U_init], [t.clone() for t in P_init])
+                _continue_to(a, stp, 0, a.gate_at)
+                study[f"{tol:g}"] = l1_of_fields(_rotor_fields(stp, a.n, ng))
+                del stp
+            finally:
+                H._PSTAR_TOL = 1e-10
+            print(f"tol study {tol:g} done", file=sys.stderr, flush=True)
+        out["pstar_tol_study"] = {
+            f"l1_vs_fp64_after_{a.gate_at}_steps": study,
+            "description": "the FP64 form with HLLD's p* residual tolerance tightened from the "
+                           "reference's 1e-10: its distance from the FP64 form is the freedom "
+                           "the reference's own stopping rule leaves"}
     if not a.no_floor:
         fl = _rotor_l1_floor(a, U_init, P_init)
         out["l1_floor"] = {
@@ -650,6 +673,9 @@ def main():
                          "of each form)")
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tol-study", action="store_true",
+                    help="rotor: also run the FP64 form with HLLD's p* tolerance at 1e-11, "
+                         "1e-12, 1e-13 and report each one's L1 from the FP64 form")
     ap.add_argument("--no-floor", action="store_true",
                     help="skip the untimed FP64 run from the one-ulp-perturbed initial state")
     ap.add_argument("--copy-calibration", action="store_true",

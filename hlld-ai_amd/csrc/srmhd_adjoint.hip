@@ -639,6 +639,11 @@ bool grid_ok_a(const srmhd_grid2d* g, const srmhd_scheme2d* s) {
   if (s->precision != SRMHD_PREC_FP64 || s->emf_mode != SRMHD_EMF_SOLVER || s->solver < 0 ||
       s->solver > 2 || s->limiter < 0 || s->limiter > 5)
     return false;
+  // the reference's default p* root (zeroed options): the one the forward taped
+  if (s->hlld.pstar_method != SRMHD_PSTAR_SECANT ||
+      s->hlld.pstar_bracket != SRMHD_BRACKET_LEGACY ||
+      !(s->hlld.pstar_tol <= 0.0 || s->hlld.pstar_tol == 1e-10))
+    return false;
   const int need = s->limiter == SRMHD_PCM ? 0
                    : (s->limiter <= SRMHD_MINMOD ? 1 : (s->limiter == SRMHD_MP7 ? 3 : 2));
   if (g->ng < (need + 1 > 2 ? need + 1 : 2) && s->limiter != SRMHD_PCM) return false;

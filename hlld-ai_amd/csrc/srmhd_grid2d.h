@@ -177,7 +177,7 @@ __device__ inline void face_ij(const G2& g, int64_t t, int& i, int& j) {
 }
 
 template <int N>
-__device__ inline int64_t n_faces(const G2& g) {
+__host__ __device__ inline int64_t n_faces(const G2& g) {
   return N == 0 ? (int64_t)(g.nxt + 1) * g.nyt : (int64_t)g.nxt * (g.nyt + 1);
 }
 

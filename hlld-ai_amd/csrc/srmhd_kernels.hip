@@ -9,6 +9,7 @@
 
 #include "srmhd_grid2d.h"
 #include "srmhd_mixed.h"
+#include "srmhd_pstar_launch.h"
 #include "srmhd_recon.h"
 #include "srmhd_riemann.h"
 
@@ -77,7 +78,7 @@ template <int N, int S>
 __global__ __launch_bounds__(BLK) void k_flux_b(int64_t n, PrimPtrs L, PrimPtrs R, srmhd_eos e,
                                                 const C2PRec* rec, srmhd_flux_diag* d,
                                                 ConsPtrs F, ConsPtrs U, int want_u,
-                                                double* pstar, uint8_t* status) {
+                                                double* pstar, uint8_t* status, double tol) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = i < n;
   int st = 0, iters = 0;
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(BLK) void k_flux_b(int64_t n, PrimPtrs L, PrimPtrs 
       k.init(h.uH, e);
       Prim ph;
       const int cst = c2p_pass_b(k, rec[i], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
-      st = hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, u, ps, iters);
+      st = hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, u, ps, iters, tol);
       if (cst & C2P_REDONE) st |= FS_C2P_REDONE;
     }
 #pragma unroll
@@ -202,7 +203,7 @@ template <int N, int LIM, int SOL, bool LOCAL = false>
 __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e,
                                                  const C2PRec* rec, const srmhd_flux_diag* d,
                                                  srmhd_flux_diag* dsh, Flx F, double* pstar_out,
-                                                 int emf_hll) {
+                                                 int emf_hll, double tol) {
   static_assert(!LOCAL || SOL == SOLVER_HLLD, "the local con2prim concerns HLLD only");
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = t < n_faces<N>(g);
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
       Prim ph;
       const int cst = LOCAL ? c2p_single(k, 1e-10, 1e-10, ph)
                             : c2p_pass_b(k, rec[t], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
-      st = hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, nullptr, ps, iters);
+      st = hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, nullptr, ps, iters, tol);
       if (cst & C2P_REDONE) st |= FS_C2P_REDONE;
     }
     F.p[0][t] = f[CD];
@@ -603,18 +604,29 @@ namespace {
 template <int N>
 int flux_dispatch(int solver, int64_t n, const PrimPtrs& L, const PrimPtrs& R, const srmhd_eos& e,
                   C2PRec* rec, srmhd_flux_diag* d, const ConsPtrs& F, const ConsPtrs& U,
-                  int want_u, double* pstar, uint8_t* status, hipStream_t s) {
+                  int want_u, double* pstar, uint8_t* status, hipStream_t s,
+                  const HlldOpts& o = HlldOpts()) {
   const int nb = blocks_for(n);
   switch (solver) {
     case SOLVER_HLLE:
-      k_flux_b<N, SOLVER_HLLE><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d, F, U, want_u, pstar, status);
+      k_flux_b<N, SOLVER_HLLE><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d, F, U, want_u, pstar, status,
+                                                  o.tol);
       break;
     case SOLVER_HLLC:
-      k_flux_b<N, SOLVER_HLLC><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d, F, U, want_u, pstar, status);
+      k_flux_b<N, SOLVER_HLLC><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d, F, U, want_u, pstar, status,
+                                                  o.tol);
       break;
     default:
-      k_flux_a<N><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d);
-      k_flux_b<N, SOLVER_HLLD><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d, F, U, want_u, pstar, status);
+      // the HLL-state con2prim batch: the legacy bracket's p_hll
+      if (o.bracket == SRMHD_BRACKET_LEGACY) k_flux_a<N><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d);
+      if (hlld_opts_fused(o)) {
+        k_flux_b<N, SOLVER_HLLD><<<nb, BLK, 0, s>>>(n, L, R, e, rec, d, F, U, want_u, pstar,
+                                                    status, o.tol);
+      } else {
+        if (const int rc = launch_status()) return rc;
+        return launch_flux_opts_b(N, n, L.p, R.p, e, rec, d, F.p, want_u ? U.p : nullptr, pstar,
+                                  status, o, s);
+      }
   }
   return launch_status();
 }
@@ -655,6 +667,7 @@ struct Ws2 {
   Cells32 cf{};                 // FP32 cell variables of the MIXED form (carved for it only)
   srmhd_flux_diag* dsh;         // DIAG_SHARDS copies of diag[0], then of diag[1]
   int emf_hll = 0;              // the scheme's emf_mode is SRMHD_EMF_HLL
+  HlldOpts o;                   // the scheme's HLLD p* options
   int64_t bytes;
 };
 
@@ -699,7 +712,7 @@ void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const W
   }
   tbegin(SRMHD_K_SWEEP_X_B, s);
   k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, shx, w.Fx, w.psx,
-                                             w.emf_hll);
+                                             w.emf_hll, w.o.tol);
   tend(SRMHD_K_SWEEP_X_B, s);
   if (SOL == SOLVER_HLLD) {
     tbegin(SRMHD_K_SWEEP_Y_A, s);
@@ -709,7 +722,36 @@ void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const W
   }
   tbegin(SRMHD_K_SWEEP_Y_B, s);
   k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, shy, w.Fy, w.psy,
-                                             w.emf_hll);
+                                             w.emf_hll, w.o.tol);
+  tend(SRMHD_K_SWEEP_Y_B, s);
+  k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
+}
+
+// HLLD sweeps under non-default p* options: the default first pass where the
+// legacy bracket needs the HLL-state con2prim, then the option kernels
+template <int LIM>
+void sweeps_opts(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const Ws2& w,
+                 hipStream_t s) {
+  const int bx = blocks_for((int64_t)(g.nxt + 1) * g.nyt);
+  const int by = blocks_for((int64_t)g.nxt * (g.nyt + 1));
+  srmhd_flux_diag* shx = w.dsh;
+  srmhd_flux_diag* shy = (srmhd_flux_diag*)((char*)w.dsh + DIAG_SHARDS * DIAG_SHARD_BYTES);
+  const bool legacy = w.o.bracket == SRMHD_BRACKET_LEGACY;
+  if (legacy) {
+    k_sweep_a<0, LIM><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, shx);
+    k_fold_diag<<<1, 64, 0, s>>>(shx, w.diag + 0);
+  }
+  tbegin(SRMHD_K_SWEEP_X_B, s);
+  launch_sweep_opts_b<0, LIM>(g, S, P, e, w.recx, w.diag + 0, shx, w.Fx, w.psx, w.emf_hll, w.o,
+                              s);
+  tend(SRMHD_K_SWEEP_X_B, s);
+  if (legacy) {
+    k_sweep_a<1, LIM><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, shy);
+    k_fold_diag<<<1, 64, 0, s>>>(shy, w.diag + 1);
+  }
+  tbegin(SRMHD_K_SWEEP_Y_B, s);
+  launch_sweep_opts_b<1, LIM>(g, S, P, e, w.recy, w.diag + 1, shy, w.Fy, w.psy, w.emf_hll, w.o,
+                              s);
   tend(SRMHD_K_SWEEP_Y_B, s);
   k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
 }
@@ -724,11 +766,11 @@ void sweeps_local(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, c
   srmhd_flux_diag* shy = (srmhd_flux_diag*)((char*)w.dsh + DIAG_SHARDS * DIAG_SHARD_BYTES);
   tbegin(SRMHD_K_SWEEP_X_B, s);
   k_sweep_b<0, LIM, SOLVER_HLLD, true><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, shx,
-                                                          w.Fx, w.psx, w.emf_hll);
+                                                          w.Fx, w.psx, w.emf_hll, w.o.tol);
   tend(SRMHD_K_SWEEP_X_B, s);
   tbegin(SRMHD_K_SWEEP_Y_B, s);
   k_sweep_b<1, LIM, SOLVER_HLLD, true><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, shy,
-                                                          w.Fy, w.psy, w.emf_hll);
+                                                          w.Fy, w.psy, w.emf_hll, w.o.tol);
   tend(SRMHD_K_SWEEP_Y_B, s);
   k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
 }
@@ -742,6 +784,8 @@ void sweeps_solver(int sol, bool local, const G2& g, const St2& S, const Pr2& P,
     default:
       if (local)
         sweeps_local<LIM>(g, S, P, e, w, s);
+      else if (!hlld_opts_fused(w.o))
+        sweeps_opts<LIM>(g, S, P, e, w, s);
       else
         sweeps<LIM, SOLVER_HLLD>(g, S, P, e, w, s);
   }
@@ -1059,13 +1103,13 @@ PrimPtrs as_in(const OutPrimPtrs& o) {
 int stage1d(const G1& g, int solver, int lim, const srmhd_eos& e, double atmo_rho, int stage,
             const double* dt_dev, const CC8& U, const CC8& U1, const PrimPtrs& P,
             const PrimPtrs& Pold, const ConsPtrs& out, const OutPrimPtrs& Pout, const Ws1& w,
-            hipStream_t s) {
+            hipStream_t s, const HlldOpts& o) {
   recon1d_dispatch(lim, g, P, w, s);
   ConsPtrs none;
   for (int k = 0; k < 8; ++k) none.p[k] = nullptr;
   srmhd_flux_diag* d = w.diag + (stage - 1);
   const int rc = flux_dispatch<0>(solver, g.n + 1, as_in(w.L), as_in(w.R), e, w.recf, d, w.F,
-                                  none, 0, nullptr, nullptr, s);
+                                  none, 0, nullptr, nullptr, s, o);
   if (rc) return rc;
   CC8 F;
   for (int k = 0; k < 8; ++k) F.p[k] = w.F.p[k];
@@ -1111,9 +1155,20 @@ int srmhd_flux(int solver, int idir, int64_t n, const double* const* primL,
                const double* const* primR, const srmhd_eos* eos, double* const* F,
                double* const* U, double* pstar, uint8_t* status, srmhd_flux_diag* diag,
                void* work, int64_t work_bytes, void* stream) {
+  return srmhd_flux_ex(solver, idir, n, primL, primR, eos, nullptr, F, U, pstar, status, diag,
+                       work, work_bytes, stream);
+}
+
+int srmhd_flux_ex(int solver, int idir, int64_t n, const double* const* primL,
+                  const double* const* primR, const srmhd_eos* eos,
+                  const srmhd_hlld_opts* opts, double* const* F, double* const* U,
+                  double* pstar, uint8_t* status, srmhd_flux_diag* diag, void* work,
+                  int64_t work_bytes, void* stream) {
   if (solver < 0 || solver > 2 || idir < 0 || idir > 2 || n < 0 || !primL || !primR || !F ||
       !eos_ok(eos) || !work)
     return SRMHD_EARG;
+  const HlldOpts o = hlld_opts(opts);
+  if (!hlld_opts_ok(o)) return SRMHD_EARG;
   if (work_bytes < srmhd_flux_workspace_bytes(solver, n)) return SRMHD_EARG;
   hipStream_t s = (hipStream_t)stream;
   Carver c{(char*)work};
@@ -1136,11 +1191,11 @@ int srmhd_flux(int solver, int idir, int64_t n, const double* const* primL,
     const int want_u = U ? 1 : 0;
     int rc;
     if (idir == 0)
-      rc = flux_dispatch<0>(solver, n, L, R, *eos, rec, d, Fp, Up, want_u, pstar, status, s);
+      rc = flux_dispatch<0>(solver, n, L, R, *eos, rec, d, Fp, Up, want_u, pstar, status, s, o);
     else if (idir == 1)
-      rc = flux_dispatch<1>(solver, n, L, R, *eos, rec, d, Fp, Up, want_u, pstar, status, s);
+      rc = flux_dispatch<1>(solver, n, L, R, *eos, rec, d, Fp, Up, want_u, pstar, status, s, o);
     else
-      rc = flux_dispatch<2>(solver, n, L, R, *eos, rec, d, Fp, Up, want_u, pstar, status, s);
+      rc = flux_dispatch<2>(solver, n, L, R, *eos, rec, d, Fp, Up, want_u, pstar, status, s, o);
     if (rc) return rc;
   }
   if (diag) k_copy_diag<<<1, 64, 0, s>>>(d, diag, 1);
@@ -1192,11 +1247,21 @@ int64_t srmhd2d_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s) 
   return carve2d(make_g2(g), nullptr, s->precision == SRMHD_PREC_MIXED).bytes;
 }
 
+// the HLLD p* options are the reference's default
+static bool hlld_default(const HlldOpts& o) {
+  return o.method == SRMHD_PSTAR_SECANT && o.bracket == SRMHD_BRACKET_LEGACY &&
+         o.tol == PSTAR_TOL;
+}
+
 static bool scheme_ok(const srmhd_scheme2d* s) {
-  return s && s->solver >= 0 && s->solver <= 2 && s->limiter >= 0 && s->limiter <= 5 &&
-         (s->precision == SRMHD_PREC_FP64 || s->precision == SRMHD_PREC_MIXED ||
-          s->precision == SRMHD_PREC_MIXED_R64 || s->precision == SRMHD_PREC_FP64_LOCAL) &&
-         (s->emf_mode == SRMHD_EMF_SOLVER || s->emf_mode == SRMHD_EMF_HLL);
+  if (!(s && s->solver >= 0 && s->solver <= 2 && s->limiter >= 0 && s->limiter <= 5 &&
+        (s->precision == SRMHD_PREC_FP64 || s->precision == SRMHD_PREC_MIXED ||
+         s->precision == SRMHD_PREC_MIXED_R64 || s->precision == SRMHD_PREC_FP64_LOCAL) &&
+        (s->emf_mode == SRMHD_EMF_SOLVER || s->emf_mode == SRMHD_EMF_HLL)))
+    return false;
+  const HlldOpts o = hlld_opts(&s->hlld);
+  // the options are the FP64 form's (the other forms have their own roots)
+  return hlld_opts_ok(o) && (s->precision == SRMHD_PREC_FP64 || hlld_default(o));
 }
 
 // the deadband thresholds of the upwind EMF, as k_corner forms them
@@ -1217,10 +1282,13 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   if (tape && (sp->precision != SRMHD_PREC_FP64 || !tape->pstar_x || !tape->pstar_y ||
                !tape->FD_x || !tape->FD_y || !tape->thresh))
     return SRMHD_EARG;
+  // the adjoint differentiates the reference's default root only
+  if (tape && !hlld_default(hlld_opts(&sp->hlld))) return SRMHD_EARG;
   const G2 g = make_g2(gp);
   Ws2 w = carve2d(g, work, sp->precision == SRMHD_PREC_MIXED);
   if (work_bytes < w.bytes) return SRMHD_EARG;
   w.emf_hll = sp->emf_mode == SRMHD_EMF_HLL;
+  w.o = hlld_opts(&sp->hlld);
   if (tape) {
     w.psx = tape->pstar_x;
     w.psy = tape->pstar_y;
@@ -1373,9 +1441,20 @@ int srmhd1d_rk2_step(const srmhd_grid1d* gp, int solver, int limiter, const srmh
                      double atmo_rho, const double* dt_dev, const double* const* cons,
                      const double* const* prim, double* const* cons_out, double* const* prim_out,
                      srmhd_flux_diag* diag2, void* work, int64_t work_bytes, void* stream) {
+  return srmhd1d_rk2_step_ex(gp, solver, limiter, eos, nullptr, atmo_rho, dt_dev, cons, prim,
+                             cons_out, prim_out, diag2, work, work_bytes, stream);
+}
+
+int srmhd1d_rk2_step_ex(const srmhd_grid1d* gp, int solver, int limiter, const srmhd_eos* eos,
+                        const srmhd_hlld_opts* opts, double atmo_rho, const double* dt_dev,
+                        const double* const* cons, const double* const* prim,
+                        double* const* cons_out, double* const* prim_out,
+                        srmhd_flux_diag* diag2, void* work, int64_t work_bytes, void* stream) {
   if (!g1_ok(gp) || solver < 0 || solver > 2 || limiter < 0 || limiter > 5 || !eos_ok(eos) ||
       !dt_dev || !cons || !prim || !cons_out || !prim_out || !work)
     return SRMHD_EARG;
+  const HlldOpts o = hlld_opts(opts);
+  if (!hlld_opts_ok(o)) return SRMHD_EARG;
   const G1 g = make_g1(gp);
   const Ws1 w = carve1d(g, work);
   if (work_bytes < w.bytes) return SRMHD_EARG;
@@ -1397,10 +1476,11 @@ int srmhd1d_rk2_step(const srmhd_grid1d* gp, int solver, int limiter, const srmh
   hipStream_t s = (hipStream_t)stream;
   k_zero_u64<<<1, 64, 0, s>>>((unsigned long long*)w.diag, N_WORDS);
   k_zero_u64<<<1, 64, 0, s>>>(w.mask, 2);
-  int rc = stage1d(g, solver, limiter, *eos, atmo_rho, 1, dt_dev, U, U1, P, P, w.U1, w.P1, w, s);
+  int rc =
+      stage1d(g, solver, limiter, *eos, atmo_rho, 1, dt_dev, U, U1, P, P, w.U1, w.P1, w, s, o);
   if (rc) return rc;
   rc = stage1d(g, solver, limiter, *eos, atmo_rho, 2, dt_dev, U, U1, as_in(w.P1), P, out, Pout,
-               w, s);
+               w, s, o);
   if (rc) return rc;
   if (diag2) k_copy_diag<<<1, 64, 0, s>>>(w.diag, diag2, 2);
   return launch_status();

@@ -392,18 +392,13 @@ SRMHD_DEV int hlld_assemble(const HlldPrep<N, T>& h, const StarRT<T>& R, const T
          (bn_zero ? FS_BN_ZERO : 0) | (failed ? FS_FALLBACK : 0);
 }
 
-// The rest of hlld_flux given p_hll   hlld.py:1265-1534.  Writes F (and U
-// when non-null), p*, and returns the face status bits; *iters gets the
-// secant iteration count (iters_used).
+// p0, the p* bracket and its expansion   hlld.py:1265-1321.  Fills R and
+// returns the bracket ends and their residuals.
 template <int N>
-SRMHD_DEV int hlld_finish(const HlldPrep<N>& h, const Prim& L, const Prim& Rp, double p_hll,
-                          double F[8], double* U, double& pstar, int& iters) {
-  const double cmin = h.cmin, cmax = h.cmax;
-  double fH[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) fH[k] = hll_flux(h.fL[k], h.fR[k], h.uL[k], h.uR[k], cmin, cmax);
+SRMHD_DEV void hlld_bracket(const HlldPrep<N>& h, const double fH[8], double p_hll, StarR& R,
+                            double& plo_out, double& phi_out, double& flo_out,
+                            double& fhi_out) {
   const double* uH = h.uH;
-
   // p0 from the Bn = 0 quadratic   hlld.py:1265-1287
   const double a = fH[CT] + fH[CD];
   const double b = -fH[CS + N] - uH[CT] - uH[CD];
@@ -421,7 +416,6 @@ SRMHD_DEV int hlld_finish(const HlldPrep<N>& h, const Prim& L, const Prim& Rp, d
   double plo = use_p0 ? p0 * 0.99 : p_hll * 0.99;
   double phi = use_p0 ? p0 * 1.01 : p_hll * 1.01;
 
-  StarR R;
   hlld_rvec<N>(h, R);
 
   // bracket expansion   hlld.py:1311-1321
@@ -434,6 +428,32 @@ SRMHD_DEV int hlld_finish(const HlldPrep<N>& h, const Prim& L, const Prim& Rp, d
     flo = star_residual(R, plo);
     fhi = star_residual(R, phi);
   }
+  plo_out = plo;
+  phi_out = phi;
+  flo_out = flo;
+  fhi_out = fhi;
+}
+
+// The HLL flux of the face   hlld.py:1232-1235
+template <int N, class T>
+SRMHD_DEV void hll_fluxes(const HlldPrep<N, T>& h, T fH[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    fH[k] = hll_flux(h.fL[k], h.fR[k], h.uL[k], h.uR[k], h.cmin, h.cmax);
+}
+
+// The rest of hlld_flux given p_hll   hlld.py:1265-1534.  Writes F (and U
+// when non-null), p*, and returns the face status bits; *iters gets the
+// secant iteration count (iters_used).  tol: the p* tolerance (hlld.py:1080).
+template <int N>
+SRMHD_DEV int hlld_finish(const HlldPrep<N>& h, const Prim& L, const Prim& Rp, double p_hll,
+                          double F[8], double* U, double& pstar, int& iters,
+                          double tol = PSTAR_TOL) {
+  double fH[8];
+  hll_fluxes<N>(h, fH);
+  StarR R;
+  double plo, phi, flo, fhi;
+  hlld_bracket<N>(h, fH, p_hll, R, plo, phi, flo, fhi);
 
   // clamped secant with frozen converged lanes   hlld.py:669-746
   double x0 = plo, x1 = phi, f0 = flo, f1 = fhi;
@@ -442,7 +462,7 @@ SRMHD_DEV int hlld_finish(const HlldPrep<N>& h, const Prim& L, const Prim& Rp, d
   int it = 0;
 #pragma unroll 1
   for (; it < PSTAR_MAXIT; ++it) {
-    if (fabs(f1) < PSTAR_TOL) {
+    if (fabs(f1) < tol) {
       conv = true;
       used = it;
       break;
@@ -457,7 +477,7 @@ SRMHD_DEV int hlld_finish(const HlldPrep<N>& h, const Prim& L, const Prim& Rp, d
     x1 = x2;
     f1 = f2;
   }
-  if (!conv && fabs(f1) < PSTAR_TOL) {
+  if (!conv && fabs(f1) < tol) {
     conv = true;
     used = PSTAR_MAXIT;
   }

@@ -18,12 +18,20 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(ROOT, "csrc")
 OUT_DIR = os.path.join(PKG, "_lib")
 LIB = os.path.join(OUT_DIR, "libsrmhd_hip.so")
-SOURCES = ["srmhd_kernels.hip", "srmhd_tubes.hip", "srmhd_mixed.hip", "srmhd_adjoint.hip"]
+# (source, extra flags, object): the tube kernels are compiled once per
+# limiter (srmhd_tubes_lim.hip with SRMHD_TUBE_LIM = the limiter id), so the
+# objects build in parallel
+SOURCES = ([("srmhd_kernels.hip", [], "srmhd_kernels.o"), ("srmhd_tubes.hip", [], "srmhd_tubes.o"),
+            ("srmhd_mixed.hip", [], "srmhd_mixed.o"), ("srmhd_adjoint.hip", [], "srmhd_adjoint.o"),
+            ("srmhd_pstar.hip", [], "srmhd_pstar.o")] +
+           [("srmhd_tubes_lim.hip", [f"-DSRMHD_TUBE_LIM={k}"], f"srmhd_tubes_lim{k}.o")
+            for k in range(6)])
 # the mixed-precision sweeps: FP32 contraction and approximate FP32 division
 # and sqrt (this file only; the FP64 sources keep -ffp-contract=off)
 EXTRA = {"srmhd_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
 HEADERS = ["srmhd_common.h", "srmhd_c2p.h", "srmhd_recon.h", "srmhd_riemann.h", "srmhd_launch.h",
-           "srmhd_grid2d.h", "srmhd_mixed.h", "srmhd_f32.h", "srmhd_dual.h"]
+           "srmhd_grid2d.h", "srmhd_mixed.h", "srmhd_f32.h", "srmhd_dual.h", "srmhd_pstar.h",
+           "srmhd_pstar_launch.h", "srmhd_tubes.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -44,12 +52,12 @@ def build(verbose=False, extra=()):
     """Compile every source to an object in parallel, then link the library."""
     os.makedirs(OUT_DIR, exist_ok=True)
     objs, procs = [], []
-    for src in SOURCES:
-        obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
+    for src, flags, objname in SOURCES:
+        obj = os.path.join(OUT_DIR, objname)
         objs.append(obj)
-        cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), *extra, "-c", "-I", CSRC, "-o", obj,
+        cmd = [HIPCC, *FLAGS, *EXTRA.get(src, []), *flags, *extra, "-c", "-I", CSRC, "-o", obj,
                os.path.join(CSRC, src)]
-        procs.append((src, _run([c for c in cmd if c != "-shared"], verbose)))
+        procs.append((objname, _run([c for c in cmd if c != "-shared"], verbose)))
     failed = []
     for src, p in procs:
         out = p.communicate()[0]

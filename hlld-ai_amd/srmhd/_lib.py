@@ -46,11 +46,20 @@ class Grid1D(ctypes.Structure):
                 ("reserved", ctypes.c_int32), ("dx", ctypes.c_double)]
 
 
+class HlldOpts(ctypes.Structure):
+    _fields_ = [("pstar_method", ctypes.c_int32), ("pstar_bracket", ctypes.c_int32),
+                ("pstar_tol", ctypes.c_double), ("pstar_fd_h", ctypes.c_double)]
+
+
+PSTAR_METHOD_IDS = {"secant": 0, "newton": 1, "newton_fd": 2, "newton_fd1": 3, "newton_safe": 4}
+PSTAR_BRACKET_IDS = {"legacy": 0, "hllc": 1}
+
+
 class Scheme2D(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int32), ("limiter", ctypes.c_int32), ("upwind", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("emf_mode", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("atmo_rho", ctypes.c_double),
-                ("atmo_eps", ctypes.c_double)]
+                ("atmo_eps", ctypes.c_double), ("hlld", HlldOpts)]
 
 
 class Tape2D(ctypes.Structure):
@@ -77,6 +86,9 @@ _SIGS = {
     "srmhd_flux_workspace_bytes": (_I64, [ctypes.c_int, _I64]),
     "srmhd_flux": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _I64, _PP, _PP, ctypes.POINTER(Eos),
                                   _PP, _PP, _P, _P, _P, _P, _I64, _P]),
+    "srmhd_flux_ex": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _I64, _PP, _PP,
+                                     ctypes.POINTER(Eos), ctypes.POINTER(HlldOpts), _PP, _PP, _P,
+                                     _P, _P, _P, _I64, _P]),
     "srmhd_c2p_workspace_bytes": (_I64, [_I64]),
     "srmhd_c2p": (ctypes.c_int, [_I64, _PP, ctypes.POINTER(Eos), _D, _D, _PP, _P, _P, _I64, _P]),
     "srmhd_p2c": (ctypes.c_int, [_I64, _PP, _PP, _P]),
@@ -105,6 +117,9 @@ _SIGS = {
     "srmhd1d_rk2_step": (ctypes.c_int, [ctypes.POINTER(Grid1D), ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(Eos), _D, _P, _PP, _PP, _PP, _PP, _P, _P,
                                         _I64, _P]),
+    "srmhd1d_rk2_step_ex": (ctypes.c_int, [ctypes.POINTER(Grid1D), ctypes.c_int, ctypes.c_int,
+                                           ctypes.POINTER(Eos), ctypes.POINTER(HlldOpts), _D, _P,
+                                           _PP, _PP, _PP, _PP, _P, _P, _I64, _P]),
     "srmhd1d_dt": (ctypes.c_int, [ctypes.POINTER(Grid1D), ctypes.POINTER(Eos), _D, ctypes.c_int,
                                   _PP, _P, _P]),
     "srmhd_tubes_workspace_bytes": (_I64, [ctypes.c_int, _I64]),

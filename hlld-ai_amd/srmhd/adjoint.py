@@ -42,9 +42,11 @@ class Rollout2D:
     def __init__(self, grid, eos, *, solver="hlld", limiter="mc", bc_x="outflow",
                  bc_y="outflow", upwind=True, scheme="rk2", checkpoint_every=20,
                  atmo_rho=1e-10, atmo_eps=1e-10, device=None):
+        # the adjoint differentiates the reference's default p* root, so the
+        # rollout keeps the default options whatever srmhd.hlld's switches say
         self.st = Stepper2D(grid, eos, solver=solver, limiter=limiter, bc_x=bc_x, bc_y=bc_y,
                             upwind=upwind, atmo_rho=atmo_rho, atmo_eps=atmo_eps, device=device,
-                            precision="fp64")
+                            precision="fp64", hlld_opts_from_globals=False)
         self.grid = grid
         self.device = self.st.device
         self.lib = self.st.lib

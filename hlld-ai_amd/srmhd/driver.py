@@ -83,12 +83,14 @@ class Stepper1D:
         ap, kp = _lib.ptrs(P)
         aco, kco = _lib.ptrs(Co)
         apo, kpo = _lib.ptrs(Po)
-        rc = self.lib.srmhd1d_rk2_step(ctypes.byref(self.cg), _lib.SOLVER_IDS[self.solver],
-                                       _lib.LIMITER_IDS[self.limiter], ctypes.byref(self.ceos),
-                                       self.atmo_rho, self.dt.data_ptr(), ac, ap, aco, apo,
-                                       self.diag.data_ptr(), self.ws.data_ptr(), self.ws.numel(),
-                                       _lib.stream_ptr(d))
-        _lib.check(rc, "srmhd1d_rk2_step")
+        from . import hlld as H
+        o = H.hlld_opts()   # the p* switches at call time, as hlld_flux reads them
+        rc = self.lib.srmhd1d_rk2_step_ex(ctypes.byref(self.cg), _lib.SOLVER_IDS[self.solver],
+                                          _lib.LIMITER_IDS[self.limiter], ctypes.byref(self.ceos),
+                                          ctypes.byref(o), self.atmo_rho, self.dt.data_ptr(), ac,
+                                          ap, aco, apo, self.diag.data_ptr(), self.ws.data_ptr(),
+                                          self.ws.numel(), _lib.stream_ptr(d))
+        _lib.check(rc, "srmhd1d_rk2_step_ex")
         return Co, Po
 
 

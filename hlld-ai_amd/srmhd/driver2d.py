@@ -35,7 +35,7 @@ class Stepper2D:
 
     def __init__(self, grid, eos, *, solver="hlld", limiter="mc", bc_x="outflow",
                  bc_y="outflow", upwind=True, atmo_rho=1e-10, atmo_eps=1e-10, device=None,
-                 precision="fp64", emf_mode="solver"):
+                 precision="fp64", emf_mode="solver", hlld_opts_from_globals=True):
         self.grid = grid
         self.device = torch.device(device) if device is not None else grid.device
         if self.device.type != "cuda":
@@ -59,6 +59,9 @@ class Stepper2D:
                                 1 if upwind else 0, _lib.PRECISION_IDS[precision],
                                 _lib.EMF_MODE_IDS[emf_mode], 0, float(atmo_rho),
                                 float(atmo_eps))
+        # HLLD's p* switches: srmhd.hlld's module globals, read at every stage
+        # like the reference's hlld_flux does (False: the default root, fixed)
+        self.hlld_opts_from_globals = bool(hlld_opts_from_globals)
         nb = self.lib.srmhd2d_workspace_bytes(ctypes.byref(self.cg), ctypes.byref(self.cs))
         if nb < 0:
             raise ValueError("grid / scheme rejected by srmhd2d_workspace_bytes")
@@ -78,7 +81,18 @@ class Stepper2D:
         return [torch.empty(g.nxt, g.nyt, dtype=torch.float64, device=d) for _ in PRIM6]
 
     # ── C-ABI calls ─────────────────────────────────────────────────────────
+    def _refresh_hlld_opts(self):
+        if not (self.hlld_opts_from_globals and self.solver == "hlld"):
+            return
+        from . import hlld as H
+        o = H.hlld_opts()
+        if self.precision != "fp64" and not H.hlld_opts_default(o):
+            raise ValueError(f"precision {self.precision!r} has its own p* root: the p* switches "
+                             "(_PSTAR_METHOD, _PSTAR_BRACKET, _PSTAR_TOL) apply to fp64 only")
+        self.cs.hlld = o
+
     def stage(self, a, b, c, U0, U, P, out, Pout):
+        self._refresh_hlld_opts()
         a0, k0 = _lib.ptrs(U0)
         a1, k1 = _lib.ptrs(U)
         ap, kp = _lib.ptrs(P)

@@ -9,9 +9,17 @@ device tensors; F, U: dicts D, tau, Sx, Sy, Sz, Bx, By, Bz; p_star < 0 marks
 an HLLE fallback.  ``LAST_DIAG`` is refreshed by every hlld_flux call like
 the reference's module global (hlld.py:1516-1529); its values are read from
 the device on first access, so a call does not synchronise the stream.
+
+HLLD's star-pressure switches are the reference's module globals, read at
+every call (hlld.py:1065-1092) by hlld_flux and by the 1D and 2D drivers:
+``_PSTAR_METHOD`` (from RMHD_PSTAR_METHOD: secant, newton, newton_fd,
+newton_fd1, newton_safe), ``_PSTAR_BRACKET`` (legacy, hllc), ``_PSTAR_TOL``
+and ``_PSTAR_FD_H`` (from RMHD_PSTAR_FD_H; None: the method's default).
+An unknown method raises ValueError, as the reference does.
 """
 
 import ctypes
+import os
 
 import torch
 
@@ -20,6 +28,31 @@ from .eos import as_c
 
 PRIM_KEYS = ("rho", "vx", "vy", "vz", "p", "eps", "Bx", "By", "Bz")
 CONS_KEYS = ("D", "tau", "Sx", "Sy", "Sz", "Bx", "By", "Bz")
+
+_PSTAR_BRACKET = "legacy"
+_PSTAR_TOL = 1.0e-10
+_PSTAR_METHOD = os.environ.get("RMHD_PSTAR_METHOD", "secant")
+_PSTAR_FD_H = (float(os.environ["RMHD_PSTAR_FD_H"]) if os.environ.get("RMHD_PSTAR_FD_H")
+               else None)
+
+
+def hlld_opts():
+    """The current p* switches as the C ABI's srmhd_hlld_opts (ValueError on
+    an unknown method or bracket, hlld.py:1342-1344)."""
+    if _PSTAR_METHOD not in _lib.PSTAR_METHOD_IDS:
+        raise ValueError("RMHD_PSTAR_METHOD=%r: expected secant, newton, newton_safe, newton_fd "
+                         "or newton_fd1" % (_PSTAR_METHOD,))
+    if _PSTAR_BRACKET not in _lib.PSTAR_BRACKET_IDS:
+        raise ValueError(f"_PSTAR_BRACKET={_PSTAR_BRACKET!r}: expected legacy or hllc")
+    if not float(_PSTAR_TOL) > 0.0:
+        raise ValueError(f"_PSTAR_TOL must be positive, got {_PSTAR_TOL!r}")
+    return _lib.HlldOpts(_lib.PSTAR_METHOD_IDS[_PSTAR_METHOD],
+                         _lib.PSTAR_BRACKET_IDS[_PSTAR_BRACKET], float(_PSTAR_TOL),
+                         float(_PSTAR_FD_H) if _PSTAR_FD_H else 0.0)
+
+
+def hlld_opts_default(o):
+    return o.pstar_method == 0 and o.pstar_bracket == 0 and o.pstar_tol == 1.0e-10
 
 
 class _LazyDiag(dict):
@@ -114,13 +147,14 @@ def _flux(solver, sL, sR, eos, idir, want_status=False):
     a_f, keep_f = _lib.ptrs(F)
     a_u, keep_u = _lib.ptrs(U)
     e = as_c(eos)
+    o = hlld_opts() if solver == "hlld" else _lib.HlldOpts(0, 0, 1.0e-10, 0.0)
     with torch.cuda.device(dev):
-        rc = lib.srmhd_flux(sid, idir, n, a_l, a_r, ctypes.byref(e), a_f, a_u, ps.data_ptr(),
-                            st.data_ptr(), diag.data_ptr(), ws.data_ptr(), ws.numel(),
-                            _lib.stream_ptr(dev))
-    _lib.check(rc, f"srmhd_flux({solver})")
+        rc = lib.srmhd_flux_ex(sid, idir, n, a_l, a_r, ctypes.byref(e), ctypes.byref(o), a_f,
+                               a_u, ps.data_ptr(), st.data_ptr(), diag.data_ptr(), ws.data_ptr(),
+                               ws.numel(), _lib.stream_ptr(dev))
+    _lib.check(rc, f"srmhd_flux_ex({solver})")
     if solver == "hlld":
-        LAST_DIAG._set_pending(diag, "hlld", idir)
+        LAST_DIAG._set_pending(diag, "hlld", idir, pstar_method=_PSTAR_METHOD)
     Fd = {k: F[i].reshape(shape) for i, k in enumerate(CONS_KEYS)}
     Ud = {k: U[i].reshape(shape) for i, k in enumerate(CONS_KEYS)}
     out = (Fd, Ud, ps.reshape(shape))

@@ -54,19 +54,32 @@ def rotor_tapered(grid, eos, r0=0.1, r1=0.115, rho_in=10.0, rho_out=1.0, omega=9
     return P, B0 * _corners(grid)[1]
 
 
+def libm(fn, v):
+    """fn (math.sin or math.cos, the C library's) of every element of the 1-D
+    float64 tensor v.  torch's vectorised CPU sin/cos round differently on
+    different CPUs; the C library gives the same values on every host."""
+    return torch.tensor([fn(a) for a in v.tolist()], dtype=torch.float64)
+
+
 def orszag_tang_2pi(grid, eos, A=1.0, vmax=0.99, rho=1.0, press=1.0):
     """Relativistic Orszag-Tang on [0, 2 pi]^2: rho = p = 1,
     v = vmax/sqrt2 (-sin y, sin x, 0), Az = A (cos y + cos(2x)/2), so that
-    B = A (-sin y, sin 2x, 0)."""
-    X, Y = grid.X, grid.Y
+    B = A (-sin y, sin 2x, 0).  The sines and cosines are taken on the 1-D
+    coordinate vectors with the C library (the same on every host) and
+    broadcast."""
+    nx, ny = grid.nxt, grid.nyt
+    sx = libm(math.sin, grid.x)[:, None].expand(nx, ny)
+    sy = libm(math.sin, grid.y)[None, :].expand(nx, ny)
+    s2x = libm(math.sin, 2.0 * grid.x)[:, None].expand(nx, ny)
     amp = vmax / math.sqrt(2.0)
     z = torch.zeros(grid.shape, dtype=torch.float64)
     P = {"rho": torch.full_like(z, rho), "p": torch.full_like(z, press),
-         "vx": -amp * torch.sin(Y), "vy": amp * torch.sin(X), "vz": z.clone(),
-         "Bx": -A * torch.sin(Y), "By": A * torch.sin(2.0 * X), "Bz": z.clone()}
+         "vx": -amp * sy, "vy": amp * sx, "vz": z.clone(),
+         "Bx": -A * sy, "By": A * s2x, "Bz": z.clone()}
     P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
-    XF, YF = _corners(grid)
-    return P, A * (torch.cos(YF) + 0.5 * torch.cos(2.0 * XF))
+    cyf = libm(math.cos, grid.yf)[None, :].expand(nx + 1, ny + 1)
+    c2xf = libm(math.cos, 2.0 * grid.xf)[:, None].expand(nx + 1, ny + 1)
+    return P, A * (cyf + 0.5 * c2xf)
 
 
 def orszag_tang(grid, eos, v_max=0.99, B0=None, press=None, rho=None):

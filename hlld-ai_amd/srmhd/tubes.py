@@ -66,6 +66,17 @@ def draw_tubes(n_tubes, seed=0, lmin=128, lmax=2048):
     return lengths, sides[0], sides[1], u(-1.0, 1.0)
 
 
+def _default_pstar_only(solver):
+    """The fused tube kernels solve HLLD's p* with the reference's default
+    switches only (secant, legacy bracket, tolerance 1e-10)."""
+    if solver != "hlld":
+        return
+    from . import hlld as H
+    if not H.hlld_opts_default(H.hlld_opts()):
+        raise ValueError("the batched tubes take HLLD with the default p* switches only "
+                         "(_PSTAR_METHOD 'secant', _PSTAR_BRACKET 'legacy', _PSTAR_TOL 1e-10)")
+
+
 def _check_lib_rc(rc, what):
     _lib.check(rc, what)
 
@@ -161,15 +172,20 @@ class TubeBatch:
 
     # ── stepping ──────────────────────────────────────────────────────────
     def run(self, steps, *, cfl=0.4, scheme="rk3", block=None, stream=None):
-        """Advance every tube ``steps`` steps with its own dt in one launch."""
+        """Advance every tube ``steps`` steps with its own dt.  block: 64,
+        128, 256 or 512 -- one persistent launch, a workgroup per tube; 0 --
+        the phased form, one launch per phase over every tube (bit-identical);
+        None -- the batch's tuned choice."""
         if scheme not in ("rk2", "rk3"):
             raise ValueError("scheme must be 'rk2' or 'rk3'")
+        _default_pstar_only(self.solver)
         s = stream if stream is not None else _lib.stream_ptr(self.device)
         rc = self.lib.srmhd_tubes_run(
             self.n_tubes, self.total_cells, ctypes.byref(self.layout), ctypes.byref(self.ceos),
             _lib.SOLVER_IDS[self.solver], _lib.LIMITER_IDS[self.limiter],
             1 if scheme == "rk3" else 0, int(steps), float(cfl), self.atmo_rho,
-            int(block or self.block), self._uptr[0], self._pptr[0], self.t.data_ptr(),
+            int(self.block if block is None else block), self._uptr[0], self._pptr[0],
+            self.t.data_ptr(),
             self.ws.data_ptr(), self.ws.numel(), s)
         _check_lib_rc(rc, "srmhd_tubes_run")
 
@@ -223,10 +239,11 @@ def _shape_key(batch, scheme):
     return (batch.n_tubes, batch.total_cells, int(L.max()), batch.limiter, batch.solver, scheme)
 
 
-def autotune_block(batch, *, steps=3, cfl=0.4, scheme="rk3", candidates=(64, 128, 256, 512),
-                   repeats=2):
-    """Time ``steps`` steps at each workgroup size on the batch itself (its
-    state is restored afterwards), keep the fastest, cache it per shape
+def autotune_block(batch, *, steps=3, cfl=0.4, scheme="rk3",
+                   candidates=(0, 64, 128, 256, 512), repeats=2):
+    """Time ``steps`` steps in each launch form on the batch itself (its
+    state is restored afterwards): the persistent kernel at each workgroup
+    size, and the phased form (0); keep the fastest, cache it per shape
     (tube count, total cells, longest tube, limiter, solver, scheme).
     Returns (best block, {block: ms})."""
     key = _shape_key(batch, scheme)
@@ -300,6 +317,7 @@ def run_tubes(UL, UR, Bn, eos, *, ncells=192, tend=0.18, cfl=0.3, limiter="mc", 
     Bz), ``Bn`` is (N,).  Returns the primitive profiles, each an
     (ncells, N) tensor with the ghosts stripped, and t.  The loop runs on the
     device: with ``graph`` the ``max_steps`` iterations are one HIP graph."""
+    _default_pstar_only(solver_name(flux))
     lib = _lib.lib()
     d = torch.device(device)
     if ng is None:

@@ -70,6 +70,27 @@ enum { SRMHD_PCM = 0, SRMHD_MC = 1, SRMHD_MINMOD = 2, SRMHD_WENO5Z = 3, SRMHD_MP
        SRMHD_MP7 = 5 };
 enum { SRMHD_BC_OUTFLOW = 0, SRMHD_BC_PERIODIC = 1 };
 
+/* ── HLLD p* options ─────────────────────────────────────────────────────────
+ * The reference's hlld_flux reads four module globals at call time
+ * (src/physics/hlld.py:1065-1092): the root finder of the star pressure
+ * (_PSTAR_METHOD, from RMHD_PSTAR_METHOD: "secant", the production clamped
+ * secant, or one of the four newton_pstar forms, hlld.py:751-904), the
+ * bracket source (_PSTAR_BRACKET: "legacy", the con2prim of the HLL state,
+ * or "hllc", the HLLC star pressure _hllc_pstar, hlld.py:1104-1199), the
+ * residual tolerance (_PSTAR_TOL, 1e-10) and the relative step of the finite
+ * differences (_PSTAR_FD_H, from RMHD_PSTAR_FD_H; None = 1.5e-8 for
+ * newton_fd1, 1e-6 for newton_fd).  A zeroed struct (or NULL) is the
+ * reference's default: secant, legacy, 1e-10. */
+enum { SRMHD_PSTAR_SECANT = 0, SRMHD_PSTAR_NEWTON = 1, SRMHD_PSTAR_NEWTON_FD = 2,
+       SRMHD_PSTAR_NEWTON_FD1 = 3, SRMHD_PSTAR_NEWTON_SAFE = 4 };
+enum { SRMHD_BRACKET_LEGACY = 0, SRMHD_BRACKET_HLLC = 1 };
+typedef struct srmhd_hlld_opts {
+  int32_t pstar_method;    /* SRMHD_PSTAR_* */
+  int32_t pstar_bracket;   /* SRMHD_BRACKET_* */
+  double pstar_tol;        /* <= 0: 1e-10 */
+  double pstar_fd_h;       /* <= 0: the method's default */
+} srmhd_hlld_opts;
+
 /* ── point-wise flux: the flux_fn plugin ──────────────────────────────────────
  * Replaces hlle_flux / hllc_flux / hlld_flux (src/physics/hlld.py:1005, 1549,
  * 1202) behind the contract flux_fn(sL, sR, eos, idir) -> (F, U, p_star)
@@ -84,6 +105,16 @@ int srmhd_flux(int solver, int idir, int64_t n, const double* const* primL,
                const double* const* primR, const srmhd_eos* eos, double* const* F,
                double* const* U, double* pstar, uint8_t* status, srmhd_flux_diag* diag,
                void* work, int64_t work_bytes, void* stream);
+
+/* srmhd_flux with the HLLD p* options (hlld_flux under the reference's
+ * _PSTAR_METHOD / _PSTAR_BRACKET / _PSTAR_TOL / _PSTAR_FD_H, hlld.py:1065-1092,
+ * 1246-1343); opts NULL = srmhd_flux.  The options concern HLLD only.  The
+ * workspace is srmhd_flux_workspace_bytes(solver, n) whatever the options. */
+int srmhd_flux_ex(int solver, int idir, int64_t n, const double* const* primL,
+                  const double* const* primR, const srmhd_eos* eos,
+                  const srmhd_hlld_opts* opts, double* const* F, double* const* U,
+                  double* pstar, uint8_t* status, srmhd_flux_diag* diag, void* work,
+                  int64_t work_bytes, void* stream);
 
 /* ── con2prim ────────────────────────────────────────────────────────────────
  * Replaces conservative_to_primitive (src/physics/c2p.py:313-363) over one
@@ -137,10 +168,14 @@ enum {
  * adjoint entry points take SRMHD_EMF_SOLVER only. */
 enum { SRMHD_EMF_SOLVER = 0, SRMHD_EMF_HLL = 1 };
 
+/* hlld: the HLLD p* options (zeroed: the reference's default); non-default
+ * methods and the hllc bracket are taken by the FP64 precision only (the
+ * other forms and the adjoint entry points refuse them, SRMHD_EARG). */
 typedef struct srmhd_scheme2d {
   int32_t solver, limiter, upwind, precision;
   int32_t emf_mode, reserved;   /* reserved: 0 */
   double atmo_rho, atmo_eps;
+  srmhd_hlld_opts hlld;
 } srmhd_scheme2d;
 
 int64_t srmhd2d_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s);
@@ -253,6 +288,13 @@ int srmhd1d_rk2_step(const srmhd_grid1d* g, int solver, int limiter, const srmhd
                      const double* const* prim, double* const* cons_out, double* const* prim_out,
                      srmhd_flux_diag* diag2, void* work, int64_t work_bytes, void* stream);
 
+/* srmhd1d_rk2_step with the HLLD p* options (NULL = srmhd1d_rk2_step). */
+int srmhd1d_rk2_step_ex(const srmhd_grid1d* g, int solver, int limiter, const srmhd_eos* eos,
+                        const srmhd_hlld_opts* opts, double atmo_rho, const double* dt_dev,
+                        const double* const* cons, const double* const* prim,
+                        double* const* cons_out, double* const* prim_out,
+                        srmhd_flux_diag* diag2, void* work, int64_t work_bytes, void* stream);
+
 /* compute_dt (src/physics/driver.py:77-127) into *dt_dev. */
 int srmhd1d_dt(const srmhd_grid1d* g, const srmhd_eos* eos, double cfl, int speed_of_light,
                const double* const* prim, double* dt_dev, void* stream);
@@ -287,7 +329,12 @@ int64_t srmhd_tubes_workspace_bytes(int n_tubes, int64_t total_cells);
  * con2prim batch = the tube's faces), L = -(F[j+1] - F[j]) / dx, con2prim of
  * every cell of the tube as one batch, two-cell-copy ghosts (which also reach
  * U's By, Bz, tube_seed.py:72-75,105-108).  t_out[t] += the tube's elapsed
- * time.  block: threads per workgroup, 64, 128, 256 or 512. */
+ * time.  block: 64, 128, 256 or 512 -- one persistent launch, a workgroup of
+ * that many threads per tube walking its tube phase by phase; or 0 -- the
+ * phased form, one launch per phase over every tube's faces or cells
+ * (2 + 4 x stages launches a step, one graph when captured), with the same
+ * arithmetic and batches, bit for bit.  HLLD runs with the reference's
+ * default p* options (secant, legacy bracket, 1e-10). */
 int srmhd_tubes_run(int n_tubes, int64_t total_cells, const srmhd_tube_layout* layout,
                     const srmhd_eos* eos, int solver, int limiter, int rk3, int steps, double cfl,
                     double atmo_rho, int block, double* const* U, double* const* P, double* t_out,

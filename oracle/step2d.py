@@ -6,6 +6,7 @@ from dataclasses import dataclass, field
 
 import torch
 
+from . import physics as PH
 from .c2p import cons_to_prim
 from .grid import Grid2D
 from .physics import hll_flux, prim_to_cons, sqrt, srmhd_fluxes
@@ -294,16 +295,34 @@ def rotor_tapered(grid, eos, r0=0.1, r1=0.115, rho_in=10.0, rho_out=1.0, omega=9
 def orszag_tang_2pi(grid, eos, A=1.0, vmax=0.99, rho=1.0, press=1.0):
     """Relativistic Orszag-Tang on [0, 2 pi]^2: rho = p = 1,
     v = vmax/sqrt2 (-sin y, sin x, 0), Az = A (cos y + cos(2x)/2) so that
-    B = A (-sin y, sin 2x, 0)."""
+    B = A (-sin y, sin 2x, 0).  In the reference's rounding mode
+    (physics.SQRT_MODE "torch") sin and cos are torch's CPU ones, as the
+    reference evaluates them; otherwise the C library's on the 1-D
+    coordinates (torch's vectorised CPU sin/cos round differently on
+    different CPUs), as the device path's initialiser does."""
     X, Y = grid.X, grid.Y
     amp = vmax / math.sqrt(2.0)
     z = torch.zeros(grid.shape, dtype=torch.float64)
-    P = {"rho": torch.full_like(z, rho), "p": torch.full_like(z, press),
-         "vx": -amp * torch.sin(Y), "vy": amp * torch.sin(X), "vz": z.clone(),
-         "Bx": -A * torch.sin(Y), "By": A * torch.sin(2.0 * X), "Bz": z.clone()}
-    P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
     XF, YF = _corners(grid)
-    return P, A * (torch.cos(YF) + 0.5 * torch.cos(2.0 * XF))
+    if PH.SQRT_MODE == "torch":
+        # the reference host's own torch CPU sin/cos (the pinning mode)
+        sy, sx, s2x = torch.sin(Y), torch.sin(X), torch.sin(2.0 * X)
+        cyf, c2xf = torch.cos(YF), torch.cos(2.0 * XF)
+    else:
+        # the C library's, on the 1-D coordinates: the same on every host
+        def libm(fn, v):
+            return torch.tensor([fn(a) for a in v.tolist()], dtype=torch.float64)
+        nx, ny = grid.nxt, grid.nyt
+        sx = libm(math.sin, grid.x)[:, None].expand(nx, ny)
+        sy = libm(math.sin, grid.y)[None, :].expand(nx, ny)
+        s2x = libm(math.sin, 2.0 * grid.x)[:, None].expand(nx, ny)
+        cyf = libm(math.cos, grid.yf)[None, :].expand(nx + 1, ny + 1)
+        c2xf = libm(math.cos, 2.0 * grid.xf)[:, None].expand(nx + 1, ny + 1)
+    P = {"rho": torch.full_like(z, rho), "p": torch.full_like(z, press),
+         "vx": -amp * sy, "vy": amp * sx, "vz": z.clone(),
+         "Bx": -A * sy, "By": A * s2x, "Bz": z.clone()}
+    P["eps"] = eos.eps__press_rho(P["p"], P["rho"])
+    return P, A * (cyf + 0.5 * c2xf)
 
 
 def init_state(grid, eos, prims, Az, bc=("outflow", "outflow")):

@@ -62,19 +62,22 @@ def test_per_tube_bitwise_vs_oracle(limiter, solver, scheme):
     _dev()
     print(f"tubes per-tube {limiter} {solver} {scheme}", flush=True)
     lengths, left, right, Bn = _workload(5, seed=11, lmin=16, lmax=48)
-    b = _batch(lengths, left, right, Bn, limiter, solver)
     steps = 4
-    b.run(steps, cfl=0.4, scheme=scheme)
-    torch.cuda.synchronize()
-    t_dev = b.t.cpu().numpy()
-    for i in range(b.n_tubes):
-        P, t = _oracle_tube(int(lengths[i]), b.ng, _side(left, i), _side(right, i), float(Bn[i]),
-                            steps, limiter, solver, scheme)
-        assert t_dev[i] == t, (i, t_dev[i], t)
-        got = b.tube(i)
-        for k in PRIM9:
-            s = _same(got[k].cpu().numpy(), P[k].numpy())
-            assert s.all(), f"tube {i} {k}: {int((~s).sum())} differ"
+    runs = []
+    for block in (None, 0):        # the persistent kernel and the phased form
+        b = _batch(lengths, left, right, Bn, limiter, solver)
+        b.run(steps, cfl=0.4, scheme=scheme, block=block)
+        torch.cuda.synchronize()
+        runs.append(b)
+    for i in range(runs[0].n_tubes):
+        P, t = _oracle_tube(int(lengths[i]), runs[0].ng, _side(left, i), _side(right, i),
+                            float(Bn[i]), steps, limiter, solver, scheme)
+        for b in runs:
+            assert b.t.cpu().numpy()[i] == t, (i, b.block, t)
+            got = b.tube(i)
+            for k in PRIM9:
+                s = _same(got[k].cpu().numpy(), P[k].numpy())
+                assert s.all(), f"tube {i} {k}: {int((~s).sum())} differ"
 
 
 def test_batch_independence_block_size_and_graph():
@@ -87,7 +90,7 @@ def test_batch_independence_block_size_and_graph():
     ref = _batch(lengths, left, right, Bn, "mp5", "hlld")
     ref.run(3, block=256)
     torch.cuda.synchronize()
-    for block in (64, 128, 512):
+    for block in (64, 128, 512, 0):   # 0: the phased form
         b = _batch(lengths, left, right, Bn, "mp5", "hlld")
         b.run(3, block=block)
         torch.cuda.synchronize()
@@ -123,7 +126,7 @@ def test_autotune_picks_a_valid_block():
     b = _batch(lengths, left, right, Bn, "mp5", "hlld")
     snap = b.snapshot()
     best, times = T.autotune_block(b, steps=1)
-    assert best in (64, 128, 256, 512) and set(times) == {64, 128, 256, 512}
+    assert best in (0, 64, 128, 256, 512) and set(times) == {0, 64, 128, 256, 512}
     for u, v in zip(b.U + b.P, snap[0] + snap[1]):   # state restored
         assert torch.equal(u, v)
     assert T.autotune_block(b, steps=1)[0] == best   # cached
