@@ -1139,6 +1139,44 @@ G1 make_g1(const srmhd_grid1d* g) {
   return r;
 }
 
+// ── the allocation-free flux's scratch ──────────────────────────────────────
+// srmhd_flux_noalloc carves srmhd_flux's workspace (the diag words, then one
+// C2PRec per face) from this array instead of a caller's buffer.  Every call
+// on a device shares that device's copy, so calls that overlap on one device
+// race; the diag words are cleared by a kernel at every call and each record
+// is written by pass A before pass B reads it.  In host code `scratch` names
+// a host-side stand-in: the device address comes from hipGetSymbolAddress,
+// once per device, and only that address reaches a kernel.
+constexpr int64_t NOALLOC_BYTES = 256 + SRMHD_NOALLOC_MAX_FACES * (int64_t)sizeof(C2PRec);
+static_assert(sizeof(srmhd_flux_diag) <= 256, "the diag words fit the first 256 bytes");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+void* g_scratch[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+// the current device's copy of `scratch` (nullptr when the device cannot be
+// named or the symbol lookup fails)
+void* scratch_ptr() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      (void)hipGetLastError();   // reported as the entry point's own error
+      return nullptr;
+    }
+    g_scratch[dev] = p;
+  }
+  return g_scratch[dev];
+}
+
 }  // namespace
 
 // ── C ABI ───────────────────────────────────────────────────────────────────
@@ -1200,6 +1238,27 @@ int srmhd_flux_ex(int solver, int idir, int64_t n, const double* const* primL,
   }
   if (diag) k_copy_diag<<<1, 64, 0, s>>>(d, diag, 1);
   return launch_status();
+}
+
+int srmhd_flux_noalloc(int solver, int idir, int64_t n, const double* const* primL,
+                       const double* const* primR, const srmhd_eos* eos,
+                       const srmhd_hlld_opts* opts, double* const* F, double* const* U,
+                       double* pstar, uint8_t* status, srmhd_flux_diag* diag, void* stream) {
+  if (n > SRMHD_NOALLOC_MAX_FACES) return SRMHD_ESIZE;
+  if (n < 0) return SRMHD_EARG;
+  if (stream) {
+    // the scratch is the current device's: the stream must be on it
+    int cur = 0, sd = 0;
+    if (hipGetDevice(&cur) != hipSuccess ||
+        hipStreamGetDevice((hipStream_t)stream, &sd) != hipSuccess || sd != cur) {
+      (void)hipGetLastError();
+      return SRMHD_EARG;
+    }
+  }
+  void* work = scratch_ptr();
+  if (!work) return SRMHD_EHIP + (int)hipErrorInvalidSymbol;
+  return srmhd_flux_ex(solver, idir, n, primL, primR, eos, opts, F, U, pstar, status, diag, work,
+                       NOALLOC_BYTES, stream);
 }
 
 int64_t srmhd_c2p_workspace_bytes(int64_t n) {

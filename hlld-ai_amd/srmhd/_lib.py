@@ -20,7 +20,9 @@ PRECISION_IDS = {"fp64": 0, "mixed": 1, "mixed_r64": 2, "fp64_local": 3}
 EMF_MODE_IDS = {"solver": 0, "hll": 1}
 
 EARG = 1
+ESIZE = 2
 EHIP = 1000
+NOALLOC_MAX_FACES = 1 << 20
 
 
 class Eos(ctypes.Structure):
@@ -89,6 +91,9 @@ _SIGS = {
     "srmhd_flux_ex": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _I64, _PP, _PP,
                                      ctypes.POINTER(Eos), ctypes.POINTER(HlldOpts), _PP, _PP, _P,
                                      _P, _P, _P, _I64, _P]),
+    "srmhd_flux_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _I64, _PP, _PP,
+                                          ctypes.POINTER(Eos), ctypes.POINTER(HlldOpts), _PP,
+                                          _PP, _P, _P, _P, _P]),
     "srmhd_c2p_workspace_bytes": (_I64, [_I64]),
     "srmhd_c2p": (ctypes.c_int, [_I64, _PP, ctypes.POINTER(Eos), _D, _D, _PP, _P, _P, _I64, _P]),
     "srmhd_p2c": (ctypes.c_int, [_I64, _PP, _PP, _P]),
@@ -169,6 +174,8 @@ def check(rc, what):
         return
     if rc == EARG:
         raise ValueError(f"{what}: invalid argument")
+    if rc == ESIZE:
+        raise ValueError(f"{what}: more than {NOALLOC_MAX_FACES} faces for the fixed scratch")
     raise RuntimeError(f"{what}: HIP error {rc - EHIP}")
 
 

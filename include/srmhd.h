@@ -4,8 +4,9 @@
  * Every entry point takes plain device pointers (float64 structure-of-arrays,
  * contiguous), sizes, and a HIP stream (hipStream_t passed as void*; NULL =
  * the null stream).  The caller owns every buffer, including the workspace
- * whose size the matching *_workspace_bytes() returns; no device memory is
- * allocated inside.  One exception holds host-side HIP objects: the first
+ * whose size the matching *_workspace_bytes() returns (srmhd_flux_noalloc
+ * takes none: it uses a fixed device array of the library's); no device
+ * memory is allocated inside.  One exception holds host-side HIP objects: the first
  * mixed-precision srmhd2d_stage (SRMHD_PREC_MIXED / _MIXED_R64, HLLD) on a
  * device creates one non-blocking stream and two events for that device
  * (default priority), kept until the process ends, to run the FP64 fix-up
@@ -38,6 +39,7 @@ extern "C" {
 
 #define SRMHD_OK 0
 #define SRMHD_EARG 1
+#define SRMHD_ESIZE 2   /* n above SRMHD_NOALLOC_MAX_FACES (srmhd_flux_noalloc) */
 #define SRMHD_EHIP 1000
 
 /* hybrid_eos (src/physics/eos.py:15-20): p = K rho^g + (eps - eps_cold) rho (g_th - 1);
@@ -115,6 +117,24 @@ int srmhd_flux_ex(int solver, int idir, int64_t n, const double* const* primL,
                   const srmhd_hlld_opts* opts, double* const* F, double* const* U,
                   double* pstar, uint8_t* status, srmhd_flux_diag* diag, void* work,
                   int64_t work_bytes, void* stream);
+
+/* srmhd_flux_ex for callers that cannot allocate device memory or hand over
+ * a workspace: code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  No counterpart in the reference.  The workspace is
+ * a fixed device array inside the library (one copy per device, 256 bytes +
+ * 24 bytes per face: 24 MiB + 256 bytes for SRMHD_NOALLOC_MAX_FACES faces,
+ * held until the process ends); n above SRMHD_NOALLOC_MAX_FACES returns
+ * SRMHD_ESIZE.  The scratch used is the CURRENT device's, so a non-NULL
+ * stream must belong to the current device (else SRMHD_EARG).  Every call on
+ * a device shares that device's array: calls that overlap on one device
+ * (concurrent streams, threads) race and must be ordered by the caller.  No
+ * allocation of any kind is made; the results equal srmhd_flux_ex's on the
+ * same inputs, bit for bit. */
+#define SRMHD_NOALLOC_MAX_FACES (1 << 20)
+int srmhd_flux_noalloc(int solver, int idir, int64_t n, const double* const* primL,
+                       const double* const* primR, const srmhd_eos* eos,
+                       const srmhd_hlld_opts* opts, double* const* F, double* const* U,
+                       double* pstar, uint8_t* status, srmhd_flux_diag* diag, void* stream);
 
 /* ── con2prim ────────────────────────────────────────────────────────────────
  * Replaces conservative_to_primitive (src/physics/c2p.py:313-363) over one
