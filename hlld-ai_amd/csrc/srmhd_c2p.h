@@ -25,6 +25,12 @@ namespace srmhd {
 constexpr double C2P_TOL = 1e-15;
 constexpr int C2P_SECANT = 60;
 constexpr int C2P_BISECT = 200;
+constexpr uint64_t C2P_ALL = (((uint64_t)1) << C2P_SECANT) - 1;   // every mask bit
+
+// bitwise equality (NaNs of one pattern equal, +0 and -0 differ)
+SRMHD_DEV bool same_bits(double a, double b) {
+  return __builtin_bit_cast(unsigned long long, a) == __builtin_bit_cast(unsigned long long, b);
+}
 
 template <class T>
 struct KastaunT {
@@ -183,7 +189,19 @@ struct KastaunT {
     f1 = f2;
   }
 
-  // 200-step bisection continuation   c2p.py:238-266 -> (mu, |f| at mu)
+  // secant_step, returning whether it left (x0, f0, x1, f1) unchanged bit for
+  // bit: the walk depends on that state only, so it is then at a fixed point
+  // and every later step leaves it unchanged too
+  SRMHD_DEV bool secant_step_fixed(double& x0, double& f0, double& x1, double& f1) const {
+    const double a0 = x0, b0 = f0, a1 = x1, b1 = f1;
+    secant_step(x0, f0, x1, f1);
+    return same_bits(a0, x0) && same_bits(b0, f0) && same_bits(a1, x1) && same_bits(b1, f1);
+  }
+
+  // 200-step bisection continuation   c2p.py:238-266 -> (mu, |f| at mu).
+  // Once the bracket has shrunk to adjacent doubles (about 55 halvings for a
+  // root of order one) the midpoint is an end point and a step maps the state
+  // to itself; the loop stops there, with the 200-step result.
   SRMHD_DEV double bisection(double mup, double& fmu) const {
     double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
     double f0 = master(x0), f1 = master(x1);
@@ -191,13 +209,17 @@ struct KastaunT {
     for (int it = 0; it < C2P_BISECT; ++it) {
       const double xm = 0.5 * (x0 + x1);
       const double fm = master(xm);
+      bool fixed;
       if ((f0 * fm) <= 0.0) {
+        fixed = same_bits(x1, xm) && same_bits(f1, fm);
         x1 = xm;
         f1 = fm;
       } else {
+        fixed = same_bits(x0, xm) && same_bits(f0, fm);
         x0 = xm;
         f0 = fm;
       }
+      if (fixed) break;
     }
     const bool pick = fabs(f0) < fabs(f1);
     fmu = pick ? f0 : f1;
@@ -246,12 +268,68 @@ SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRec& rec) {
   uint64_t mask = 0;
 #pragma unroll 1
   for (int it = 0; it < C2P_SECANT; ++it) {
-    if (!(fabs(f1) < C2P_TOL)) mask |= (uint64_t)1 << it;
-    k.secant_step(x0, f0, x1, f1);
+    const bool nc = !(fabs(f1) < C2P_TOL);
+    if (nc) mask |= (uint64_t)1 << it;
+    if (k.secant_step_fixed(x0, f0, x1, f1)) {
+      // a fixed point: every later step sees this state, so its bits repeat
+      if (nc) mask |= C2P_ALL & ~(((uint64_t)2 << it) - 1);
+      break;
+    }
   }
   rec.mup_signed = good ? mup : -mup;
   rec.x1 = x1;
   rec.f1 = f1;
+  return mask;
+}
+
+// Pass-A record that also keeps the secant state at the element's own first
+// convergence, step k (the first step with |f| < tol; k = 60 when there is
+// none).  The batch's break iteration K is the first step at which every
+// element has converged, so K >= k: a batch that breaks early (K < 60)
+// replays K - k steps from that state instead of K from the start, and one
+// that never breaks (K = 60) takes the 60-step state as it is.  For batches
+// that often break early (a tube's cells and faces).  The walk depends only
+// on (x0, f0, x1, f1), so the replay is bit-identical.
+struct C2PRecK {
+  double mup_signed, x1, f1;       // as C2PRec: the 60-step state
+  double kx0, kf0, kx1, kf1;       // the state after k steps
+  int32_t k, pad;
+};
+
+SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRecK& rec) {
+  const double mup = k.mu_plus_fast();
+  double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
+  double f0 = k.master(x0), f1 = k.master(x1);
+  const bool good = (f0 * f1) < 0.0;
+  uint64_t mask = 0;
+  int ke = C2P_SECANT;
+  double s0 = 0.0, g0 = 0.0, s1 = 0.0, g1 = 0.0;
+#pragma unroll 1
+  for (int it = 0; it < C2P_SECANT; ++it) {
+    const bool nc = !(fabs(f1) < C2P_TOL);
+    if (nc) {
+      mask |= (uint64_t)1 << it;
+    } else if (ke == C2P_SECANT) {
+      ke = it;
+      s0 = x0;
+      g0 = f0;
+      s1 = x1;
+      g1 = f1;
+    }
+    if (k.secant_step_fixed(x0, f0, x1, f1)) {
+      if (nc) mask |= C2P_ALL & ~(((uint64_t)2 << it) - 1);
+      break;
+    }
+  }
+  rec.mup_signed = good ? mup : -mup;
+  rec.x1 = x1;
+  rec.f1 = f1;
+  rec.kx0 = s0;
+  rec.kf0 = g0;
+  rec.kx1 = s1;
+  rec.kf1 = g1;
+  rec.k = ke;
+  rec.pad = 0;
   return mask;
 }
 
@@ -275,13 +353,30 @@ SRMHD_DEV void c2p_secant_at(const Kastaun& k, const C2PRec& rec, int K, double&
     double f0 = k.master(x0);
     f1 = k.master(x1);
 #pragma unroll 1
-    for (int it = 0; it < K; ++it) k.secant_step(x0, f0, x1, f1);
+    for (int it = 0; it < K; ++it)
+      if (k.secant_step_fixed(x0, f0, x1, f1)) break;
+  }
+}
+
+// the same from the first-convergence record (K >= rec.k)
+SRMHD_DEV void c2p_secant_at(const Kastaun& k, const C2PRecK& rec, int K, double& x1,
+                             double& f1) {
+  x1 = rec.x1;
+  f1 = rec.f1;
+  if (K < C2P_SECANT) {
+    double x0 = rec.kx0, f0 = rec.kf0;
+    x1 = rec.kx1;
+    f1 = rec.kf1;
+#pragma unroll 1
+    for (int it = rec.k; it < K; ++it)
+      if (k.secant_step_fixed(x0, f0, x1, f1)) break;
   }
 }
 
 // whether the element goes to the bisection redo: not converged after the
 // batch's K secant steps but with a good bracket   c2p.py:338
-SRMHD_DEV bool c2p_needs_redo(const Kastaun& k, const C2PRec& rec, int K) {
+template <class Rec>
+SRMHD_DEV bool c2p_needs_redo(const Kastaun& k, const Rec& rec, int K) {
   double x1, f1;
   c2p_secant_at(k, rec, K, x1, f1);
   return !(fabs(f1) < C2P_TOL) && rec.mup_signed > 0.0;
@@ -292,7 +387,8 @@ SRMHD_DEV bool c2p_needs_redo(const Kastaun& k, const C2PRec& rec, int K) {
 // force < 0: the element is redone by bisection when it needs it itself;
 // force >= 0: the caller decides (force != 0 redoes), for batches whose redo
 // set is wider than the elements that need it.
-SRMHD_DEV int c2p_pass_b(const Kastaun& k, const C2PRec& rec, int K, double atmo_rho,
+template <class Rec>
+SRMHD_DEV int c2p_pass_b(const Kastaun& k, const Rec& rec, int K, double atmo_rho,
                          double atmo_eps, Prim& P, int force = -1) {
   const bool good = rec.mup_signed > 0.0;
   const double mup = fabs(rec.mup_signed);
@@ -340,7 +436,7 @@ SRMHD_DEV int c2p_single(const Kastaun& k, double atmo_rho, double atmo_eps, Pri
 #pragma unroll 1
   for (int it = 0; it < C2P_SECANT; ++it) {
     if (fabs(f1) < C2P_TOL) break;
-    k.secant_step(x0, f0, x1, f1);
+    if (k.secant_step_fixed(x0, f0, x1, f1)) break;
   }
   bool conv = fabs(f1) < C2P_TOL;
   bool clamped;

@@ -113,12 +113,29 @@ SRMHD_DEV auto sdiv(const A& num, const B& den) -> decltype(num / den) {
 // ── EOS (eos.py:15-76) ──────────────────────────────────────────────────────
 // K*rho**gamma is 0 for K = 0 and finite rho >= 0; the pow is only taken
 // where it can change the result (K != 0, or rho negative / non-finite).
+// K rho^gamma off the inlined path: pow expands to a few hundred
+// instructions, and inlined the compiler evaluates it on every lane of a
+// hot loop even where the K = 0 shortcut below applies to all of them
+__host__ __device__ __noinline__ inline double cold_press_pow(double K, double rho, double g) {
+  return K * pow(rho, g);
+}
+SRMHD_DEV double cold_press(const srmhd_eos& e, double rho) {
+  return cold_press_pow(e.K, rho, e.gamma);
+}
+template <int K>
+SRMHD_DEV Dual<K> cold_press(const srmhd_eos& e, const Dual<K>& rho) {
+  return e.K * tpow(rho, e.gamma);
+}
+
+// 0 * rho^gamma is +0 wherever rho^gamma is finite: rho in [0, 1e100] with
+// 0 <= gamma <= 3 (the shortcut of the reference's K * rho**gamma at K = 0)
 template <class T>
 SRMHD_DEV void eos_cold(const srmhd_eos& e, const T& rho, T& pc, T& ec) {
-  if (e.K == 0.0 && finite_pos_or_zero(val(rho))) {
+  const bool zero_k = e.K == 0.0 && e.gamma >= 0.0 && e.gamma <= 3.0;
+  if (zero_k && val(rho) >= 0.0 && val(rho) <= 1e100) {
     pc = T(0.0);
   } else {
-    pc = e.K * tpow(rho, e.gamma);
+    pc = cold_press(e, rho);
   }
   const T den = rho * (e.gamma - 1.0);
   ec = (val(pc) == 0.0 && val(den) > 0.0 && val(den) <= DBL_MAX) ? T(0.0) : T(pc / den);

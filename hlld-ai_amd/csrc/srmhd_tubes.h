@@ -25,8 +25,8 @@ struct TubeArgs {
   double* U[3][7];        // U[0]: caller's state (in/out); U[1..2]: stage buffers
   double* P[6];
   double* F[7];
-  C2PRec* recf;
-  C2PRec* recc;
+  C2PRecK* recf;
+  C2PRecK* recc;
   TubeState ts;           // the phased form's per-tube words
   int64_t cells;          // all tubes' cells
 };

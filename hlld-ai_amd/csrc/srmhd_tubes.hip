@@ -36,7 +36,7 @@ int dispatch(int lim, int sol, int mode, int block, int n, const TubeArgs& A, co
 struct WsT {
   double* U[2][7];
   double* F[7];
-  C2PRec *recf, *recc;
+  C2PRecK *recf, *recc;
   TubeState ts;
   int64_t bytes;
 };
@@ -48,8 +48,8 @@ WsT carve_tubes(int64_t cells, int64_t faces, void* work) {
   for (int b = 0; b < 2; ++b)
     for (int k = 0; k < 7; ++k) w.U[b][k] = c.take<double>(cells);
   for (int k = 0; k < 7; ++k) w.F[k] = c.take<double>(faces);
-  w.recf = c.take<C2PRec>(faces);
-  w.recc = c.take<C2PRec>(cells);
+  w.recf = c.take<C2PRecK>(faces);
+  w.recc = c.take<C2PRecK>(cells);
   w.ts.dt = c.take<double>(n);
   w.ts.mf = c.take<unsigned long long>(n);
   w.ts.mc = c.take<unsigned long long>(n);

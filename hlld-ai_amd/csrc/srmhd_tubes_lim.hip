@@ -34,8 +34,8 @@ struct Tube {
   double dx, Bn;
   double* P[6];
   double* F[7];
-  C2PRec* recf;
-  C2PRec* recc;
+  C2PRecK* recf;
+  C2PRecK* recc;
   __device__ Tube(const TubeArgs& A) {
     id = A.order ? A.order[blockIdx.x] : (int)blockIdx.x;
     oc = A.off_c[id];
