@@ -23,7 +23,7 @@ LIB = os.path.join(OUT_DIR, "libsrmhd_hip.so")
 # objects build in parallel
 SOURCES = ([("srmhd_kernels.hip", [], "srmhd_kernels.o"), ("srmhd_tubes.hip", [], "srmhd_tubes.o"),
             ("srmhd_mixed.hip", [], "srmhd_mixed.o"), ("srmhd_adjoint.hip", [], "srmhd_adjoint.o"),
-            ("srmhd_pstar.hip", [], "srmhd_pstar.o")] +
+            ("srmhd_pstar.hip", [], "srmhd_pstar.o"), ("srmhd_linear.hip", [], "srmhd_linear.o")] +
            [("srmhd_tubes_lim.hip", [f"-DSRMHD_TUBE_LIM={k}"], f"srmhd_tubes_lim{k}.o")
             for k in range(6)])
 # the mixed-precision sweeps: FP32 contraction and approximate FP32 division
@@ -31,7 +31,7 @@ SOURCES = ([("srmhd_kernels.hip", [], "srmhd_kernels.o"), ("srmhd_tubes.hip", []
 EXTRA = {"srmhd_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
 HEADERS = ["srmhd_common.h", "srmhd_c2p.h", "srmhd_recon.h", "srmhd_riemann.h", "srmhd_launch.h",
            "srmhd_grid2d.h", "srmhd_mixed.h", "srmhd_f32.h", "srmhd_dual.h", "srmhd_pstar.h",
-           "srmhd_pstar_launch.h", "srmhd_tubes.h"]
+           "srmhd_pstar_launch.h", "srmhd_tubes.h", "srmhd_linear.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

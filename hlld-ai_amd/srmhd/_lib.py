@@ -53,6 +53,13 @@ class HlldOpts(ctypes.Structure):
                 ("pstar_tol", ctypes.c_double), ("pstar_fd_h", ctypes.c_double)]
 
 
+class LinearOpts(ctypes.Structure):
+    _fields_ = [("fallback", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("fd_rel", ctypes.c_double), ("imag_tol", ctypes.c_double),
+                ("cond_max", ctypes.c_double), ("hlld", HlldOpts)]
+
+
+LINEAR_FALLBACK_IDS = {"hlld": 0, "hlle": 1, "hllc": 2}
 PSTAR_METHOD_IDS = {"secant": 0, "newton": 1, "newton_fd": 2, "newton_fd1": 3, "newton_safe": 4}
 PSTAR_BRACKET_IDS = {"legacy": 0, "hllc": 1}
 
@@ -94,6 +101,10 @@ _SIGS = {
     "srmhd_flux_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _I64, _PP, _PP,
                                           ctypes.POINTER(Eos), ctypes.POINTER(HlldOpts), _PP,
                                           _PP, _P, _P, _P, _P]),
+    "srmhd_linear_workspace_bytes": (_I64, [_I64]),
+    "srmhd_linear_flux": (ctypes.c_int, [ctypes.c_int, _I64, _PP, _PP, ctypes.POINTER(Eos),
+                                         ctypes.POINTER(LinearOpts), _PP, _PP, _P, _P, _P, _P,
+                                         _P, _I64, _P]),
     "srmhd_c2p_workspace_bytes": (_I64, [_I64]),
     "srmhd_c2p": (ctypes.c_int, [_I64, _PP, ctypes.POINTER(Eos), _D, _D, _PP, _P, _P, _I64, _P]),
     "srmhd_p2c": (ctypes.c_int, [_I64, _PP, _PP, _P]),

@@ -136,6 +136,40 @@ int srmhd_flux_noalloc(int solver, int idir, int64_t n, const double* const* pri
                        const srmhd_hlld_opts* opts, double* const* F, double* const* U,
                        double* pstar, uint8_t* status, srmhd_flux_diag* diag, void* stream);
 
+/* ── the linearised flux: --solver linear ────────────────────────────────────
+ * Replaces linearised_flux (src/physics/linearised.py:129-237), the same
+ * flux_fn contract: the fallback solver's (F, U, p*) on every face, then on
+ * each face the linearisation answers, the physical flux of the state on the
+ * ray xi = 0 from the 7x7 eigen-decomposition of the mean state's
+ * quasi-linear matrix (Jacobians by central differences with relative step
+ * fd_rel), its conserved state and total pressure.  The faces kept by the
+ * fallback: non-finite Jacobians or a non-positive density or pressure on a
+ * side, a singular dU/dV or eigenvector matrix, complex eigenvalues (any
+ * imaginary part: imag_tol bounds only the rounding the reference allows),
+ * an eigenvector matrix of 2-norm condition >= cond_max, or an unphysical
+ * ray state.  linear_mask (uint8 per face, may be NULL): 1 where the
+ * linearisation answered; n_linear (one device word, may be NULL): their
+ * count; diag, status words of the fallback as srmhd_flux's.  The eigen
+ * decomposition is the library's own (Hessenberg reduction and Francis QR),
+ * so the results agree with the reference's LAPACK ones to rounding. */
+enum { SRMHD_LINEAR_FALLBACK_HLLD = 0, SRMHD_LINEAR_FALLBACK_HLLE = 1,
+       SRMHD_LINEAR_FALLBACK_HLLC = 2 };
+typedef struct srmhd_linear_opts {
+  int32_t fallback;        /* SRMHD_LINEAR_FALLBACK_* (linearised.py:130 fallback=hlld_flux) */
+  int32_t reserved;
+  double fd_rel;           /* <= 0: 1e-5 */
+  double imag_tol;         /* <= 0: 1e-6 */
+  double cond_max;         /* <= 0: 1e12 */
+  srmhd_hlld_opts hlld;    /* the HLLD fallback's p* options (zeroed: the default) */
+} srmhd_linear_opts;
+int64_t srmhd_linear_workspace_bytes(int64_t n);
+int srmhd_linear_flux(int idir, int64_t n, const double* const* primL,
+                      const double* const* primR, const srmhd_eos* eos,
+                      const srmhd_linear_opts* opts, double* const* F, double* const* U,
+                      double* pstar, uint8_t* linear_mask, srmhd_flux_diag* diag,
+                      unsigned long long* n_linear, void* work, int64_t work_bytes,
+                      void* stream);
+
 /* ── con2prim ────────────────────────────────────────────────────────────────
  * Replaces conservative_to_primitive (src/physics/c2p.py:313-363) over one
  * batch of n cells: cons 8 pointers in, prim 9 pointers out, status (uint8:

@@ -137,6 +137,52 @@ def gen_pstar():
     np.savez_compressed(HERE / "pstar_methods.npz", **out)
 
 
+LINEAR_SHRINK = (1.0, 0.1, 0.01)
+
+
+def shrink_pair(sL, sR, s, eos_eps):
+    """The jump of every primitive (rho, p, v, B) scaled by ``s`` about the
+    pair's mean; eps from the EOS."""
+    L2, R2 = {}, {}
+    for k in ("rho", "p", "vx", "vy", "vz", "Bx", "By", "Bz"):
+        m = 0.5 * (sL[k] + sR[k])
+        L2[k] = m + s * (sL[k] - m)
+        R2[k] = m + s * (sR[k] - m)
+    L2["eps"] = eos_eps(L2["p"], L2["rho"])
+    R2["eps"] = eos_eps(R2["p"], R2["rho"])
+    return L2, R2
+
+
+def gen_linear():
+    """linearised_flux (src/physics/linearised.py:129-237, HLLD fallback,
+    default fd_rel / imag_tol / cond_max) on the flux_random interface pairs
+    with their jumps scaled by 1, 0.1 and 0.01 about the mean (the weak
+    problems it is meant for), every direction: F, U, p* and the mask of the
+    faces it answered."""
+    from src.physics.linearised import LAST_LINEAR, linearised_flux
+    g = np.load(HERE / "flux_random.npz")
+    eos = hybrid_eos(K=0.0, gamma=5.0 / 3.0, gamma_th=5.0 / 3.0)
+    out = {}
+    for tag in ("a", "b"):
+        sL = {k: torch.from_numpy(g[f"{tag}_L_{k}"].copy()) for k in PRIM9}
+        sR = {k: torch.from_numpy(g[f"{tag}_R_{k}"].copy()) for k in PRIM9}
+        for si, sc in enumerate(LINEAR_SHRINK):
+            L2, R2 = shrink_pair(sL, sR, sc, eos.eps__press_rho)
+            for k in PRIM9:
+                out[f"{tag}{si}_L_{k}"] = _np(L2[k])
+                out[f"{tag}{si}_R_{k}"] = _np(R2[k])
+            for idir in (0, 1, 2):
+                F, U, ps, ok = linearised_flux(L2, R2, eos, idir=idir, return_mask=True)
+                pre = f"{tag}{si}_{idir}"
+                for k in CONS8:
+                    out[f"{pre}_F_{k}"] = _np(F[k])
+                    out[f"{pre}_U_{k}"] = _np(U[k])
+                out[f"{pre}_pstar"] = _np(ps)
+                out[f"{pre}_ok"] = _np(ok)
+                out[f"{pre}_n_linear"] = np.array(LAST_LINEAR["n_linear"], dtype=np.int64)
+    np.savez_compressed(HERE / "linear_flux.npz", **out)
+
+
 def gen_c2p():
     eos = hybrid_eos(K=0.0, gamma=5.0 / 3.0, gamma_th=5.0 / 3.0)
     rng = np.random.default_rng(7)
@@ -503,7 +549,7 @@ def gen_host():
 if __name__ == "__main__":
     torch.set_num_threads(4)
     which = sys.argv[1:] or ["flux", "c2p", "recon", "2d", "1d", "tubes", "host", "init2d",
-                             "2d_long", "dt1d", "tubes_pertube", "run2d", "pstar"]
+                             "2d_long", "dt1d", "tubes_pertube", "run2d", "pstar", "linear"]
     for w in which:
         print("generating", w, flush=True)
         globals()[f"gen_{w}"]()
