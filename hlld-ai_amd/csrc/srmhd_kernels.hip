@@ -9,6 +9,7 @@
 
 #include "srmhd_grid2d.h"
 #include "srmhd_mixed.h"
+#include "srmhd_linear_launch.h"
 #include "srmhd_pstar_launch.h"
 #include "srmhd_recon.h"
 #include "srmhd_riemann.h"
@@ -663,7 +664,8 @@ struct Ws2 {
   double* Ez;
   C2PRec *recx, *recy, *recc;
   srmhd_flux_diag* diag;        // [0] x sweep, [1] y sweep
-  unsigned long long* words;    // [0] cell c2p mask, [1..2] dt maxima
+  unsigned long long* words;    // [0] cell c2p mask, [1..2] dt maxima, [3] c2p status OR,
+                                // [4..7] the linear solver's |F_D| maxima and counts
   Cells32 cf{};                 // FP32 cell variables of the MIXED form (carved for it only)
   srmhd_flux_diag* dsh;         // DIAG_SHARDS copies of diag[0], then of diag[1]
   int emf_hll = 0;              // the scheme's emf_mode is SRMHD_EMF_HLL
@@ -677,7 +679,7 @@ Ws2 carve2d(const G2& g, void* work, bool cells32 = false) {
   const int64_t nfx = (int64_t)(g.nxt + 1) * g.nyt, nfy = (int64_t)g.nxt * (g.nyt + 1);
   const int64_t ncell = (int64_t)g.nxt * g.nyt, ncor = (int64_t)(g.nxt + 1) * (g.nyt + 1);
   w.diag = c.take<srmhd_flux_diag>(2);
-  w.words = c.take<unsigned long long>(4);
+  w.words = c.take<unsigned long long>(8);
   for (int k = 0; k < 7; ++k) w.Fx.p[k] = c.take<double>(nfx);
   for (int k = 0; k < 7; ++k) w.Fy.p[k] = c.take<double>(nfy);
   w.Ez = c.take<double>(ncor);
@@ -1108,9 +1110,16 @@ int stage1d(const G1& g, int solver, int lim, const srmhd_eos& e, double atmo_rh
   ConsPtrs none;
   for (int k = 0; k < 8; ++k) none.p[k] = nullptr;
   srmhd_flux_diag* d = w.diag + (stage - 1);
-  const int rc = flux_dispatch<0>(solver, g.n + 1, as_in(w.L), as_in(w.R), e, w.recf, d, w.F,
-                                  none, 0, nullptr, nullptr, s, o);
+  const bool linear = solver == SRMHD_LINEAR;
+  const int rc = flux_dispatch<0>(linear ? SOLVER_HLLD : solver, g.n + 1, as_in(w.L),
+                                  as_in(w.R), e, w.recf, d, w.F, none, 0, nullptr, nullptr, s, o);
   if (rc) return rc;
+  if (linear) {
+    // the linearisation where it answers, over the HLLD fallback
+    const int rl = launch_linear_flat(0, g.n + 1, as_in(w.L).p, as_in(w.R).p, e, LinOpts(),
+                                      w.F.p, nullptr, nullptr, nullptr, nullptr, s);
+    if (rl) return rl;
+  }
   CC8 F;
   for (int k = 0; k < 8; ++k) F.p[k] = w.F.p[k];
   k_heun1d<<<blocks_for(g.nt), BLK, 0, s>>>(g, stage, dt_dev, U, U1, F, out);
@@ -1313,11 +1322,13 @@ static bool hlld_default(const HlldOpts& o) {
 }
 
 static bool scheme_ok(const srmhd_scheme2d* s) {
-  if (!(s && s->solver >= 0 && s->solver <= 2 && s->limiter >= 0 && s->limiter <= 5 &&
+  if (!(s && s->solver >= 0 && s->solver <= 3 && s->limiter >= 0 && s->limiter <= 5 &&
         (s->precision == SRMHD_PREC_FP64 || s->precision == SRMHD_PREC_MIXED ||
          s->precision == SRMHD_PREC_MIXED_R64 || s->precision == SRMHD_PREC_FP64_LOCAL) &&
         (s->emf_mode == SRMHD_EMF_SOLVER || s->emf_mode == SRMHD_EMF_HLL)))
     return false;
+  // the linearised flux is an FP64 solver
+  if (s->solver == SRMHD_LINEAR && s->precision != SRMHD_PREC_FP64) return false;
   const HlldOpts o = hlld_opts(&s->hlld);
   // the options are the FP64 form's (the other forms have their own roots)
   return hlld_opts_ok(o) && (s->precision == SRMHD_PREC_FP64 || hlld_default(o));
@@ -1341,8 +1352,9 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   if (tape && (sp->precision != SRMHD_PREC_FP64 || !tape->pstar_x || !tape->pstar_y ||
                !tape->FD_x || !tape->FD_y || !tape->thresh))
     return SRMHD_EARG;
-  // the adjoint differentiates the reference's default root only
-  if (tape && !hlld_default(hlld_opts(&sp->hlld))) return SRMHD_EARG;
+  // the adjoint differentiates the reference's default root (of HLLD) only
+  if (tape && (!hlld_default(hlld_opts(&sp->hlld)) || sp->solver == SRMHD_LINEAR))
+    return SRMHD_EARG;
   const G2 g = make_g2(gp);
   Ws2 w = carve2d(g, work, sp->precision == SRMHD_PREC_MIXED);
   if (work_bytes < w.bytes) return SRMHD_EARG;
@@ -1382,7 +1394,7 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
     z.p[1] = (unsigned long long*)w.dsh;
     z.n[1] = DIAG_SHARD_WORDS;
     z.p[2] = w.words;
-    z.n[2] = 4;
+    z.n[2] = 8;
     z.p[3] = mixed ? (unsigned long long*)w.recx : nullptr;
     z.n[3] = 1;
     z.p[4] = mixed ? (unsigned long long*)w.recy : nullptr;
@@ -1393,6 +1405,12 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   if (mixed) {
     const int rc = sweeps_mixed(sp->limiter, sp->solver, sp->precision == SRMHD_PREC_MIXED_R64,
                                 g, s1, pr, *eos, w, st);
+    if (rc) return rc;
+  } else if (sp->solver == SRMHD_LINEAR) {
+    // HLLD on every face, then the linearisation where it answers
+    sweeps_dispatch(sp->limiter, SOLVER_HLLD, local, g, s1, pr, *eos, w, st);
+    const int rc = launch_linear_sweeps(sp->limiter, g, s1, pr, *eos, LinOpts(), w.Fx, w.Fy,
+                                        w.emf_hll, w.diag, w.words + 4, st);
     if (rc) return rc;
   } else {
     sweeps_dispatch(sp->limiter, sp->solver, local, g, s1, pr, *eos, w, st);
@@ -1509,7 +1527,7 @@ int srmhd1d_rk2_step_ex(const srmhd_grid1d* gp, int solver, int limiter, const s
                         const double* const* cons, const double* const* prim,
                         double* const* cons_out, double* const* prim_out,
                         srmhd_flux_diag* diag2, void* work, int64_t work_bytes, void* stream) {
-  if (!g1_ok(gp) || solver < 0 || solver > 2 || limiter < 0 || limiter > 5 || !eos_ok(eos) ||
+  if (!g1_ok(gp) || solver < 0 || solver > 3 || limiter < 0 || limiter > 5 || !eos_ok(eos) ||
       !dt_dev || !cons || !prim || !cons_out || !prim_out || !work)
     return SRMHD_EARG;
   const HlldOpts o = hlld_opts(opts);

@@ -16,6 +16,17 @@
 // eigenvector matrix with 2-norm condition number >= cond_max, or a V* that
 // is not physical.
 //
+// Where the mean state's spectrum is degenerate (B_n = 0, or B_t = 0, makes
+// the Alfven and slow waves coincide with each other or with the contact)
+// the eigenvectors of the repeated eigenvalue are any basis of its
+// eigenspace.  V* does not depend on the basis, but cond(R) does, and the
+// basis LAPACK's geev returns there is, face by face, either well
+// conditioned or all but singular (measured on the rotor: cond 3-177 or
+// 5e16-3e19, with no rule in the physics that tells which); this code's
+// basis is well conditioned.  On such faces the two take the answered /
+// fallback branches differently -- a documented difference from the
+// reference, not a rounding one.
+//
 // The linear algebra is this file's own, on three 7x7 matrices per face held
 // in LDS (Mat: element (r, c) of a thread's matrix at stride LIN_BLK, so the
 // 64 lanes of a wave touch 64 consecutive words):

@@ -4,7 +4,7 @@
 // LDS and overwrites the faces it answers.  64-face workgroups (one wave):
 // 3 x 49 doubles x 64 lanes = 73.5 KiB of LDS, two workgroups per CU.
 #include "srmhd_launch.h"
-#include "srmhd_linear.h"
+#include "srmhd_linear_launch.h"
 
 using namespace srmhd;
 
@@ -67,7 +67,111 @@ __global__ __launch_bounds__(LIN_BLK) void k_linear(int64_t n, LPrim Lp, LPrim R
   if ((threadIdx.x & 63) == 0 && b && count) atomicAdd(count, (unsigned long long)__popcll(b));
 }
 
+// the 2D sweep along N: the face states of face_prims2d (the sweep's own),
+// the linearisation, the seven sweep fluxes (D, Sx, Sy, Sz, tau, Bz, E) where
+// it answers, and the wave maxima of the final |F_D| and the answered count
+template <int N, int LIM>
+__global__ __launch_bounds__(LIN_BLK) void k_sweep_linear(G2 g, St2 S, Pr2 P, srmhd_eos e,
+                                                          LinOpts o, Flx F, int emf_hll,
+                                                          unsigned long long* words) {
+  __shared__ double sm[3 * LIN_NV * LIN_NV * LIN_BLK];
+  const int64_t t = (int64_t)blockIdx.x * LIN_BLK + threadIdx.x;
+  const bool active = t < n_faces<N>(g);
+  bool ok = false;
+  double fD = 0.0;
+  if (active) {
+    int i, j;
+    face_ij<N>(g, t, i, j);
+    Prim l, r;
+    face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
+    const LinMat M1{sm + threadIdx.x}, M2{sm + LIN_NV * LIN_NV * LIN_BLK + threadIdx.x},
+        M3{sm + 2 * LIN_NV * LIN_NV * LIN_BLK + threadIdx.x};
+    double f[8], u[8], ps;
+    ok = linear_face<N>(l, r, e, o, M1, M2, M3, f, u, ps);
+    if (ok) {
+      F.p[0][t] = f[CD];
+      F.p[1][t] = f[CS];
+      F.p[2][t] = f[CS + 1];
+      F.p[3][t] = f[CS + 2];
+      F.p[4][t] = f[CT];
+      F.p[5][t] = f[CB + 2];
+      if (!emf_hll) F.p[6][t] = N == 0 ? f[CB + 1] : f[CB + 0];
+      fD = f[CD];
+    } else {
+      fD = F.p[0][t];
+    }
+  }
+  const unsigned long long m = wave_max(active ? dbits(fabs(fD)) : 0ull);
+  const unsigned long long b = __ballot(ok);
+  if ((threadIdx.x & 63) == 0) {
+    if (m) atomicMax(&words[N], m);
+    if (b) atomicAdd(&words[2 + N], (unsigned long long)__popcll(b));
+  }
+}
+
+__global__ void k_zero4(unsigned long long* w) {
+  if (threadIdx.x < 4) w[threadIdx.x] = 0ull;
+}
+
+__global__ void k_linear_fd(srmhd_flux_diag* d, const unsigned long long* w) {
+  if (threadIdx.x < 2) d[threadIdx.x].absF_D_max = w[threadIdx.x];
+}
+
+template <int LIM>
+void sweeps_linear(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const LinOpts& o,
+                   const Flx& Fx, const Flx& Fy, int emf_hll, unsigned long long* words,
+                   hipStream_t s) {
+  const int bx = (int)((n_faces<0>(g) + LIN_BLK - 1) / LIN_BLK);
+  const int by = (int)((n_faces<1>(g) + LIN_BLK - 1) / LIN_BLK);
+  k_sweep_linear<0, LIM><<<bx, LIN_BLK, 0, s>>>(g, S, P, e, o, Fx, emf_hll, words);
+  k_sweep_linear<1, LIM><<<by, LIN_BLK, 0, s>>>(g, S, P, e, o, Fy, emf_hll, words);
+}
+
 }  // namespace
+
+namespace srmhd {
+
+int launch_linear_flat(int idir, int64_t n, const double* const* L, const double* const* R,
+                       const srmhd_eos& e, const LinOpts& o, double* const* F, double* const* U,
+                       double* pstar, uint8_t* mask, unsigned long long* count, hipStream_t s) {
+  if (n <= 0) return launch_status();
+  LPrim Lp, Rp;
+  for (int k = 0; k < 9; ++k) {
+    Lp.p[k] = L[k];
+    Rp.p[k] = R[k];
+  }
+  LCons Fp, Up;
+  for (int k = 0; k < 8; ++k) {
+    Fp.p[k] = F[k];
+    Up.p[k] = U ? U[k] : nullptr;
+  }
+  const int nb = (int)((n + LIN_BLK - 1) / LIN_BLK);
+  if (idir == 0)
+    k_linear<0><<<nb, LIN_BLK, 0, s>>>(n, Lp, Rp, e, o, Fp, Up, pstar, mask, count);
+  else if (idir == 1)
+    k_linear<1><<<nb, LIN_BLK, 0, s>>>(n, Lp, Rp, e, o, Fp, Up, pstar, mask, count);
+  else
+    k_linear<2><<<nb, LIN_BLK, 0, s>>>(n, Lp, Rp, e, o, Fp, Up, pstar, mask, count);
+  return launch_status();
+}
+
+int launch_linear_sweeps(int lim, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
+                         const LinOpts& o, const Flx& Fx, const Flx& Fy, int emf_hll,
+                         srmhd_flux_diag* diag, unsigned long long* words, hipStream_t s) {
+  k_zero4<<<1, 64, 0, s>>>(words);
+  switch (lim) {
+    case LIM_PCM: sweeps_linear<LIM_PCM>(g, S, P, e, o, Fx, Fy, emf_hll, words, s); break;
+    case LIM_MC: sweeps_linear<LIM_MC>(g, S, P, e, o, Fx, Fy, emf_hll, words, s); break;
+    case LIM_MINMOD: sweeps_linear<LIM_MINMOD>(g, S, P, e, o, Fx, Fy, emf_hll, words, s); break;
+    case LIM_WENO5Z: sweeps_linear<LIM_WENO5Z>(g, S, P, e, o, Fx, Fy, emf_hll, words, s); break;
+    case LIM_MP5: sweeps_linear<LIM_MP5>(g, S, P, e, o, Fx, Fy, emf_hll, words, s); break;
+    default: sweeps_linear<LIM_MP7>(g, S, P, e, o, Fx, Fy, emf_hll, words, s);
+  }
+  k_linear_fd<<<1, 64, 0, s>>>(diag, words);
+  return launch_status();
+}
+
+}  // namespace srmhd
 
 extern "C" {
 
@@ -103,25 +207,8 @@ int srmhd_linear_flux(int idir, int64_t n, const double* const* primL,
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
   if (n_linear) k_zero_count<<<1, 64, 0, s>>>(n_linear);
-  if (n == 0) return launch_status();
-  LPrim Lp, Rp;
-  for (int k = 0; k < 9; ++k) {
-    Lp.p[k] = primL[k];
-    Rp.p[k] = primR[k];
-  }
-  LCons Fp, Up;
-  for (int k = 0; k < 8; ++k) {
-    Fp.p[k] = F[k];
-    Up.p[k] = U ? U[k] : nullptr;
-  }
-  const int nb = (int)((n + LIN_BLK - 1) / LIN_BLK);
-  if (idir == 0)
-    k_linear<0><<<nb, LIN_BLK, 0, s>>>(n, Lp, Rp, *eos, o, Fp, Up, pstar, linear_mask, n_linear);
-  else if (idir == 1)
-    k_linear<1><<<nb, LIN_BLK, 0, s>>>(n, Lp, Rp, *eos, o, Fp, Up, pstar, linear_mask, n_linear);
-  else
-    k_linear<2><<<nb, LIN_BLK, 0, s>>>(n, Lp, Rp, *eos, o, Fp, Up, pstar, linear_mask, n_linear);
-  return launch_status();
+  return launch_linear_flat(idir, n, primL, primR, *eos, o, F, U, pstar, linear_mask, n_linear,
+                            s);
 }
 
 }  // extern "C"

@@ -13,7 +13,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libsrmhd_hip.so")
 
-SOLVER_IDS = {"hlle": 0, "hllc": 1, "hlld": 2}
+SOLVER_IDS = {"hlle": 0, "hllc": 1, "hlld": 2, "linear": 3}
 LIMITER_IDS = {"pcm": 0, "mc": 1, "minmod": 2, "weno5z": 3, "mp5": 4, "mp7": 5}
 BC_IDS = {"outflow": 0, "periodic": 1, "constant": 2}
 PRECISION_IDS = {"fp64": 0, "mixed": 1, "mixed_r64": 2, "fp64_local": 3}

@@ -14,7 +14,7 @@ import torch
 
 from . import _lib
 from .eos import as_c, hybrid_eos
-from .hlld import LAST_DIAG, primitive_to_conserved, solver_name
+from .hlld import LAST_DIAG, diag_solver, primitive_to_conserved, solver_name
 
 PRIM_KEYS = ("rho", "vx", "vy", "vz", "p", "eps", "Bx", "By", "Bz")
 CONS_C = ("D", "tau", "Sx", "Sy", "Sz", "Bx", "By", "Bz")      # the C ABI's order
@@ -145,9 +145,9 @@ def run(cfg, write_snapshots=True):
     t_end, cfl = rc["t_end"], rc.get("cfl", 0.4)
     atmo_rho = rc.get("atmo_rho", 1e-10)
     solver = rc.get("solver", "hlld")
-    if solver in ("linear", "exact", "exact_batched"):
-        raise ValueError(f"solver {solver!r} is not on the HIP path (hlle, hllc, hlld)")
-    solver = solver if solver in ("hlle", "hllc", "hlld") else "hlld"   # driver.py:379-380
+    if solver in ("exact", "exact_batched"):
+        raise ValueError(f"solver {solver!r} is not on the HIP path (hlle, hllc, hlld, linear)")
+    solver = solver if solver in ("hlle", "hllc", "hlld", "linear") else "hlld"   # driver.py:379-380
     limiter = rc.get("limiter", "pcm")
     dt_c = bool(rc.get("dt_speed_of_light", True))
     out_dir = Path(oc.get("dir", "output"))
@@ -178,5 +178,5 @@ def run(cfg, write_snapshots=True):
             save_snapshot(out_dir / f"snap_{step:06d}.npz", t, grid,
                           dict(zip(PRIM_KEYS, P)), dict(zip(CONS_C, C)))
             t_next = t + (out_dt_max or 0.0)
-    LAST_DIAG._set_pending(stp.diag.clone()[_lib.DIAG_WORDS:], stp.solver, 0)
+    LAST_DIAG._set_pending(stp.diag.clone()[_lib.DIAG_WORDS:], diag_solver(stp.solver), 0)
     return t, grid, dict(zip(PRIM_KEYS, P)), dict(zip(CONS_C, C))

@@ -18,7 +18,7 @@ import torch
 from . import _lib
 from .c2p import conservative_to_primitive
 from .eos import as_c
-from .hlld import LAST_DIAG, _LazyDiag, primitive_to_conserved, solver_name
+from .hlld import LAST_DIAG, _LazyDiag, diag_solver, primitive_to_conserved, solver_name
 from .state import EVOLVED_KEYS, PRIM6, State2D
 
 RK = {"rk2": ((1.0, 0.0, 1.0), (0.5, 0.5, 0.5)),          # driver2d.py:104-105
@@ -82,7 +82,7 @@ class Stepper2D:
 
     # ── C-ABI calls ─────────────────────────────────────────────────────────
     def _refresh_hlld_opts(self):
-        if not (self.hlld_opts_from_globals and self.solver == "hlld"):
+        if not (self.hlld_opts_from_globals and self.solver in ("hlld", "linear")):
             return
         from . import hlld as H
         o = H.hlld_opts()
@@ -138,8 +138,8 @@ class Stepper2D:
         # the FP32-flux forms count their own solver's decisions, not the
         # reference's secant iterations
         counted = self.precision in ("fp64", "fp64_local")
-        dx._set_pending(w[:_lib.DIAG_WORDS], self.solver, 0, iters_counted=counted)
-        dy._set_pending(w[_lib.DIAG_WORDS:], self.solver, 1, iters_counted=counted)
+        dx._set_pending(w[:_lib.DIAG_WORDS], diag_solver(self.solver), 0, iters_counted=counted)
+        dy._set_pending(w[_lib.DIAG_WORDS:], diag_solver(self.solver), 1, iters_counted=counted)
         return {"x": {"diag": dx}, "y": {"diag": dy}}
 
     # ── resident step loop ─────────────────────────────────────────────────
@@ -268,7 +268,7 @@ def rk_step_ct(state, grid, eos, dt, *, scheme="rk3", bc_x="outflow", bc_y="outf
     S = State2D({k: U[i] for i, k in enumerate(EVOLVED_KEYS)}, U[6], U[7])
     _finish_prims(S, P, grid, bc_x, bc_y)
     diag = st.diag_dicts()
-    LAST_DIAG._set_pending(st.diag.clone()[_lib.DIAG_WORDS:], st.solver, 1,
+    LAST_DIAG._set_pending(st.diag.clone()[_lib.DIAG_WORDS:], diag_solver(st.solver), 1,
                            iters_counted=st.precision in ("fp64", "fp64_local"))
     return S, diag
 

@@ -204,12 +204,21 @@ SOLVERS = {"hlle": hlle_flux, "hllc": hllc_flux, "hlld": hlld_flux}
 
 
 def solver_name(flux_fn):
-    """Map a flux_fn (this module's, the reference's, or a name) to its name."""
+    """Map a flux_fn (this package's, the reference's, or a name) to its
+    name: hlle, hllc, hlld or linear (linearised_flux with its default HLLD
+    fallback, linearised.py:129)."""
     if isinstance(flux_fn, str):
         name = flux_fn
     else:
         name = getattr(flux_fn, "__name__", "")
-        name = {"hlle_flux": "hlle", "hllc_flux": "hllc", "hlld_flux": "hlld"}.get(name, name)
-    if name not in SOLVERS:
-        raise ValueError(f"unsupported flux_fn {flux_fn!r}: expected hlle, hllc or hlld")
+        name = {"hlle_flux": "hlle", "hllc_flux": "hllc", "hlld_flux": "hlld",
+                "linearised_flux": "linear"}.get(name, name)
+    if name not in SOLVERS and name != "linear":
+        raise ValueError(f"unsupported flux_fn {flux_fn!r}: expected hlle, hllc, hlld or linear")
     return name
+
+
+def diag_solver(name):
+    """The solver whose counters LAST_DIAG reports: the linearised flux's
+    are its HLLD fallback's, as in the reference (linearised.py:141)."""
+    return "hlld" if name == "linear" else name

@@ -18,9 +18,10 @@ Files written into ``--out`` (the reference's schema, run_2d.py:170-334):
                   so it is bit-identical to the uninterrupted run.
 
 The step itself is ``driver2d.Stepper2D`` (one C-ABI call per RK stage, the
-state resident on the device).  The solvers of the reference that stay on
-its CPU path (``linear``, ``exact``) and its interface recorders are not
-offered here.
+state resident on the device).  ``--solver linear`` is the linearised flux
+(srmhd.linearised) over the HLLD fallback, in the FP64 form.  The exact
+solver of the reference stays on its CPU path, and its interface recorders
+are not offered here.
 """
 
 import argparse
@@ -57,7 +58,7 @@ PROBLEMS = {
     "orszag_tang_2pi": dict(box=(0.0, 2.0 * math.pi), gamma=4.0 / 3.0, tend=1.0,
                             bc=("periodic", "periodic"), sym=False),
 }
-SOLVERS = ("hlld", "hlle", "hllc")
+SOLVERS = ("hlld", "hlle", "hllc", "linear")
 DIAG_HEADER = ("step,t,dt,divB_max,divB_l2,sym_err,rho_max,rho_min,p_min,W_max,hlle_frac_x,"
                "hlle_frac_y,mean_iters,c2p_bad,n_exact_x,n_verified_x\n")
 

@@ -67,7 +67,10 @@ typedef struct srmhd_flux_diag {
   unsigned long long n_c2p_redo;    /* HLL-state con2prims finished by the bisection redo */
 } srmhd_flux_diag;
 
-enum { SRMHD_HLLE = 0, SRMHD_HLLC = 1, SRMHD_HLLD = 2 };
+/* SRMHD_LINEAR (the linearised flux with the HLLD fallback, the reference's
+ * --solver linear) is a solver of srmhd2d_stage and srmhd1d_rk2_step in the
+ * FP64 form; the flat flux entry point for it is srmhd_linear_flux. */
+enum { SRMHD_HLLE = 0, SRMHD_HLLC = 1, SRMHD_HLLD = 2, SRMHD_LINEAR = 3 };
 enum { SRMHD_PCM = 0, SRMHD_MC = 1, SRMHD_MINMOD = 2, SRMHD_WENO5Z = 3, SRMHD_MP5 = 4,
        SRMHD_MP7 = 5 };
 enum { SRMHD_BC_OUTFLOW = 0, SRMHD_BC_PERIODIC = 1 };

@@ -95,7 +95,10 @@ def run(cfg, max_steps=None):
     grid.outflow(cons)
     t_end, cfl = rc["t_end"], rc.get("cfl", 0.4)
     atmo = rc.get("atmo_rho", 1e-10)
-    flux_fn = SOLVERS.get(rc.get("solver", "hlld"), hlld_flux)
+    if rc.get("solver") == "linear":              # driver.py:378-380
+        from .linear import linearised_flux as flux_fn
+    else:
+        flux_fn = SOLVERS.get(rc.get("solver", "hlld"), hlld_flux)
     limiter = rc.get("limiter", "pcm")
     dt_c = bool(rc.get("dt_speed_of_light", True))
     t, step = 0.0, 0

@@ -289,7 +289,9 @@ def _run2d(name, problem, n, limiter, solver, scheme, steps, cfl, gamma, box, bc
     st = sync_state(st, g, eos, *bc)
     out = {}
     _state_arrays("init_", st, out)
-    flux = {"hlld": R_hlld.hlld_flux, "hllc": R_hlld.hllc_flux, "hlle": R_hlld.hlle_flux}[solver]
+    from src.physics.linearised import linearised_flux
+    flux = {"hlld": R_hlld.hlld_flux, "hllc": R_hlld.hllc_flux, "hlle": R_hlld.hlle_flux,
+            "linear": linearised_flux}[solver]
     dts = []
     for i in range(steps):
         dt = compute_dt_2d(st.prims, g, eos, cfl)
@@ -312,6 +314,20 @@ def gen_2d():
     _run2d("rotor_tapered_mp5_hllc", "tapered", 16, "mp5", "hllc", "rk3", 1, 0.4, 5.0 / 3.0, r, oo)
     _run2d("rotor_sharp_mc_hlle", "rotor", 16, "mc", "hlle", "rk2", 1, 0.4, 5.0 / 3.0, r, oo)
     _run2d("ot2pi_mc_hlld", "ot", 16, "mc", "hlld", "rk2", 2, 0.4, 4.0 / 3.0,
+           (0.0, 2.0 * np.pi), ("periodic", "periodic"))
+
+
+def gen_2d_linear():
+    """rk_step_ct with flux_fn=linearised_flux (run_2d.py --solver linear):
+    the tapered rotor, MC and WENO5-Z, two RK3 steps."""
+    r = (-0.5, 0.5)
+    oo = ("outflow", "outflow")
+    _run2d("rotor_tapered_mc_linear", "tapered", 24, "mc", "linear", "rk3", 2, 0.4, 5.0 / 3.0,
+           r, oo, first=True)
+    _run2d("rotor_tapered_weno5z_linear", "tapered", 20, "weno5z", "linear", "rk3", 1, 0.4,
+           5.0 / 3.0, r, oo)
+    # no face of the 2 pi Orszag-Tang problem has a degenerate spectrum
+    _run2d("ot2pi_mc_linear", "ot", 24, "mc", "linear", "rk2", 3, 0.4, 4.0 / 3.0,
            (0.0, 2.0 * np.pi), ("periodic", "periodic"))
 
 
@@ -500,6 +516,39 @@ def gen_1d():
     shutil.rmtree(HERE / "_tmp_out", ignore_errors=True)
 
 
+def gen_1d_linear():
+    """driver.run on Balsara 1 with solver "linear" (driver.py:378-380), MC,
+    to t = 0.02."""
+    cfg = {"grid": {"xmin": 0.0, "xmax": 1.0, "ncells": 400, "ng": 2},
+           "eos": {"K": 0.0, "gamma": 2.0, "gamma_th": 2.0},
+           "run": {"problem": "generic", "t_end": 0.02, "cfl": 0.4, "atmo_rho": 1.0e-10,
+                   "solver": "linear", "limiter": "mc", "device": "cpu",
+                   "primL": {"rho": 1.0, "p": 1.0, "vx": 0.0, "vy": 0.0, "vz": 0.0,
+                             "Bx": 0.5, "By": 1.0, "Bz": 0.0},
+                   "primR": {"rho": 0.125, "p": 0.1, "vx": 0.0, "vy": 0.0, "vz": 0.0,
+                             "Bx": 0.5, "By": -1.0, "Bz": 0.0},
+                   "x_interface": 0.5},
+           "output": {"dir": str(HERE / "_tmp_out"), "every_n_steps": 100000}}
+    out = {}
+    t, grid, prims, cons = R_run(cfg)
+    out["t"] = np.array(t)
+    for k in PRIM9:
+        out[f"prim_{k}"] = _np(prims[k])
+    # a non-coplanar drifting tube: no face's mean state has B_t = 0 or a
+    # contact at rest, so no spectrum is degenerate
+    cfg["run"]["primL"] = {"rho": 1.0, "p": 1.0, "vx": 0.1, "vy": 0.05, "vz": 0.0,
+                           "Bx": 0.5, "By": 1.0, "Bz": 0.4}
+    cfg["run"]["primR"] = {"rho": 0.125, "p": 0.1, "vx": 0.1, "vy": 0.0, "vz": 0.05,
+                           "Bx": 0.5, "By": -1.0, "Bz": 0.2}
+    t, grid, prims, cons = R_run(cfg)
+    out["nc_t"] = np.array(t)
+    for k in PRIM9:
+        out[f"nc_prim_{k}"] = _np(prims[k])
+    np.savez_compressed(HERE / "balsara1_linear.npz", **out)
+    import shutil
+    shutil.rmtree(HERE / "_tmp_out", ignore_errors=True)
+
+
 def gen_tubes():
     eos = hybrid_eos(K=0.0, gamma=5.0 / 3.0, gamma_th=5.0 / 3.0)
     BW_L, BW_R = [1.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0], [0.125, 0.6, 0.0, 0.0, 0.0, -1.0, 0.0]
@@ -549,7 +598,7 @@ def gen_host():
 if __name__ == "__main__":
     torch.set_num_threads(4)
     which = sys.argv[1:] or ["flux", "c2p", "recon", "2d", "1d", "tubes", "host", "init2d",
-                             "2d_long", "dt1d", "tubes_pertube", "run2d", "pstar", "linear"]
+                             "2d_long", "dt1d", "tubes_pertube", "run2d", "pstar", "linear", "2d_linear", "1d_linear"]
     for w in which:
         print("generating", w, flush=True)
         globals()[f"gen_{w}"]()

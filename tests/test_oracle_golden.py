@@ -116,7 +116,20 @@ CASES_2D = [
     ("rotor_tapered_mp5_hllc", "tapered", 16, "mp5", "hllc", "rk3", 5 / 3, (-0.5, 0.5), "outflow"),
     ("rotor_sharp_mc_hlle", "rotor", 16, "mc", "hlle", "rk2", 5 / 3, (-0.5, 0.5), "outflow"),
     ("ot2pi_mc_hlld", "ot", 16, "mc", "hlld", "rk2", 4 / 3, (0.0, 2 * np.pi), "periodic"),
+    # --solver linear: rk_step_ct with flux_fn=linearised_flux (its HLLD fallback)
+    ("rotor_tapered_mc_linear", "tapered", 24, "mc", "linear", "rk3", 5 / 3, (-0.5, 0.5),
+     "outflow"),
+    ("rotor_tapered_weno5z_linear", "tapered", 20, "weno5z", "linear", "rk3", 5 / 3,
+     (-0.5, 0.5), "outflow"),
+    ("ot2pi_mc_linear", "ot", 24, "mc", "linear", "rk2", 4 / 3, (0.0, 2 * np.pi), "periodic"),
 ]
+
+
+def _flux_fn(solver):
+    if solver == "linear":
+        from oracle import linear as OL
+        return OL.linearised_flux
+    return R.SOLVERS[solver]
 
 
 @pytest.mark.parametrize("case", CASES_2D, ids=[c[0] for c in CASES_2D])
@@ -138,7 +151,7 @@ def test_2d_steps_bitwise(golden_dir, case):
         dt = S2.compute_dt_2d(st.prims, grid, eos, 0.4)
         assert dt == dt_ref
         st, _ = S2.rk_step_ct(st, grid, eos, dt, scheme=scheme, bc_x=bc, bc_y=bc,
-                              flux_fn=R.SOLVERS[solver], limiter=lim)
+                              flux_fn=_flux_fn(solver), limiter=lim)
     ref = _state_from(g, "fin_")
     for k in S2.EVOLVED_KEYS:
         _eq(st.cons[k].numpy(), ref.cons[k].numpy(), f"cons {k}")
@@ -164,6 +177,29 @@ def test_balsara1_bitwise(golden_dir, solver):
     assert t == float(g[f"{solver}_t"])
     for k in PRIM9:
         _eq(P[k].numpy(), g[f"{solver}_prim_{k}"], f"{solver} prim {k}")
+
+
+LINEAR_NC_L = {"rho": 1.0, "p": 1.0, "vx": 0.1, "vy": 0.05, "vz": 0.0, "Bx": 0.5, "By": 1.0,
+               "Bz": 0.4}
+LINEAR_NC_R = {"rho": 0.125, "p": 0.1, "vx": 0.1, "vy": 0.0, "vz": 0.05, "Bx": 0.5, "By": -1.0,
+               "Bz": 0.2}
+
+
+@pytest.mark.parametrize("case", ["balsara1", "nc"])
+def test_1d_linear_bitwise(golden_dir, case):
+    """driver.run with solver "linear" (MC, to t = 0.02): Balsara 1, and a
+    non-coplanar drifting tube (no degenerate spectrum on any face)."""
+    g = _load(golden_dir, "balsara1_linear.npz")
+    cfg = S1.balsara1_cfg(solver="linear")
+    cfg["run"]["t_end"] = 0.02
+    pre = ""
+    if case == "nc":
+        cfg["run"]["primL"], cfg["run"]["primR"] = dict(LINEAR_NC_L), dict(LINEAR_NC_R)
+        pre = "nc_"
+    t, grid, P, cons, _ = S1.run(cfg)
+    assert t == float(g[f"{pre}t"])
+    for k in PRIM9:
+        _eq(P[k].numpy(), g[f"{pre}prim_{k}"], f"linear prim {k}")
 
 
 @pytest.mark.parametrize("lim", ["mc", "mp5"])
@@ -235,7 +271,7 @@ def _run_case(g, case, steps, prefix):
         dt = S2.compute_dt_2d(st.prims, grid, eos, 0.4)
         assert dt == dt_ref
         st, _ = S2.rk_step_ct(st, grid, eos, dt, scheme=scheme, bc_x=bc, bc_y=bc,
-                              flux_fn=R.SOLVERS[solver], limiter=lim)
+                              flux_fn=_flux_fn(solver), limiter=lim)
     ref = _state_from(g, prefix)
     for k in S2.EVOLVED_KEYS:
         _eq(st.cons[k].numpy(), ref.cons[k].numpy(), f"{prefix} cons {k}")
