@@ -282,19 +282,35 @@ SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRec& rec) {
   return mask;
 }
 
-// Pass-A record that also keeps the secant state at the element's own first
-// convergence, step k (the first step with |f| < tol; k = 60 when there is
-// none).  The batch's break iteration K is the first step at which every
-// element has converged, so K >= k: a batch that breaks early (K < 60)
-// replays K - k steps from that state instead of K from the start, and one
-// that never breaks (K = 60) takes the 60-step state as it is.  For batches
-// that often break early (a tube's cells and faces).  The walk depends only
-// on (x0, f0, x1, f1), so the replay is bit-identical.
+// Pass-A record for batches that often break early (a tube's cells and
+// faces).  Besides the final state it keeps where the element's walk became
+// stationary and three intermediate secant states, so that pass B reaches
+// the state after the batch's K steps with at most a few steps:
+//  * jfix: the step from which the walk maps its state to itself (60: none).
+//    The state after K >= jfix steps is the final one.
+//  * ck[0]: the state after k steps, k = the element's first convergence
+//    (|f| < tol; 60 when there is none).  The batch breaks at the first step
+//    at which EVERY element has converged, so K >= k whenever K < 60.
+//  * ck[1], ck[2]: the states after CK1 and CK2 steps (written only when the
+//    walk is still moving there).  Batches that break, break late: after
+//    ~49-59 steps on random Riemann tubes, with first convergence at ~5-13
+//    steps.
+// The walk depends only on (x0, f0, x1, f1), so every replay from a stored
+// state is bit-identical to the uninterrupted walk.
+constexpr int C2P_CK1 = 52, C2P_CK2 = 56;
+
 struct C2PRecK {
-  double mup_signed, x1, f1;       // as C2PRec: the 60-step state
-  double kx0, kf0, kx1, kf1;       // the state after k steps
-  int32_t k, pad;
+  double mup_signed, x1, f1;       // as C2PRec: the state after 60 steps
+  double ck[3][4];                 // (x0, f0, x1, f1) after k, CK1, CK2 steps
+  int32_t k, jfix;
 };
+
+SRMHD_DEV void rec_put(double* c, double x0, double f0, double x1, double f1) {
+  c[0] = x0;
+  c[1] = f0;
+  c[2] = x1;
+  c[3] = f1;
+}
 
 SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRecK& rec) {
   const double mup = k.mu_plus_fast();
@@ -302,8 +318,7 @@ SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRecK& rec) {
   double f0 = k.master(x0), f1 = k.master(x1);
   const bool good = (f0 * f1) < 0.0;
   uint64_t mask = 0;
-  int ke = C2P_SECANT;
-  double s0 = 0.0, g0 = 0.0, s1 = 0.0, g1 = 0.0;
+  int ke = C2P_SECANT, jf = C2P_SECANT;
 #pragma unroll 1
   for (int it = 0; it < C2P_SECANT; ++it) {
     const bool nc = !(fabs(f1) < C2P_TOL);
@@ -311,25 +326,21 @@ SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRecK& rec) {
       mask |= (uint64_t)1 << it;
     } else if (ke == C2P_SECANT) {
       ke = it;
-      s0 = x0;
-      g0 = f0;
-      s1 = x1;
-      g1 = f1;
+      rec_put(rec.ck[0], x0, f0, x1, f1);
     }
+    if (it == C2P_CK1) rec_put(rec.ck[1], x0, f0, x1, f1);
+    if (it == C2P_CK2) rec_put(rec.ck[2], x0, f0, x1, f1);
     if (k.secant_step_fixed(x0, f0, x1, f1)) {
       if (nc) mask |= C2P_ALL & ~(((uint64_t)2 << it) - 1);
+      jf = it;
       break;
     }
   }
   rec.mup_signed = good ? mup : -mup;
   rec.x1 = x1;
   rec.f1 = f1;
-  rec.kx0 = s0;
-  rec.kf0 = g0;
-  rec.kx1 = s1;
-  rec.kf1 = g1;
   rec.k = ke;
-  rec.pad = 0;
+  rec.jfix = jf;
   return mask;
 }
 
@@ -358,17 +369,22 @@ SRMHD_DEV void c2p_secant_at(const Kastaun& k, const C2PRec& rec, int K, double&
   }
 }
 
-// the same from the first-convergence record (K >= rec.k)
+// the same from the checkpointed record: the final state when the walk was
+// stationary by step K, else a replay from the latest stored state at or
+// before K (K >= rec.k when K < 60)
 SRMHD_DEV void c2p_secant_at(const Kastaun& k, const C2PRecK& rec, int K, double& x1,
                              double& f1) {
   x1 = rec.x1;
   f1 = rec.f1;
-  if (K < C2P_SECANT) {
-    double x0 = rec.kx0, f0 = rec.kf0;
-    x1 = rec.kx1;
-    f1 = rec.kf1;
+  if (K < C2P_SECANT && K < rec.jfix) {
+    const int slot = K >= C2P_CK2 ? 2 : (K >= C2P_CK1 ? 1 : 0);
+    const int from = slot == 2 ? C2P_CK2 : (slot == 1 ? C2P_CK1 : rec.k);
+    const double* c = rec.ck[slot];
+    double x0 = c[0], f0 = c[1];
+    x1 = c[2];
+    f1 = c[3];
 #pragma unroll 1
-    for (int it = rec.k; it < K; ++it)
+    for (int it = from; it < K; ++it)
       if (k.secant_step_fixed(x0, f0, x1, f1)) break;
   }
 }
@@ -383,17 +399,16 @@ SRMHD_DEV bool c2p_needs_redo(const Kastaun& k, const Rec& rec, int K) {
 }
 
 // Pass B: full conservative_to_primitive of one element given its pass-A
-// record and the batch break iteration K.   c2p.py:313-363
+// record and the secant state (x1, f1) after the batch's K steps
+// (c2p_secant_at).   c2p.py:313-363
 // force < 0: the element is redone by bisection when it needs it itself;
 // force >= 0: the caller decides (force != 0 redoes), for batches whose redo
 // set is wider than the elements that need it.
 template <class Rec>
-SRMHD_DEV int c2p_pass_b(const Kastaun& k, const Rec& rec, int K, double atmo_rho,
-                         double atmo_eps, Prim& P, int force = -1) {
+SRMHD_DEV int c2p_pass_at(const Kastaun& k, const Rec& rec, double x1, double f1,
+                          double atmo_rho, double atmo_eps, Prim& P, int force = -1) {
   const bool good = rec.mup_signed > 0.0;
   const double mup = fabs(rec.mup_signed);
-  double x1, f1;
-  c2p_secant_at(k, rec, K, x1, f1);
   bool conv = fabs(f1) < C2P_TOL;
   bool clamped;
   const bool redo = force < 0 ? (!conv && good) : (force != 0);
@@ -416,6 +431,15 @@ SRMHD_DEV int c2p_pass_b(const Kastaun& k, const Rec& rec, int K, double atmo_rh
   P.p = eos_press(k.e, P.eps, P.rho);
   return (good ? C2P_BRACKET_OK : 0) | (conv ? C2P_CONVERGED : 0) | (clamped ? C2P_V_CLAMPED : 0) |
          (atmo ? C2P_ATMO : 0) | (redo ? C2P_REDONE : 0);
+}
+
+// the same from the record and the batch break iteration K
+template <class Rec>
+SRMHD_DEV int c2p_pass_b(const Kastaun& k, const Rec& rec, int K, double atmo_rho,
+                         double atmo_eps, Prim& P, int force = -1) {
+  double x1, f1;
+  c2p_secant_at(k, rec, K, x1, f1);
+  return c2p_pass_at(k, rec, x1, f1, atmo_rho, atmo_eps, P, force);
 }
 
 // conservative_to_primitive of ONE element as its own batch (c2p.py:313-363

@@ -179,7 +179,12 @@ __global__ __launch_bounds__(BLK) void k_p2c(int64_t n, PrimPtrs P, ConsPtrs C) 
 }
 
 
-template <int N, int LIM>
+// Pass A of the sweep's HLL-state con2prims.  DEFER: faces whose con2prim
+// is inert (hll_c2p_inert: B_n = 0 on both sides, so the flux is the HLLE
+// one) are left out; their result matters only when the other faces'
+// masks leave the batch's break iteration below 60, and k_sweep_a_rescue
+// then runs them.  On the rotor ~95 % of the y faces are such faces.
+template <int N, int LIM, bool DEFER = false>
 __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e, C2PRec* rec,
                                                  srmhd_flux_diag* dsh) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
@@ -191,20 +196,57 @@ __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e
     face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
     HlldPrep<N> h;
     h.run(l, r, e);
-    Kastaun k;
-    k.init(h.uH, e);
-    m = c2p_pass_a(k, rec[t]);
+    if (!(DEFER && hll_c2p_inert<N>(h))) {
+      Kastaun k;
+      k.init(h.uH, e);
+      m = c2p_pass_a(k, rec[t]);
+    }
   }
   block_or_atomic(&diag_shard(dsh)->c2p_mask, m);
 }
 
+SRMHD_DEV bool c2p_mask_full(uint64_t m) { return (m & C2P_ALL) == C2P_ALL; }
+
+// Pass A of the faces k_sweep_a<N, LIM, true> left out, when the other
+// faces' mask d->c2p_mask leaves the break iteration below 60 (otherwise
+// every workgroup returns at once); their mask bits go to *mres.
+template <int N, int LIM>
+__global__ __launch_bounds__(BLK) void k_sweep_a_rescue(G2 g, St2 S, Pr2 P, srmhd_eos e,
+                                                        C2PRec* rec, const srmhd_flux_diag* d,
+                                                        unsigned long long* mres) {
+  if (c2p_mask_full(d->c2p_mask)) return;
+  const int64_t nf = n_faces<N>(g);
+  uint64_t m = 0;
+  for (int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x; t < nf;
+       t += (int64_t)gridDim.x * BLK) {
+    int i, j;
+    face_ij<N>(g, t, i, j);
+    Prim l, r;
+    face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
+    HlldPrep<N> h;
+    h.run(l, r, e);
+    if (hll_c2p_inert<N>(h)) {
+      Kastaun k;
+      k.init(h.uH, e);
+      m |= c2p_pass_a(k, rec[t]);
+    }
+  }
+  block_or_atomic(mres, m);
+}
+
 // LOCAL: the con2prim of each face's HLL state is solved as its own batch
-// (c2p_single) instead of finishing pass A's batch-coupled iteration
+// (c2p_single) instead of finishing pass A's batch-coupled iteration.
+// mres (non-NULL after k_sweep_a<N, LIM, true>): the mask of the deferred
+// faces.  When the other faces' mask alone is full, the deferred faces have
+// no pass-A record and skip their con2prim: p_hll then reaches only the
+// fallback sentinel of p* (written as -1 - 1e-10 there: negative, as every
+// fallback's), and the rest of the solver is unchanged.
 template <int N, int LIM, int SOL, bool LOCAL = false>
 __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e,
                                                  const C2PRec* rec, const srmhd_flux_diag* d,
                                                  srmhd_flux_diag* dsh, Flx F, double* pstar_out,
-                                                 int emf_hll, double tol) {
+                                                 int emf_hll, double tol,
+                                                 const unsigned long long* mres = nullptr) {
   static_assert(!LOCAL || SOL == SOLVER_HLLD, "the local con2prim concerns HLLD only");
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = t < n_faces<N>(g);
@@ -225,12 +267,18 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
     } else {
       HlldPrep<N> h;
       h.run(l, r, e);
-      Kastaun k;
-      k.init(h.uH, e);
-      Prim ph;
-      const int cst = LOCAL ? c2p_single(k, 1e-10, 1e-10, ph)
-                            : c2p_pass_b(k, rec[t], c2p_break_iter(d->c2p_mask), 1e-10, 1e-10, ph);
-      st = hlld_finish<N>(h, l, r, p_hll_from_prim(ph, e), f, nullptr, ps, iters, tol);
+      double p_hll = 1.0;
+      int cst = 0;
+      if (!(!LOCAL && mres && c2p_mask_full(d->c2p_mask) && hll_c2p_inert<N>(h))) {
+        Kastaun k;
+        k.init(h.uH, e);
+        Prim ph;
+        const uint64_t mask = d->c2p_mask | (mres ? *mres : 0ull);
+        cst = LOCAL ? c2p_single(k, 1e-10, 1e-10, ph)
+                    : c2p_pass_b(k, rec[t], c2p_break_iter(mask), 1e-10, 1e-10, ph);
+        p_hll = p_hll_from_prim(ph, e);
+      }
+      st = hlld_finish<N>(h, l, r, p_hll, f, nullptr, ps, iters, tol);
       if (cst & C2P_REDONE) st |= FS_C2P_REDONE;
     }
     F.p[0][t] = f[CD];
@@ -657,6 +705,8 @@ bool grid_ok(const srmhd_grid2d* g, int limiter) {
   return g->dx > 0.0 && g->dy > 0.0;
 }
 
+constexpr int N_STAGE_WORDS = 10;
+
 struct Ws2 {
   double* psx = nullptr;        // optional p* tape of the x and y sweeps (not carved)
   double* psy = nullptr;
@@ -665,7 +715,8 @@ struct Ws2 {
   C2PRec *recx, *recy, *recc;
   srmhd_flux_diag* diag;        // [0] x sweep, [1] y sweep
   unsigned long long* words;    // [0] cell c2p mask, [1..2] dt maxima, [3] c2p status OR,
-                                // [4..7] the linear solver's |F_D| maxima and counts
+                                // [4..7] the linear solver's |F_D| maxima and counts,
+                                // [8..9] the deferred faces' con2prim masks (x, y)
   Cells32 cf{};                 // FP32 cell variables of the MIXED form (carved for it only)
   srmhd_flux_diag* dsh;         // DIAG_SHARDS copies of diag[0], then of diag[1]
   int emf_hll = 0;              // the scheme's emf_mode is SRMHD_EMF_HLL
@@ -679,7 +730,7 @@ Ws2 carve2d(const G2& g, void* work, bool cells32 = false) {
   const int64_t nfx = (int64_t)(g.nxt + 1) * g.nyt, nfy = (int64_t)g.nxt * (g.nyt + 1);
   const int64_t ncell = (int64_t)g.nxt * g.nyt, ncor = (int64_t)(g.nxt + 1) * (g.nyt + 1);
   w.diag = c.take<srmhd_flux_diag>(2);
-  w.words = c.take<unsigned long long>(8);
+  w.words = c.take<unsigned long long>(N_STAGE_WORDS);
   for (int k = 0; k < 7; ++k) w.Fx.p[k] = c.take<double>(nfx);
   for (int k = 0; k < 7; ++k) w.Fy.p[k] = c.take<double>(nfy);
   w.Ez = c.take<double>(ncor);
@@ -698,6 +749,9 @@ constexpr int DIAG_SHARD_WORDS = 2 * DIAG_SHARDS * DIAG_SHARD_BYTES / 8;
 constexpr int DIAG_ZERO_BLOCKS = (DIAG_SHARD_WORDS + 255) / 256;
 static_assert(N_WORDS <= DIAG_ZERO_BLOCKS * 256, "the stage's zeroing grid covers the diagnostics");
 
+// workgroups of k_sweep_a_rescue (grid-stride over the faces)
+constexpr int RESCUE_BLOCKS = 1024;
+
 // the two sweeps of ct_stage for one (limiter, solver)
 template <int LIM, int SOL>
 void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const Ws2& w,
@@ -706,25 +760,33 @@ void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const W
   const int by = blocks_for((int64_t)g.nxt * (g.nyt + 1));
   srmhd_flux_diag* shx = w.dsh;
   srmhd_flux_diag* shy = (srmhd_flux_diag*)((char*)w.dsh + DIAG_SHARDS * DIAG_SHARD_BYTES);
-  if (SOL == SOLVER_HLLD) {
+  // HLLD: pass A with the inert faces deferred, the batch mask folded, the
+  // deferred faces' pass A only if that mask is not full, then pass B
+  const bool hd = SOL == SOLVER_HLLD;
+  unsigned long long* mres = hd ? w.words + 8 : nullptr;
+  if (hd) {
     tbegin(SRMHD_K_SWEEP_X_A, s);
-    k_sweep_a<0, LIM><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, shx);
+    k_sweep_a<0, LIM, true><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, shx);
     tend(SRMHD_K_SWEEP_X_A, s);
     k_fold_diag<<<1, 64, 0, s>>>(shx, w.diag + 0);   // the batch mask for pass B
+    k_sweep_a_rescue<0, LIM><<<bx < RESCUE_BLOCKS ? bx : RESCUE_BLOCKS, BLK, 0, s>>>(
+        g, S, P, e, w.recx, w.diag + 0, mres);
   }
   tbegin(SRMHD_K_SWEEP_X_B, s);
   k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, shx, w.Fx, w.psx,
-                                             w.emf_hll, w.o.tol);
+                                             w.emf_hll, w.o.tol, mres);
   tend(SRMHD_K_SWEEP_X_B, s);
-  if (SOL == SOLVER_HLLD) {
+  if (hd) {
     tbegin(SRMHD_K_SWEEP_Y_A, s);
-    k_sweep_a<1, LIM><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, shy);
+    k_sweep_a<1, LIM, true><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, shy);
     tend(SRMHD_K_SWEEP_Y_A, s);
     k_fold_diag<<<1, 64, 0, s>>>(shy, w.diag + 1);
+    k_sweep_a_rescue<1, LIM><<<by < RESCUE_BLOCKS ? by : RESCUE_BLOCKS, BLK, 0, s>>>(
+        g, S, P, e, w.recy, w.diag + 1, mres + 1);
   }
   tbegin(SRMHD_K_SWEEP_Y_B, s);
   k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, shy, w.Fy, w.psy,
-                                             w.emf_hll, w.o.tol);
+                                             w.emf_hll, w.o.tol, hd ? mres + 1 : nullptr);
   tend(SRMHD_K_SWEEP_Y_B, s);
   k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
 }
@@ -1394,7 +1456,7 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
     z.p[1] = (unsigned long long*)w.dsh;
     z.n[1] = DIAG_SHARD_WORDS;
     z.p[2] = w.words;
-    z.n[2] = 8;
+    z.n[2] = N_STAGE_WORDS;
     z.p[3] = mixed ? (unsigned long long*)w.recx : nullptr;
     z.n[3] = 1;
     z.p[4] = mixed ? (unsigned long long*)w.recy : nullptr;

@@ -35,6 +35,20 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
 }
 __device__ inline bool lane0() { return (threadIdx.x & 63) == 0; }
 
+// Appends idx to the work list q for the lanes with want set: one atomic per
+// wave on the count *n.  Every lane of the wave that is still running calls
+// it.  The order of the entries is unspecified.
+__device__ inline void queue_push(int64_t* q, unsigned long long* n, int64_t idx, bool want) {
+  const unsigned long long m = __ballot(want);
+  if (!m) return;
+  const int lane = (int)(threadIdx.x & 63);
+  const int leader = __builtin_ctzll(m);
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(n, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  if (want) q[base + __popcll(m & ((1ull << lane) - 1ull))] = idx;
+}
+
 __device__ inline unsigned long long dbits(double x) { return __double_as_longlong(x); }
 
 __device__ inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }

@@ -263,6 +263,32 @@ SRMHD_DEV T hll_emf_flux(const PrimT<T>& L, const PrimT<T>& R, const srmhd_eos& 
   return hll_flux(fL[c], fR[c], uL[c], uR[c], cmin, cmax);
 }
 
+// Whether a face's HLL-state con2prim is inert in the 2D stage: it can then
+// only reach the batch's break iteration, never the flux.  hlld_assemble
+// falls back to the HLLE flux where |B_n| < 1e-14 on both sides (hlld.py:
+// 1303, 1513) whatever p* is; p_hll (the con2prim's total pressure) enters
+// only the p* bracket's use_p0 test, 0.5 (B_nL^2 + B_nR^2) / p_hll < 0.1
+// (hlld.py:1295-1301), and the fallback sentinel -|p_hll| - 1e-10.  The test
+// holds for every value the con2prim can return when the HLL state is
+// moderate: with 1e-20 <= D <= 1e20 and every other component within 1e20
+// the con2prim's arithmetic stays finite (mu in [0, 1], each term bounded),
+// its velocity has v^2 <= 1 - 1e-10, so p_hll >= 0.5 b^2 >= 0.5 B^2 / W^2 >=
+// 5e-11 B^2 (p >= 0), and B^2 >= 1e13 * 0.5 (B_nL^2 + B_nR^2), B^2 >= 1e-80
+// put the ratio below 1e-2 with p_hll > 0.  The bracket then starts from p0,
+// independent of p_hll, and the secant's iteration counts are the same.
+template <int N>
+SRMHD_DEV bool hll_c2p_inert(const HlldPrep<N>& h) {
+  const double BnL = h.uL[CB + N], BnR = h.uR[CB + N];
+  if (!((fabs(BnL) < 1e-14) && (fabs(BnR) < 1e-14))) return false;
+  const double* u = h.uH;
+  bool ok = u[CD] >= 1e-20 && u[CD] <= 1e20;
+#pragma unroll
+  for (int k = 1; k < 8; ++k) ok = ok && fabs(u[k]) <= 1e20;
+  const double B2 = sq(u[CB]) + sq(u[CB + 1]) + sq(u[CB + 2]);
+  const double Bn2 = 0.5 * (BnL * BnL + BnR * BnR);
+  return ok && B2 >= 1e-80 && B2 >= 1e13 * Bn2;
+}
+
 // p_hll from the con2prim of the HLL state   hlld.py:1255-1263
 template <class T>
 SRMHD_DEV T p_hll_from_prim(const PrimT<T>& P, const srmhd_eos& e) {

@@ -13,6 +13,11 @@ struct TubeState {
   unsigned long long* mf;    // HLL-state con2prim mask of the stage, per tube
   unsigned long long* mc;    // cell con2prim mask of the stage, per tube
   double* tacc;              // elapsed time of this run, per tube
+  // work lists of the elements whose con2prim takes the bisection redo
+  // (faces, cells) and their counts qn[0], qn[1]
+  int64_t* qf;
+  int64_t* qc;
+  unsigned long long* qn;
 };
 
 struct TubeArgs {

@@ -54,6 +54,9 @@ WsT carve_tubes(int64_t cells, int64_t faces, void* work) {
   w.ts.mf = c.take<unsigned long long>(n);
   w.ts.mc = c.take<unsigned long long>(n);
   w.ts.tacc = c.take<double>(n);
+  w.ts.qf = c.take<int64_t>(faces);
+  w.ts.qc = c.take<int64_t>(cells);
+  w.ts.qn = c.take<unsigned long long>(2);
   w.bytes = c.off;
   return w;
 }

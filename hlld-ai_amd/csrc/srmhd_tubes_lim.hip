@@ -376,8 +376,10 @@ __global__ __launch_bounds__(NT) void k_tubes(TubeArgs A, srmhd_eos e, int steps
 // tubes' elements at once: each kernel keeps only its own registers live
 // (k_tubes holds every phase's at once and runs one workgroup per CU), and
 // no workgroup waits at a barrier for its slowest wave.  A tube's con2prim
-// masks and its dt live in per-tube words between the launches.  A step is
-// 2 + 4 x stages launches; a run is one stream of them (one graph when
+// masks and its dt live in per-tube words between the launches.  The
+// elements whose con2prim takes the bisection redo are finished by their own
+// launches from work lists.  A step is 1 + 6 x stages launches with HLLD
+// (1 + 4 x stages otherwise); a run is one stream of them (one graph when
 // captured).
 
 // the tube of global cell i (off_c[t] <= i < off_c[t+1]) or global face i
@@ -445,6 +447,7 @@ template <int LIM>
 __global__ __launch_bounds__(BLK) void k_tp_flux_a(TubeArgs A, TubeState S, int n, int64_t nf,
                                                    int bu, srmhd_eos e) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i == 0) S.qn[0] = 0ull;   // the face work list of this stage starts empty
   const bool active = i < nf;
   int tb = 0;
   uint64_t m = 0;
@@ -466,21 +469,45 @@ __global__ __launch_bounds__(BLK) void k_tp_flux_a(TubeArgs A, TubeState S, int 
   tube_or(S.mf, tb, m, active);
 }
 
+__device__ inline void put_flux(const TubeArgs& A, int64_t i, const double fl[8]) {
+  A.F[0][i] = fl[CD];
+  A.F[1][i] = fl[CS];
+  A.F[2][i] = fl[CS + 1];
+  A.F[3][i] = fl[CS + 2];
+  A.F[4][i] = fl[CT];
+  A.F[5][i] = fl[CB + 1];
+  A.F[6][i] = fl[CB + 2];
+}
+
+// the tube and the face states of global face i
+template <int LIM>
+__device__ inline int tp_face(const TubeArgs& A, int n, int64_t i, int bu, const srmhd_eos& e,
+                              Prim& l, Prim& r, int& f) {
+  const int tb = tube_of<true>(A.off_c, n, i);
+  const int64_t oc = A.off_c[tb];
+  const int nt = (int)(A.off_c[tb + 1] - oc);
+  f = (int)(i - oc - tb);
+  const double* P[6];
+  tube_prims(A, oc, P);
+  tube_face<LIM>(P, A.U[bu][5] + oc, A.U[bu][6] + oc, A.Bn[tb], f, nt, e, l, r);
+  return tb;
+}
+
 // the flux on every face of every tube; the first face of a tube clears the
-// tube's cell mask for this stage
+// tube's cell mask for this stage.  HLLD faces whose HLL-state con2prim takes
+// the bisection redo (unconverged after the batch's K secant steps, with a
+// good bracket) go to the face work list instead: up to half of a tube's
+// elements take it, and a wave would otherwise run the ~55-step bisection
+// for its few lanes that need it.
 template <int LIM, int SOL>
 __global__ __launch_bounds__(BLK) void k_tp_flux_b(TubeArgs A, TubeState S, int n, int64_t nf,
                                                    int bu, srmhd_eos e) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i >= nf) return;
-  const int tb = tube_of<true>(A.off_c, n, i);
-  const int64_t oc = A.off_c[tb];
-  const int nt = (int)(A.off_c[tb + 1] - oc);
-  const int f = (int)(i - oc - tb);
-  const double* P[6];
-  tube_prims(A, oc, P);
   Prim l, r;
-  tube_face<LIM>(P, A.U[bu][5] + oc, A.U[bu][6] + oc, A.Bn[tb], f, nt, e, l, r);
+  int f;
+  const int tb = tp_face<LIM>(A, n, i, bu, e, l, r, f);
+  if (f == 0) S.mc[tb] = 0ull;
   double fl[8], u[8], ps;
   if (SOL == SOLVER_HLLE) {
     hlle<0>(l, r, e, fl, u, ps);
@@ -491,19 +518,44 @@ __global__ __launch_bounds__(BLK) void k_tp_flux_b(TubeArgs A, TubeState S, int 
     h.run(l, r, e);
     Kastaun k;
     k.init(h.uH, e);
+    const C2PRecK& rec = A.recf[i];
+    double x1, f1;
+    c2p_secant_at(k, rec, c2p_break_iter(S.mf[tb]), x1, f1);
+    const bool redo = !(fabs(f1) < C2P_TOL) && rec.mup_signed > 0.0;
+    queue_push(S.qf, &S.qn[0], i, redo);
+    if (redo) return;
     Prim ph;
-    c2p_pass_b(k, A.recf[i], c2p_break_iter(S.mf[tb]), 1e-10, 1e-10, ph, -1);
+    c2p_pass_at(k, rec, x1, f1, 1e-10, 1e-10, ph, 0);
     int it;
     hlld_finish<0>(h, l, r, p_hll_from_prim(ph, e), fl, nullptr, ps, it);
   }
-  A.F[0][i] = fl[CD];
-  A.F[1][i] = fl[CS];
-  A.F[2][i] = fl[CS + 1];
-  A.F[3][i] = fl[CS + 2];
-  A.F[4][i] = fl[CT];
-  A.F[5][i] = fl[CB + 1];
-  A.F[6][i] = fl[CB + 2];
-  if (f == 0) S.mc[tb] = 0ull;
+  put_flux(A, i, fl);
+}
+
+// the HLLD flux of the faces on the work list (the bisection redo of their
+// HLL-state con2prim, then the rest of the solver)
+template <int LIM>
+__global__ __launch_bounds__(BLK) void k_tp_flux_redo(TubeArgs A, TubeState S, int n, int bu,
+                                                      srmhd_eos e) {
+  const int64_t cnt = (int64_t)S.qn[0];
+  for (int64_t q = (int64_t)blockIdx.x * BLK + threadIdx.x; q < cnt;
+       q += (int64_t)gridDim.x * BLK) {
+    const int64_t i = S.qf[q];
+    Prim l, r;
+    int f;
+    tp_face<LIM>(A, n, i, bu, e, l, r, f);
+    HlldPrep<0> h;
+    h.run(l, r, e);
+    Kastaun k;
+    k.init(h.uH, e);
+    const C2PRecK& rec = A.recf[i];
+    Prim ph;
+    c2p_pass_at(k, rec, rec.x1, rec.f1, 1e-10, 1e-10, ph, 1);
+    double fl[8], ps;
+    int it;
+    hlld_finish<0>(h, l, r, p_hll_from_prim(ph, e), fl, nullptr, ps, it);
+    put_flux(A, i, fl);
+  }
 }
 
 // the stage combination a U0 + b U + (c dt) L of every cell, then its
@@ -512,6 +564,7 @@ __global__ __launch_bounds__(BLK) void k_tp_update_c2pa(TubeArgs A, TubeState S,
                                                         int64_t nc, int bu, int bo, double a,
                                                         double b, double cw, srmhd_eos e) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i == 0) S.qn[1] = 0ull;   // the cell work list of this stage starts empty
   const bool active = i < nc;
   int tb = 0;
   uint64_t m = 0;
@@ -547,10 +600,10 @@ __global__ __launch_bounds__(BLK) void k_tp_update_c2pa(TubeArgs A, TubeState S,
 // cell's (two-cell-copy ghosts, tube_seed.py:72-75: the same values, formed
 // again) and copies its conserved By, Bz; the first cell of a tube clears
 // the tube's face mask for the next stage
-__global__ __launch_bounds__(BLK) void k_tp_c2p_b(TubeArgs A, TubeState S, int n, int64_t nc,
-                                                  int bo, double atmo_rho, srmhd_eos e) {
-  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (i >= nc) return;
+// the conserved state of global cell i's con2prim source cell (itself, or
+// for a ghost the physical cell it copies) -> the tube; *si = that cell
+__device__ inline int tp_cell(const TubeArgs& A, int n, int64_t i, int bo, double u[8],
+                              int64_t* si, bool* ghost, bool* first) {
   const int tb = tube_of<false>(A.off_c, n, i);
   const int64_t oc = A.off_c[tb];
   const int nt = (int)(A.off_c[tb + 1] - oc);
@@ -558,31 +611,75 @@ __global__ __launch_bounds__(BLK) void k_tp_c2p_b(TubeArgs A, TubeState S, int n
   const int g = (nt - L) / 2;
   const int c = (int)(i - oc);
   const int src = c < g ? g : (c >= g + L ? g + L - 1 : c);
-  const int64_t si = oc + src;
-  double u[8];
-  u[CD] = A.U[bo][0][si];
-  u[CT] = A.U[bo][4][si];
-  u[CS] = A.U[bo][1][si];
-  u[CS + 1] = A.U[bo][2][si];
-  u[CS + 2] = A.U[bo][3][si];
+  *si = oc + src;
+  *ghost = src != c;
+  *first = c == 0;
+  u[CD] = A.U[bo][0][*si];
+  u[CT] = A.U[bo][4][*si];
+  u[CS] = A.U[bo][1][*si];
+  u[CS + 1] = A.U[bo][2][*si];
+  u[CS + 2] = A.U[bo][3][*si];
   u[CB] = A.Bn[tb];
-  u[CB + 1] = A.U[bo][5][si];
-  u[CB + 2] = A.U[bo][6][si];
-  Kastaun k;
-  k.init(u, e);
-  Prim q;
-  c2p_pass_b(k, A.recc[si], c2p_break_iter(S.mc[tb]), atmo_rho, 1e-10, q, -1);
+  u[CB + 1] = A.U[bo][5][*si];
+  u[CB + 2] = A.U[bo][6][*si];
+  return tb;
+}
+
+__device__ inline void put_prims(const TubeArgs& A, int64_t i, const Prim& q) {
   A.P[0][i] = q.rho;
   A.P[1][i] = q.v[0];
   A.P[2][i] = q.v[1];
   A.P[3][i] = q.v[2];
   A.P[4][i] = q.p;
   A.P[5][i] = q.eps;
-  if (src != c) {
+}
+
+// Cells whose con2prim takes the bisection redo go to the cell work list
+// (as the faces in k_tp_flux_b).
+__global__ __launch_bounds__(BLK) void k_tp_c2p_b(TubeArgs A, TubeState S, int n, int64_t nc,
+                                                  int bo, double atmo_rho, srmhd_eos e) {
+  const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (i >= nc) return;
+  double u[8];
+  int64_t si;
+  bool ghost, first;
+  const int tb = tp_cell(A, n, i, bo, u, &si, &ghost, &first);
+  if (ghost) {
     A.U[bo][5][i] = u[CB + 1];
     A.U[bo][6][i] = u[CB + 2];
   }
-  if (c == 0) S.mf[tb] = 0ull;
+  Kastaun k;
+  k.init(u, e);
+  const C2PRecK& rec = A.recc[si];
+  double x1, f1;
+  c2p_secant_at(k, rec, c2p_break_iter(S.mc[tb]), x1, f1);
+  if (first) S.mf[tb] = 0ull;
+  const bool redo = !(fabs(f1) < C2P_TOL) && rec.mup_signed > 0.0;
+  queue_push(S.qc, &S.qn[1], i, redo);
+  if (redo) return;
+  Prim q;
+  c2p_pass_at(k, rec, x1, f1, atmo_rho, 1e-10, q, 0);
+  put_prims(A, i, q);
+}
+
+// con2prim of the cells on the work list (the bisection redo)
+__global__ __launch_bounds__(BLK) void k_tp_c2p_redo(TubeArgs A, TubeState S, int n, int bo,
+                                                     double atmo_rho, srmhd_eos e) {
+  const int64_t cnt = (int64_t)S.qn[1];
+  for (int64_t q = (int64_t)blockIdx.x * BLK + threadIdx.x; q < cnt;
+       q += (int64_t)gridDim.x * BLK) {
+    const int64_t i = S.qc[q];
+    double u[8];
+    int64_t si;
+    bool ghost, first;
+    tp_cell(A, n, i, bo, u, &si, &ghost, &first);
+    Kastaun k;
+    k.init(u, e);
+    const C2PRecK& rec = A.recc[si];
+    Prim p;
+    c2p_pass_at(k, rec, rec.x1, rec.f1, atmo_rho, 1e-10, p, 1);
+    put_prims(A, i, p);
+  }
 }
 
 __global__ __launch_bounds__(BLK) void k_tp_final(TubeArgs A, TubeState S, int n) {
@@ -750,6 +847,10 @@ int run_nt(int n, const TubeArgs& A, const srmhd_eos& e, int steps, double cfl, 
   return launch_status();
 }
 
+// workgroups of the work-list kernels (grid-stride over the list, whose
+// length is known only on the device)
+constexpr int REDO_BLOCKS = 2048;
+
 template <int LIM, int SOL>
 int run_phased(int n, const TubeArgs& A, const srmhd_eos& e, int steps, double cfl, int rk3,
                double atmo, hipStream_t st) {
@@ -766,10 +867,13 @@ int run_phased(int n, const TubeArgs& A, const srmhd_eos& e, int steps, double c
       const int bo = (s == nst - 1) ? 0 : 1 + (s % 2);
       if (SOL == SOLVER_HLLD) k_tp_flux_a<LIM><<<bf, BLK, 0, st>>>(A, S, n, faces, bu, e);
       k_tp_flux_b<LIM, SOL><<<bf, BLK, 0, st>>>(A, S, n, faces, bu, e);
+      if (SOL == SOLVER_HLLD)
+        k_tp_flux_redo<LIM><<<(bf < REDO_BLOCKS ? bf : REDO_BLOCKS), BLK, 0, st>>>(A, S, n, bu, e);
       double a, b, c;
       rk_weights(rk3, s, a, b, c);
       k_tp_update_c2pa<<<bc, BLK, 0, st>>>(A, S, n, cells, bu, bo, a, b, c, e);
       k_tp_c2p_b<<<bc, BLK, 0, st>>>(A, S, n, cells, bo, atmo, e);
+      k_tp_c2p_redo<<<(bc < REDO_BLOCKS ? bc : REDO_BLOCKS), BLK, 0, st>>>(A, S, n, bo, atmo, e);
       bu = bo;
     }
     if (const int rc = launch_status()) return rc;

@@ -285,6 +285,50 @@ def test_2d_sweep_diag_equals_oracle(solver, emf):
             assert d["mean_iters"] == do["mean_iters"], ax
 
 
+def test_2d_all_faces_inert_bitwise_vs_oracle():
+    """The HLLD sweeps leave out of pass A the faces whose HLL-state
+    con2prim cannot reach the flux (B_n = 0 on both sides, moderate HLL
+    state) and run them afterwards only when the other faces' mask leaves
+    the batch's break iteration below 60.  A field along z alone makes every
+    face such a face, so every sweep takes that second pass (on the rotors
+    of the other cases the y faces are deferred and skipped): 3 RK3 steps of
+    a 16^2 tapered rotor with B = (0, 0, 1), bitwise vs the oracle, and the
+    last stage's counters equal."""
+    from srmhd import driver2d as D2, eos as E, grid as G, initial_data2d as I
+    import oracle.eos as OE
+    import oracle.riemann as OR
+    import oracle.step2d as OS
+    _flush("2d all faces inert (B along z)")
+    dev = _dev()
+    n, lim = 16, "weno5z"
+    eos = E.hybrid_eos(0.0, 5 / 3, 5 / 3)
+    grid = G.Grid2D(-0.5, 0.5, n, -0.5, 0.5, n, ng=3, device=dev)
+    P, Az = I.rotor_tapered(grid, eos)
+    oe = OE.ideal(5 / 3)
+    go = OS.make_grid(n, (-0.5, 0.5), lim)
+    Po, Azo = OS.rotor_tapered(go, oe)
+    for d in (P, Po):
+        d["Bx"] = torch.zeros_like(d["Bx"])
+        d["Bz"] = torch.ones_like(d["Bz"])
+    st = I.init_state(grid, eos, P, torch.zeros_like(Az), limiter=lim)
+    so = OS.init_state(go, oe, Po, torch.zeros_like(Azo))
+    for _ in range(3):
+        dt = D2.compute_dt_2d(st.prims, grid, eos, 0.4)
+        st, diag = D2.rk_step_ct(st, grid, eos, dt, scheme="rk3", flux_fn="hlld", limiter=lim)
+        dto = OS.compute_dt_2d(so.prims, go, oe, 0.4)
+        so, aux = OS.rk_step_ct(so, go, oe, dto, scheme="rk3", flux_fn=OR.SOLVERS["hlld"],
+                                limiter=lim)
+        assert dt == dto
+    torch.cuda.synchronize()
+    _state_eq(st, so, "oracle")
+    for ax in ("x", "y"):
+        d, do = dict(diag[ax]["diag"]), dict(aux[ax]["diag"])
+        assert d["n_bn_zero"] == do["n_bn_zero"] == d["n_interfaces"], ax
+        for key in ("n_not_converged", "n_wave_order_bad", "n_hlle_fallback", "max_iters"):
+            assert d[key] == do[key], (ax, key)
+        assert d["mean_iters"] == do["mean_iters"], ax
+
+
 def test_periodic_ot_emf_hll_bitwise_vs_oracle():
     _flush("2d ot2pi mc hlld emf hll")
     args = ("orszag_tang_2pi", 16, "mc", "hlld", "rk2", 4 / 3, (0.0, 2 * np.pi), "periodic")
