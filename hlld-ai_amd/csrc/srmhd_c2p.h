@@ -36,6 +36,12 @@ template <class T>
 struct KastaunT {
   T D, q, r[3], r2, bt[3], bt2, br, bperp2, v02, B[3];
   srmhd_eos e;
+  // The element's master function may be evaluated by master_fast: an
+  // ideal-gas EOS and moderate coefficients, 1e-30 <= D <= 1e30, |q|, r^2,
+  // b~^2, (B.r)^2-term <= 1e30 (all finite).  See master_fast for why this
+  // puts five of its divisions and its square root in the range where the
+  // shortened forms are exact.
+  bool tame;
 
   // c2p.py:48-106; cons order D, tau, Sx, Sy, Sz, Bx, By, Bz
   SRMHD_DEV void init(const T c[8], const srmhd_eos& eos) {
@@ -67,6 +73,11 @@ struct KastaunT {
     br = BS / (D * sD + TINY);
     bperp2 = r2 * bt2 - br * br;
     v02 = r2 / (r2 + 1.0);
+    const double dD = val(D), dq = val(q), dr2 = val(r2), dbt2 = val(bt2), dbr = val(br);
+    tame = e.K == 0.0 && e.gamma > 1.0 && e.gamma <= 3.0 &&
+           (e.is_cold || (e.gamma_th > 1.0 && e.gamma_th <= 10.0)) && dD >= 1e-30 &&
+           dD <= 1e30 && fabs(dq) <= 1e30 && dr2 >= 0.0 && dr2 <= 1e30 && dbt2 >= 0.0 &&
+           dbt2 <= 1e30 && dbr * dbr <= 1e30;
   }
 
   template <class M>
@@ -103,6 +114,45 @@ struct KastaunT {
     if (eps_out) *eps_out = eps;
     return mu - sdiv(1.0, nu + mu * rb2);
   }
+
+  // master() for a tame element, bit for bit.  mu is in [0, 1] (every
+  // secant and bisection iterate lies in the initial bracket [0, mu_+]).
+  // Then, with the bounds of `tame`:
+  //   chi = 1/(1 + mu b~^2):            divisor in [1, 1 + 1e30]
+  //   vh^2 in [0, 1], so clamp(1 - vh^2, min=1e-10) is in [1e-10, 1]: its
+  //   square root is in [1e-5, 1], and Wh = 1/sqrt in [1, 1e5]
+  //   D / Wh:                           D in [1e-30, 1e30]
+  //   rho in [1e-12, 1e15]; K = 0 gives eps_lo = 0 and p = (g_th - 1) rho
+  //   eps >= 0 with eps in [0, 1e5], so a = p / (...) >= 0 and
+  //   nuA = (1 + a)(1 + eps) / Wh:      dividend in [1, 1e7], divisor Wh
+  //   nu = max(nuA, nuB) >= nuA >= 1e-5 and |qb|, rbar^2 <= 1e61, so
+  //   1 / (nu + mu rbar^2):             divisor in [1e-5, 1e62]
+  // -- all inside div_nr's range, and the square root inside csqrt_mid's.
+  // The two divisions whose dividend can be arbitrarily small (vh^2 Wh^2 and
+  // p) stay IEEE divisions.
+  SRMHD_DEV double master_fast(double mu, double* rho_out = nullptr,
+                               double* eps_out = nullptr) const {
+    const double c = div_nr(1.0, 1.0 + mu * bt2);
+    const double mc = mu * c;
+    const double qb = q - 0.5 * bt2 - 0.5 * (mc * mc) * bperp2;
+    const double rb2 = rbar2(mu, c);
+    const double vh2 = chi_(mu * mu * rb2, v02);
+    const double Wh = div_nr(1.0, csqrt_mid(clo(1.0 - vh2, 1e-10)));
+    const double rho = tmin(tmax(div_nr(D, Wh), 1e-12), 1e15);
+    const double eps0 = Wh * (qb - mu * rb2) + vh2 * (Wh * Wh) / (1.0 + Wh);
+    const double eps = tmin(tmax(eps0, eos_eps_lo(e, rho)), 1e5);
+    const double p = eos_press(e, eps, rho);
+    const double a = p / (rho * (1.0 + eps) + TINY);
+    const double nuA = div_nr((1.0 + a) * (1.0 + eps), Wh);
+    const double nuB = (1.0 + a) * (1.0 + qb - mu * rb2);
+    const double nu = tmax(nuA, nuB);
+    if (rho_out) *rho_out = rho;
+    if (eps_out) *eps_out = eps;
+    return mu - div_nr(1.0, nu + mu * rb2);
+  }
+
+  template <bool FAST>
+  SRMHD_DEV double master_at(double mu) const { return FAST ? master_fast(mu) : master(mu); }
 
   // 50-step bisection of fa on [0, 1] (c2p.py:162-184).  The reference
   // re-evaluates fa(lo) every iteration; fa is deterministic, so carrying
@@ -174,13 +224,15 @@ struct KastaunT {
     return hi;
   }
 
-  // one guarded secant step   c2p.py:219-232
+  // one guarded secant step   c2p.py:219-232 (FAST: master_fast, tame
+  // elements only)
+  template <bool FAST = false>
   SRMHD_DEV void secant_step(double& x0, double& f0, double& x1, double& f1) const {
     const double df = f1 - f0;
     const double mid = 0.5 * (x0 + x1);
     double x2 = (fabs(df) > 1e-30) ? x1 - f1 * (x1 - x0) / df : mid;
     x2 = ((x2 - x0) * (x2 - x1) < 0.0) ? x2 : mid;
-    const double f2 = master(x2);
+    const double f2 = master_at<FAST>(x2);
     if (!((f0 * f2) <= 0.0)) {
       x0 = x1;
       f0 = f1;
@@ -192,9 +244,10 @@ struct KastaunT {
   // secant_step, returning whether it left (x0, f0, x1, f1) unchanged bit for
   // bit: the walk depends on that state only, so it is then at a fixed point
   // and every later step leaves it unchanged too
+  template <bool FAST = false>
   SRMHD_DEV bool secant_step_fixed(double& x0, double& f0, double& x1, double& f1) const {
     const double a0 = x0, b0 = f0, a1 = x1, b1 = f1;
-    secant_step(x0, f0, x1, f1);
+    secant_step<FAST>(x0, f0, x1, f1);
     return same_bits(a0, x0) && same_bits(b0, f0) && same_bits(a1, x1) && same_bits(b1, f1);
   }
 
@@ -202,13 +255,14 @@ struct KastaunT {
   // Once the bracket has shrunk to adjacent doubles (about 55 halvings for a
   // root of order one) the midpoint is an end point and a step maps the state
   // to itself; the loop stops there, with the 200-step result.
+  template <bool FAST = false>
   SRMHD_DEV double bisection(double mup, double& fmu) const {
     double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
-    double f0 = master(x0), f1 = master(x1);
+    double f0 = master_at<FAST>(x0), f1 = master_at<FAST>(x1);
 #pragma unroll 1
     for (int it = 0; it < C2P_BISECT; ++it) {
       const double xm = 0.5 * (x0 + x1);
-      const double fm = master(xm);
+      const double fm = master_at<FAST>(xm);
       bool fixed;
       if ((f0 * fm) <= 0.0) {
         fixed = same_bits(x1, xm) && same_bits(f1, fm);
@@ -260,17 +314,18 @@ struct C2PRec {
 };
 
 // Pass A.  Returns the element's not-converged mask (bit k: after k steps).
-SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRec& rec) {
+template <bool FAST>
+SRMHD_DEV uint64_t c2p_pass_a_impl(const Kastaun& k, C2PRec& rec) {
   const double mup = k.mu_plus_fast();   // bit-identical to k.mu_plus()
   double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
-  double f0 = k.master(x0), f1 = k.master(x1);
+  double f0 = k.master_at<FAST>(x0), f1 = k.master_at<FAST>(x1);
   const bool good = (f0 * f1) < 0.0;
   uint64_t mask = 0;
 #pragma unroll 1
   for (int it = 0; it < C2P_SECANT; ++it) {
     const bool nc = !(fabs(f1) < C2P_TOL);
     if (nc) mask |= (uint64_t)1 << it;
-    if (k.secant_step_fixed(x0, f0, x1, f1)) {
+    if (k.secant_step_fixed<FAST>(x0, f0, x1, f1)) {
       // a fixed point: every later step sees this state, so its bits repeat
       if (nc) mask |= C2P_ALL & ~(((uint64_t)2 << it) - 1);
       break;
@@ -280,6 +335,13 @@ SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRec& rec) {
   rec.x1 = x1;
   rec.f1 = f1;
   return mask;
+}
+
+// The wrappers pick the tame form (master_fast) when every active lane of
+// the wave holds a tame element; it returns the same bits, in fewer
+// instructions.
+SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRec& rec) {
+  return wave_all(k.tame) ? c2p_pass_a_impl<true>(k, rec) : c2p_pass_a_impl<false>(k, rec);
 }
 
 // Pass-A record for batches that often break early (a tube's cells and
@@ -312,10 +374,11 @@ SRMHD_DEV void rec_put(double* c, double x0, double f0, double x1, double f1) {
   c[3] = f1;
 }
 
-SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRecK& rec) {
+template <bool FAST>
+SRMHD_DEV uint64_t c2p_pass_a_impl(const Kastaun& k, C2PRecK& rec) {
   const double mup = k.mu_plus_fast();
   double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
-  double f0 = k.master(x0), f1 = k.master(x1);
+  double f0 = k.master_at<FAST>(x0), f1 = k.master_at<FAST>(x1);
   const bool good = (f0 * f1) < 0.0;
   uint64_t mask = 0;
   int ke = C2P_SECANT, jf = C2P_SECANT;
@@ -330,7 +393,7 @@ SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRecK& rec) {
     }
     if (it == C2P_CK1) rec_put(rec.ck[1], x0, f0, x1, f1);
     if (it == C2P_CK2) rec_put(rec.ck[2], x0, f0, x1, f1);
-    if (k.secant_step_fixed(x0, f0, x1, f1)) {
+    if (k.secant_step_fixed<FAST>(x0, f0, x1, f1)) {
       if (nc) mask |= C2P_ALL & ~(((uint64_t)2 << it) - 1);
       jf = it;
       break;
@@ -344,6 +407,10 @@ SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRecK& rec) {
   return mask;
 }
 
+SRMHD_DEV uint64_t c2p_pass_a(const Kastaun& k, C2PRecK& rec) {
+  return wave_all(k.tame) ? c2p_pass_a_impl<true>(k, rec) : c2p_pass_a_impl<false>(k, rec);
+}
+
 // break iteration of the batch from the OR of every element's mask
 SRMHD_DEV int c2p_break_iter(uint64_t ormask) {
   const uint64_t zeros = ~ormask & ((((uint64_t)1) << C2P_SECANT) - 1);
@@ -354,26 +421,28 @@ SRMHD_DEV int c2p_break_iter(uint64_t ormask) {
 enum { C2P_BRACKET_OK = 1, C2P_CONVERGED = 2, C2P_V_CLAMPED = 4, C2P_ATMO = 8, C2P_REDONE = 16 };
 
 // the secant state after the batch's K steps (pass A kept the 60-step one)
-SRMHD_DEV void c2p_secant_at(const Kastaun& k, const C2PRec& rec, int K, double& x1,
-                             double& f1) {
+template <bool FAST>
+SRMHD_DEV void c2p_secant_at_impl(const Kastaun& k, const C2PRec& rec, int K, double& x1,
+                                  double& f1) {
   x1 = rec.x1;
   f1 = rec.f1;
   if (K < C2P_SECANT) {
     double x0 = 0.0;
     x1 = fabs(rec.mup_signed) * (1.0 - 1e-10);
-    double f0 = k.master(x0);
-    f1 = k.master(x1);
+    double f0 = k.master_at<FAST>(x0);
+    f1 = k.master_at<FAST>(x1);
 #pragma unroll 1
     for (int it = 0; it < K; ++it)
-      if (k.secant_step_fixed(x0, f0, x1, f1)) break;
+      if (k.secant_step_fixed<FAST>(x0, f0, x1, f1)) break;
   }
 }
 
 // the same from the checkpointed record: the final state when the walk was
 // stationary by step K, else a replay from the latest stored state at or
 // before K (K >= rec.k when K < 60)
-SRMHD_DEV void c2p_secant_at(const Kastaun& k, const C2PRecK& rec, int K, double& x1,
-                             double& f1) {
+template <bool FAST>
+SRMHD_DEV void c2p_secant_at_impl(const Kastaun& k, const C2PRecK& rec, int K, double& x1,
+                                  double& f1) {
   x1 = rec.x1;
   f1 = rec.f1;
   if (K < C2P_SECANT && K < rec.jfix) {
@@ -385,8 +454,16 @@ SRMHD_DEV void c2p_secant_at(const Kastaun& k, const C2PRecK& rec, int K, double
     f1 = c[3];
 #pragma unroll 1
     for (int it = from; it < K; ++it)
-      if (k.secant_step_fixed(x0, f0, x1, f1)) break;
+      if (k.secant_step_fixed<FAST>(x0, f0, x1, f1)) break;
   }
+}
+
+template <class Rec>
+SRMHD_DEV void c2p_secant_at(const Kastaun& k, const Rec& rec, int K, double& x1, double& f1) {
+  if (wave_all(k.tame))
+    c2p_secant_at_impl<true>(k, rec, K, x1, f1);
+  else
+    c2p_secant_at_impl<false>(k, rec, K, x1, f1);
 }
 
 // whether the element goes to the bisection redo: not converged after the
@@ -404,9 +481,9 @@ SRMHD_DEV bool c2p_needs_redo(const Kastaun& k, const Rec& rec, int K) {
 // force < 0: the element is redone by bisection when it needs it itself;
 // force >= 0: the caller decides (force != 0 redoes), for batches whose redo
 // set is wider than the elements that need it.
-template <class Rec>
-SRMHD_DEV int c2p_pass_at(const Kastaun& k, const Rec& rec, double x1, double f1,
-                          double atmo_rho, double atmo_eps, Prim& P, int force = -1) {
+template <bool FAST, class Rec>
+SRMHD_DEV int c2p_pass_at_impl(const Kastaun& k, const Rec& rec, double x1, double f1,
+                               double atmo_rho, double atmo_eps, Prim& P, int force) {
   const bool good = rec.mup_signed > 0.0;
   const double mup = fabs(rec.mup_signed);
   bool conv = fabs(f1) < C2P_TOL;
@@ -414,7 +491,7 @@ SRMHD_DEV int c2p_pass_at(const Kastaun& k, const Rec& rec, double x1, double f1
   const bool redo = force < 0 ? (!conv && good) : (force != 0);
   if (redo) {
     double fmu;
-    const double mu = k.bisection(mup, fmu);
+    const double mu = k.bisection<FAST>(mup, fmu);
     conv = fabs(fmu) < C2P_TOL;
     clamped = k.finish(mu, P);
   } else {
@@ -431,6 +508,13 @@ SRMHD_DEV int c2p_pass_at(const Kastaun& k, const Rec& rec, double x1, double f1
   P.p = eos_press(k.e, P.eps, P.rho);
   return (good ? C2P_BRACKET_OK : 0) | (conv ? C2P_CONVERGED : 0) | (clamped ? C2P_V_CLAMPED : 0) |
          (atmo ? C2P_ATMO : 0) | (redo ? C2P_REDONE : 0);
+}
+
+template <class Rec>
+SRMHD_DEV int c2p_pass_at(const Kastaun& k, const Rec& rec, double x1, double f1,
+                          double atmo_rho, double atmo_eps, Prim& P, int force = -1) {
+  return wave_all(k.tame) ? c2p_pass_at_impl<true>(k, rec, x1, f1, atmo_rho, atmo_eps, P, force)
+                          : c2p_pass_at_impl<false>(k, rec, x1, f1, atmo_rho, atmo_eps, P, force);
 }
 
 // the same from the record and the batch break iteration K
@@ -451,23 +535,24 @@ SRMHD_DEV int c2p_pass_b(const Kastaun& k, const Rec& rec, int K, double atmo_rh
 // mup_in >= 0 replaces the mu_+ bisection by a given upper bracket end (any
 // value at or above mu_+ keeps the root bracketed; the secant converges to
 // the same root from it).
-SRMHD_DEV int c2p_single(const Kastaun& k, double atmo_rho, double atmo_eps, Prim& P,
-                         double* mu_out = nullptr, double mup_in = -1.0) {
+template <bool FAST>
+SRMHD_DEV int c2p_single_impl(const Kastaun& k, double atmo_rho, double atmo_eps, Prim& P,
+                              double* mu_out, double mup_in) {
   const double mup = mup_in >= 0.0 ? mup_in : k.mu_plus_fast();
   double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
-  double f0 = k.master(x0), f1 = k.master(x1);
+  double f0 = k.master_at<FAST>(x0), f1 = k.master_at<FAST>(x1);
   const bool good = (f0 * f1) < 0.0;
 #pragma unroll 1
   for (int it = 0; it < C2P_SECANT; ++it) {
     if (fabs(f1) < C2P_TOL) break;
-    if (k.secant_step_fixed(x0, f0, x1, f1)) break;
+    if (k.secant_step_fixed<FAST>(x0, f0, x1, f1)) break;
   }
   bool conv = fabs(f1) < C2P_TOL;
   bool clamped;
   const bool redo = !conv && good;
   if (redo) {
     double fmu;
-    const double mu = k.bisection(mup, fmu);
+    const double mu = k.bisection<FAST>(mup, fmu);
     conv = fabs(fmu) < C2P_TOL;
     clamped = k.finish(mu, P);
     if (mu_out) *mu_out = mu;
@@ -486,6 +571,12 @@ SRMHD_DEV int c2p_single(const Kastaun& k, double atmo_rho, double atmo_eps, Pri
   P.p = eos_press(k.e, P.eps, P.rho);
   return (good ? C2P_BRACKET_OK : 0) | (conv ? C2P_CONVERGED : 0) | (clamped ? C2P_V_CLAMPED : 0) |
          (atmo ? C2P_ATMO : 0) | (redo ? C2P_REDONE : 0);
+}
+
+SRMHD_DEV int c2p_single(const Kastaun& k, double atmo_rho, double atmo_eps, Prim& P,
+                         double* mu_out = nullptr, double mup_in = -1.0) {
+  return wave_all(k.tame) ? c2p_single_impl<true>(k, atmo_rho, atmo_eps, P, mu_out, mup_in)
+                          : c2p_single_impl<false>(k, atmo_rho, atmo_eps, P, mu_out, mup_in);
 }
 
 }  // namespace srmhd

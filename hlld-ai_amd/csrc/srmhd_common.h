@@ -81,6 +81,65 @@ SRMHD_DEV double csqrt(double x) {
   }
   return y;
 }
+// n / d correctly rounded, where the compiler's IEEE f64 division scales
+// nothing: its own instructions (reciprocal seed, two Newton steps, the
+// quotient and one residual correction) without the scale and fix-up steps,
+// which are identities for normal operands with 2^-900 < |n|, |d|, |n/d|
+// < 2^900 (then 1/d is normal too).  Callers establish that range; the
+// result is then the IEEE quotient bit for bit.  The host build divides.
+SRMHD_DEV double div_nr(double n, double d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  const double q = n * r;
+  e = __builtin_fma(-d, q, n);
+  return __builtin_fma(e, r, q);
+#else
+  return n / d;
+#endif
+}
+
+// csqrt(x) for 2^-760 <= x <= 2^900, bit for bit: the compiler's f64 sqrt
+// expansion without its argument scaling (an identity in this range) --
+// the hardware reciprocal square root and three Newton corrections -- then
+// the same Tuckerman test.
+SRMHD_DEV double csqrt_mid(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r = __builtin_amdgcn_rsq(x);
+  double g = x * r, h = 0.5 * r;
+  const double e = __builtin_fma(-h, g, 0.5);
+  g = __builtin_fma(g, e, g);
+  h = __builtin_fma(h, e, h);
+  double d = __builtin_fma(-g, g, x);
+  g = __builtin_fma(d, h, g);
+  d = __builtin_fma(-g, g, x);
+  double y = __builtin_fma(d, h, g);
+  const long long b = __builtin_bit_cast(long long, y);
+  const double up = __builtin_bit_cast(double, b + 1);
+  const double dn = __builtin_bit_cast(double, b - 1);
+  if (__builtin_fma(y, up, -x) < 0.0) {
+    y = up;
+  } else if (__builtin_fma(y, dn, -x) >= 0.0) {
+    y = dn;
+  }
+  return y;
+#else
+  return csqrt(x);
+#endif
+}
+
+// all active lanes of the wave agree that p holds (the host build: p)
+SRMHD_DEV bool wave_all(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __all((int)p) != 0;
+#else
+  return p;
+#endif
+}
+
 template <int K>
 SRMHD_DEV Dual<K> csqrt(const Dual<K>& x) {
   Dual<K> r;

@@ -137,6 +137,56 @@ def test_c2p_bitwise_vs_oracle_and_close_to_reference(golden_dir):
         _close(P[k], g[f"out_{k}"], f"golden c2p {k}")
 
 
+def _random_cons(n, seed):
+    """Conserved states from random primitives (Gamma = 5/3): D in
+    [1e-3, 1e3], W - 1 in [1e-6, 10], h - 1 in [1e-4, 100], |B| / sqrt(D) in
+    [1e-4, 10], all log-uniform, directions uniform; every 50th state gets
+    tau scaled into [-0.5, 1.5) x or |S| x 3 (inadmissible)."""
+    rng = np.random.default_rng(seed)
+
+    def logu(lo, hi, size=n):
+        return np.exp(rng.uniform(np.log(lo), np.log(hi), size))
+
+    D, W, h = logu(1e-3, 1e3), 1.0 + logu(1e-6, 10.0), 1.0 + logu(1e-4, 100.0)
+    v = np.sqrt(1.0 - 1.0 / W ** 2)
+    ct, ph = rng.uniform(-1, 1, n), rng.uniform(0, 2 * np.pi, n)
+    st = np.sqrt(1 - ct ** 2)
+    vv = np.stack([v * st * np.cos(ph), v * st * np.sin(ph), v * ct])
+    B = rng.uniform(-1, 1, (3, n)) * logu(1e-4, 10.0) * np.sqrt(D)
+    rho = D / W
+    p = (2.0 / 3.0) * rho * (h - 1.0) / (5.0 / 3.0)
+    rhoh = rho * h
+    B2, vB = (B ** 2).sum(0), (vv * B).sum(0)
+    S = (rhoh * W ** 2 + B2) * vv - vB * B
+    tau = rhoh * W ** 2 - p - D + 0.5 * (B2 + B2 * v ** 2 - vB ** 2)
+    tau[::50] *= rng.uniform(-0.5, 1.5, tau[::50].shape)
+    S[0, 25::50] *= 3.0
+    vals = {"D": D, "tau": tau, "Sx": S[0], "Sy": S[1], "Sz": S[2], "Bx": B[0], "By": B[1],
+            "Bz": B[2]}
+    return {k: torch.from_numpy(np.ascontiguousarray(vals[k])) for k in CONS8}
+
+
+@pytest.mark.parametrize("n", [16, 4096, 60000])
+def test_c2p_random_states_bitwise_vs_oracle(n):
+    """The con2prim on random states (the moderate ones take the shortened
+    but exact master-function arithmetic, the rest the plain one; waves mix
+    both), one batch of n, bitwise vs the oracle: primitives and status."""
+    dev = _dev()
+    import oracle.c2p as OC
+    import oracle.eos as OE
+    from srmhd import c2p as C, eos as E
+    _flush(f"c2p random batch {n}")
+    cs = _random_cons(n, 20261204 + n)
+    P, S = C.conservative_to_primitive({k: v.to(dev) for k, v in cs.items()},
+                                       E.hybrid_eos(0.0, 5.0 / 3.0, 5.0 / 3.0), return_status=True)
+    Po, So = OC.cons_to_prim({k: v.clone() for k, v in cs.items()}, OE.ideal(5.0 / 3.0),
+                             return_status=True)
+    for k in PRIM9:
+        _eq(P[k], Po[k], f"oracle c2p {k}")
+    for k in ("bracket_ok", "converged", "v_clamped", "atmo_floored"):
+        _eq(S[k], So[k], f"oracle status {k}")
+
+
 CASES_2D = [
     ("rotor_sharp_mc_hlld", "rotor", 24, "mc", "hlld", "rk3", 5 / 3, (-0.5, 0.5), "outflow"),
     ("rotor_tapered_weno5z_hlld", "rotor_tapered", 20, "weno5z", "hlld", "rk3", 5 / 3,
