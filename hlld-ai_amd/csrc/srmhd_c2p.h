@@ -140,8 +140,10 @@ struct KastaunT {
     const double Wh = div_nr(1.0, csqrt_mid(clo(1.0 - vh2, 1e-10)));
     const double rho = tmin(tmax(div_nr(D, Wh), 1e-12), 1e15);
     const double eps0 = Wh * (qb - mu * rb2) + vh2 * (Wh * Wh) / (1.0 + Wh);
-    const double eps = tmin(tmax(eps0, eos_eps_lo(e, rho)), 1e5);
-    const double p = eos_press(e, eps, rho);
+    // the EOS at K = 0 with rho in [1e-12, 1e15]: cold pressure and energy
+    // +0, so eps_lo = +0 and p = 0 + (eps - 0) rho (g_th - 1) (eos_press)
+    const double eps = tmin(tmax(eps0, 0.0), 1e5);
+    const double p = e.is_cold ? 0.0 : 0.0 + (tmax(eps, 0.0) - 0.0) * rho * (e.gamma_th - 1.0);
     const double a = p / (rho * (1.0 + eps) + TINY);
     const double nuA = div_nr((1.0 + a) * (1.0 + eps), Wh);
     const double nuB = (1.0 + a) * (1.0 + qb - mu * rb2);
@@ -153,6 +155,12 @@ struct KastaunT {
 
   template <bool FAST>
   SRMHD_DEV double master_at(double mu) const { return FAST ? master_fast(mu) : master(mu); }
+
+  // fa() for a tame element, bit for bit: 1 + mu b~^2 in [1, 1 + 1e30] and
+  // 1 + rbar^2 in [1, 1 + 3e30]
+  SRMHD_DEV double fa_fast(double mu) const {
+    return mu * csqrt_mid(1.0 + rbar2(mu, div_nr(1.0, 1.0 + mu * bt2))) - 1.0;
+  }
 
   // 50-step bisection of fa on [0, 1] (c2p.py:162-184).  The reference
   // re-evaluates fa(lo) every iteration; fa is deterministic, so carrying
@@ -189,6 +197,7 @@ struct KastaunT {
   // from bracketed Newton steps on [m0, 1] (forward-mode derivative) to
   // |fa| <= 1e-14; anything non-finite, no convergence within 40 steps or
   // r^2 > 1e8 takes the full bisection.
+  template <bool FAST = false>
   SRMHD_DEV double mu_plus_fast() const {
     if (!(r2 <= 1e8)) return mu_plus();
     double a = 1.0 / csqrt(1.0 + r2), b = 1.0 / (1.0 + TINY), x = a;
@@ -215,7 +224,7 @@ struct KastaunT {
 #pragma unroll 1
     for (int it = 0; it < 50; ++it) {
       const double mid = 0.5 * (lo + hi);
-      const bool up = fabs(mid - x) > delta ? mid > x : fa(mid) >= 0.0;
+      const bool up = fabs(mid - x) > delta ? mid > x : (FAST ? fa_fast(mid) : fa(mid)) >= 0.0;
       if (up)
         hi = mid;
       else
@@ -316,7 +325,7 @@ struct C2PRec {
 // Pass A.  Returns the element's not-converged mask (bit k: after k steps).
 template <bool FAST>
 SRMHD_DEV uint64_t c2p_pass_a_impl(const Kastaun& k, C2PRec& rec) {
-  const double mup = k.mu_plus_fast();   // bit-identical to k.mu_plus()
+  const double mup = k.mu_plus_fast<FAST>();   // bit-identical to k.mu_plus()
   double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
   double f0 = k.master_at<FAST>(x0), f1 = k.master_at<FAST>(x1);
   const bool good = (f0 * f1) < 0.0;
@@ -376,7 +385,7 @@ SRMHD_DEV void rec_put(double* c, double x0, double f0, double x1, double f1) {
 
 template <bool FAST>
 SRMHD_DEV uint64_t c2p_pass_a_impl(const Kastaun& k, C2PRecK& rec) {
-  const double mup = k.mu_plus_fast();
+  const double mup = k.mu_plus_fast<FAST>();
   double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
   double f0 = k.master_at<FAST>(x0), f1 = k.master_at<FAST>(x1);
   const bool good = (f0 * f1) < 0.0;
@@ -538,7 +547,7 @@ SRMHD_DEV int c2p_pass_b(const Kastaun& k, const Rec& rec, int K, double atmo_rh
 template <bool FAST>
 SRMHD_DEV int c2p_single_impl(const Kastaun& k, double atmo_rho, double atmo_eps, Prim& P,
                               double* mu_out, double mup_in) {
-  const double mup = mup_in >= 0.0 ? mup_in : k.mu_plus_fast();
+  const double mup = mup_in >= 0.0 ? mup_in : k.mu_plus_fast<FAST>();
   double x0 = 0.0, x1 = mup * (1.0 - 1e-10);
   double f0 = k.master_at<FAST>(x0), f1 = k.master_at<FAST>(x1);
   const bool good = (f0 * f1) < 0.0;
