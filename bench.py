@@ -41,8 +41,8 @@ FP64_PEAK_TFLOPS = 78.6
 # SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 on the 2048^2 rotor
 # (profiles/rotor2048_counters.md): flops = 64 lanes x (add + mul + 2 fma +
 # trans) per wave, divided by the elements of the launch.
-FP64_FLOPS_PER_ELEMENT = {("fp64", "sweep_x_a"): 16980.0, ("fp64", "sweep_y_a"): 16980.0,
-                          ("fp64", "c2p_a"): 13584.0}
+FP64_FLOPS_PER_ELEMENT = {("fp64", "sweep_x_a"): 13663.0, ("fp64", "sweep_y_a"): 4117.0,
+                          ("fp64", "c2p_a"): 10267.0}
 
 # the kernel class each form spends most time in (rocprofv3 --stats of the
 # 2048^2 rotor: profiles/rotor2048_weno5z_hlld_kernel_stats.md)
@@ -50,11 +50,14 @@ DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "swee
                    "fp64_local": "sweep_x_b"}
 
 # wave-level VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU on the
-# 2048^2 rotor, profiles/rotor2048_counters.md) and the issue peak they are
-# held against: 256 CUs x 4 SIMDs, one wave64 VALU instruction per 4 cycles
-# (16 lanes), 2.4 GHz
-VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 1.836e8, ("fp64", "sweep_x_a", 2048): 1.394e9}
-VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+# 2048^2 rotor, profiles/rotor2048_counters.md)
+VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 1.836e8, ("fp64", "sweep_x_a", 2048): 1.023e9}
+
+# measured VALU busy fraction of the launch (same profile): SQ_ACTIVE_INST_VALU
+# (quad-cycles in which a SIMD issued a VALU instruction, summed over the
+# chip) / (256 CUs x GRBM_GUI_ACTIVE cycles of one XCD) -- 1 means every SIMD
+# issued a vector instruction every quad-cycle of the launch
+VALU_BUSY = {("fp64", "sweep_x_a", 2048): 0.973}
 
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
@@ -307,10 +310,11 @@ def _rotor_form(a, precision, kernel, U_init, P_init):
                 "frac": tf / FP64_PEAK_TFLOPS}
     insts = VALU_INSTS_PER_LAUNCH.get((precision, kernel, n))
     if insts is not None:
-        rate = insts / (avg_ms * 1e-3)
-        roof["valu_issue"] = {"insts_per_launch": insts, "achieved": rate,
-                              "peak": VALU_ISSUE_PEAK, "unit": "wave-instructions/s",
-                              "frac": rate / VALU_ISSUE_PEAK}
+        roof["valu"] = {"insts_per_launch": insts,
+                        "issue_rate": insts / (avg_ms * 1e-3), "unit": "wave-instructions/s",
+                        "busy_measured": VALU_BUSY.get((precision, kernel, n)),
+                        "busy_source": "rocprofv3 --pmc SQ_ACTIVE_INST_VALU GRBM_GUI_ACTIVE "
+                                       "(profiles/rotor2048_counters.md)"}
     return {"step": step, "zc": zc, "wall": wall, "roofline": roof, "fp64_valu": valu,
             "ms_per_step": 1e3 * wall / a.steps}
 
