@@ -17,17 +17,22 @@
 //                                  (limiter branches as the forward took them);
 //                                  HLLD's p* by implicit differentiation of the
 //                                  residual at the taped root (hlld.py:586-602),
-//                                  the HLLE flux where the forward fell back.
-// The local Jacobians come from forward-mode dual numbers evaluated on the
-// same templated device code as the forward (srmhd_common.h .. srmhd_riemann.h),
-// compiled with -ffp-contract=off like the forward, so every value, and
-// therefore every branch, is the forward's own.
+//                                  the HLLE flux where the forward fell back;
+//                                  HLLD's part after the side states in reverse
+//                                  mode (srmhd_flux_adj.h).
+// The other local Jacobians come from forward-mode dual numbers evaluated on
+// the same templated device code as the forward (srmhd_common.h ..
+// srmhd_riemann.h), compiled with -ffp-contract=off like the forward, so every
+// value, and therefore every branch, is the forward's own; the reverse-mode
+// code recomputes the forward's values with the same functions and takes the
+// same branches.
 //
 // Every adjoint accumulation is a gather in a fixed order (no atomics), so the
 // gradient is deterministic.
 #include "srmhd_grid2d.h"
 #include "srmhd_recon.h"
 #include "srmhd_riemann.h"
+#include "srmhd_flux_adj.h"
 
 using namespace srmhd;
 
@@ -363,35 +368,6 @@ __global__ __launch_bounds__(BLK) void k_adj_ec(G2 g, St2 S, Pr2 P, const double
   lBcy[t] = -(lEc * vx);
 }
 
-// Face primitives from the reconstructed variables (rho, p, z, B_t1, B_t2)
-// and the face normal field, as face_prims2d forms them; the tangent is
-// seeded on input `dir` (0..6 left, 7..13 right, 14 B_n; < 0: none).
-template <int N, class T>
-__device__ void prims_from_q(const double ql[7], const double qr[7], double Bn,
-                             const srmhd_eos& e, int dir, PrimT<T>& L, PrimT<T>& R) {
-  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
-  const T bn = T::seed(Bn, dir == 14 ? 0 : -1);
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const double* q = s ? qr : ql;
-    T x[7];
-#pragma unroll
-    for (int m = 0; m < 7; ++m) x[m] = T::seed(q[m], dir == s * 7 + m ? 0 : -1);
-    PrimT<T>& Q = s ? R : L;
-    const T z2 = sq(x[2]) + sq(x[3]) + sq(x[4]);
-    const T fac = 1.0 / csqrt(1.0 + z2);
-    Q.v[0] = x[2] * fac;
-    Q.v[1] = x[3] * fac;
-    Q.v[2] = x[4] * fac;
-    Q.rho = clo(x[0], 1e-12);
-    Q.p = clo(x[1], 1e-14);
-    Q.eps = eos_eps(e, Q.p, Q.rho);
-    Q.B[N] = bn;
-    Q.B[T1] = x[5];
-    Q.B[T2] = x[6];
-  }
-}
-
 // Per face of the sweep along N: the adjoint of reconstruct_prims + flux
 // (sweep.py:30-119).  lamF: the seven flux adjoints of the face; writes the
 // adjoints of the stencil's reconstructed variables (W[(v*KW + s)*nf + t]:
@@ -464,50 +440,14 @@ __global__ __launch_bounds__(BLK) void k_adj_sweep(G2 g, St2 S, Pr2 P, srmhd_eos
   lf[CB + 2] = lamF.p[5][t];
   lf[N == 0 ? CB + 1 : CB + 0] = lamF.p[6][t];
 
-  // level 1: the flux with respect to the 15 face inputs
+  // level 1: the flux with respect to the 15 face inputs (HLLD in reverse
+  // mode; HLLE, HLLC by one dual pass per input)
   const double ps = SOL == SOLVER_HLLD ? pstar[t] : -1.0;
-  double Rp = 1.0;
   double lq[15];
-#pragma unroll 1
-  for (int d = (SOL == SOLVER_HLLD && ps >= 0.0) ? -1 : 0; d < 15; ++d) {
-    PrimT<D1> L, R;
-    prims_from_q<N>(ql, qr, Bn, e, d, L, R);
-    D1 F[8];
-    if (SOL == SOLVER_HLLE) {
-      D1 U[8], pso;
-      hlle<N>(L, R, e, F, U, pso);
-    } else if (SOL == SOLVER_HLLC) {
-      D1 U[8], pso;
-      hllc<N>(L, R, e, F, U, pso);
-    } else {
-      HlldPrep<N, D1> h;
-      h.run(L, R, e);
-      D1 fH[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) fH[k] = hll_flux(h.fL[k], h.fR[k], h.uL[k], h.uR[k], h.cmin, h.cmax);
-      if (ps < 0.0) {   // the forward fell back to the HLLE flux
-#pragma unroll
-        for (int k = 0; k < 8; ++k) F[k] = fH[k];
-      } else {
-        StarRT<D1> Rv;
-        hlld_rvec<N>(h, Rv);
-        if (d < 0) {   // d(residual)/dp* at the root
-          Rp = star_residual(Rv, D1::seed(ps, 0)).d[0];
-          continue;
-        }
-        // p* by implicit differentiation of the residual at the taped root
-        const D1 r = star_residual(Rv, D1(ps));
-        D1 psd(ps);
-        psd.d[0] = -r.d[0] / Rp;
-        D1 pso;
-        hlld_assemble<N>(h, Rv, fH, psd, true, 0.0, F, (D1*)nullptr, pso);
-      }
-    }
-    double acc = 0.0;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) acc += lf[k] * F[k].d[0];
-    lq[d] = acc;
-  }
+  if (SOL == SOLVER_HLLD)
+    flux_vjp_hlld<N>(ql, qr, Bn, e, ps, lf, lq);
+  else
+    flux_vjp_dual<N, SOL>(ql, qr, Bn, e, ps, lf, lq);
 
   // level 2: the reconstruction with respect to the window (all slots at once)
 #pragma unroll 1

@@ -31,7 +31,7 @@ SOURCES = ([("srmhd_kernels.hip", [], "srmhd_kernels.o"), ("srmhd_tubes.hip", []
 EXTRA = {"srmhd_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
 HEADERS = ["srmhd_common.h", "srmhd_c2p.h", "srmhd_recon.h", "srmhd_riemann.h", "srmhd_launch.h",
            "srmhd_grid2d.h", "srmhd_mixed.h", "srmhd_f32.h", "srmhd_dual.h", "srmhd_pstar.h",
-           "srmhd_pstar_launch.h", "srmhd_tubes.h", "srmhd_linear.h"]
+           "srmhd_pstar_launch.h", "srmhd_tubes.h", "srmhd_linear.h", "srmhd_flux_adj.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
