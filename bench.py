@@ -469,8 +469,10 @@ def bench_adjoint(a):
     fwd_wall = (time.perf_counter() - t1) / 2
     avg_ms = ms.value / max(cnt.value, 1)
     nfx = (n + 5) * (n + 4)
-    # the x-sweep adjoint per face: 8 state/primitive arrays of the stencil and
+    # the x-sweep adjoint per face (its two launches: the flux adjoint and
+    # the reconstruction adjoint): 8 state/primitive arrays of the stencil and
     # p* in, 7 flux adjoints in, 7 x 4 window adjoints + the B_n adjoint out
+    # (the 14 face-state adjoints passed between the launches are not counted)
     kb = nfx * (8 * 8 + 8 + 7 * 8 + 29 * 8)
     achieved = kb / (avg_ms * 1e-3) / 1e9
     out = {
@@ -492,7 +494,7 @@ def bench_adjoint(a):
         "peak_device_memory_GB": peak / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_adj_sweep<0, 1, 2> (x-sweep adjoint)", "avg_launch_ms": avg_ms,
+                     "kernel": "k_adj_flux<0, 1, 2> + k_adj_recon<0, 1> (x-sweep adjoint)", "avg_launch_ms": avg_ms,
                      "launches": cnt.value, "algorithmic_bytes_per_launch": kb},
     }
     if not a.no_cpu:

@@ -40,6 +40,12 @@ namespace {
 
 using D1 = Dual<1>;
 
+#ifndef SRMHD_C2P_ADJ_W
+#define SRMHD_C2P_ADJ_W 8
+#endif
+constexpr int C2P_ADJ_W = SRMHD_C2P_ADJ_W;   // 1, 2, 4 or 8
+static_assert(8 % C2P_ADJ_W == 0, "the 8 conserved inputs in whole passes");
+
 struct CFlx {  // read-only flux adjoints: D, Sx, Sy, Sz, tau, Bz, E-component
   const double* p[7];
 };
@@ -105,20 +111,25 @@ __global__ __launch_bounds__(BLK) void k_adj_c2p(G2 g, St2 U, srmhd_eos e, doubl
   double lu[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (!(q.rho < atmo_rho)) {   // the atmosphere floor is constant in U
     const double fmu = k.master(D1::seed(mu, 0)).d[0];
+    constexpr int CW = C2P_ADJ_W;   // tangent directions of U per pass
+    using DW = Dual<CW>;
 #pragma unroll 1
-    for (int d = 0; d < 8; ++d) {
-      D1 ud[8];
+    for (int base = 0; base < 8; base += CW) {
+      DW ud[8];
 #pragma unroll
-      for (int m = 0; m < 8; ++m) ud[m] = D1::seed(u[m], m == d ? 0 : -1);
-      KastaunT<D1> kd;
+      for (int m = 0; m < 8; ++m) ud[m] = DW::seed(u[m], m - base);
+      KastaunT<DW> kd;
       kd.init(ud, e);
-      const D1 fth = kd.master(mu);
-      D1 mud(mu);
-      mud.d[0] = -fth.d[0] / fmu;
-      PrimT<D1> Pd;
+      const DW fth = kd.master(mu);
+      DW mud(mu);
+#pragma unroll
+      for (int c = 0; c < CW; ++c) mud.d[c] = -fth.d[c] / fmu;
+      PrimT<DW> Pd;
       kd.finish(mud, Pd);
-      lu[d] = lp[0] * Pd.rho.d[0] + lp[1] * Pd.v[0].d[0] + lp[2] * Pd.v[1].d[0] +
-              lp[3] * Pd.v[2].d[0] + lp[4] * Pd.p.d[0] + lp[5] * Pd.eps.d[0];
+#pragma unroll
+      for (int c = 0; c < CW; ++c)
+        lu[base + c] = lp[0] * Pd.rho.d[c] + lp[1] * Pd.v[0].d[c] + lp[2] * Pd.v[1].d[c] +
+                       lp[3] * Pd.v[2].d[c] + lp[4] * Pd.p.d[c] + lp[5] * Pd.eps.d[c];
     }
   }
   lamU.p[0][c] += lu[CD];
@@ -368,39 +379,28 @@ __global__ __launch_bounds__(BLK) void k_adj_ec(G2 g, St2 S, Pr2 P, const double
   lBcy[t] = -(lEc * vx);
 }
 
-// Per face of the sweep along N: the adjoint of reconstruct_prims + flux
-// (sweep.py:30-119).  lamF: the seven flux adjoints of the face; writes the
-// adjoints of the stencil's reconstructed variables (W[(v*KW + s)*nf + t]:
-// variable v of rho, p, zx, zy, zz, B_t1, B_t2 at window slot k0 + s) and of
-// the normal field (lBn).
-template <int N, int LIM, int SOL>
-__global__ __launch_bounds__(BLK) void k_adj_sweep(G2 g, St2 S, Pr2 P, srmhd_eos e,
-                                                   const double* pstar, CFlx lamF, double* W,
-                                                   double* lBn) {
+// The stencil window of face t of the sweep along N, with face_prims2d's
+// arithmetic: a[v][k] for the reconstructed variables (rho, p, z_x, z_y,
+// z_z, B_t1, B_t2) at slots k0..k1; the face index f along the sweep, the
+// line length n and whether the face took the PCM fallback.
+template <int N, int LIM>
+__device__ inline void adj_window(const G2& g, const St2& S, const Pr2& P, int64_t t,
+                                  double a[7][8], int& f, int& n, bool& bad, double ql[7],
+                                  double qr[7], double& Bn) {
   constexpr int hw = stencil_halfwidth(LIM);
-  constexpr int k0 = 3 - hw, k1 = 4 + hw, KW = k1 - k0 + 1;
-  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
-  const int64_t nf = n_faces<N>(g);
-  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
-  if (t >= nf) return;
+  constexpr int k0 = 3 - hw, k1 = 4 + hw;
   int i, j;
   face_ij<N>(g, t, i, j);
-  const int n = N == 0 ? g.nxt : g.nyt;
-  const int f = N == 0 ? i : j;
-  auto cidx = [&](int k) -> int64_t {
-    const int c = clampi(f + k - 4, 0, n - 1);
-    return N == 0 ? (int64_t)c * g.nyt + j : (int64_t)i * g.nyt + c;
-  };
-  // the stencil window, with face_prims2d's arithmetic
-  double a[7][8];
+  n = N == 0 ? g.nxt : g.nyt;
+  f = N == 0 ? i : j;
 #pragma unroll
   for (int v = 0; v < 7; ++v)
 #pragma unroll
     for (int k = 0; k < 8; ++k) a[v][k] = 0.0;
 #pragma unroll
   for (int k = k0; k <= k1; ++k) {
-    const int64_t c = cidx(k);
     const int cc = clampi(f + k - 4, 0, n - 1);
+    const int64_t c = N == 0 ? (int64_t)cc * g.nyt + j : (int64_t)i * g.nyt + cc;
     const int ci = N == 0 ? cc : i, cj = N == 0 ? j : cc;
     a[0][k] = P.p[0][c];
     a[1][k] = P.p[4][c];
@@ -413,10 +413,9 @@ __global__ __launch_bounds__(BLK) void k_adj_sweep(G2 g, St2 S, Pr2 P, srmhd_eos
     a[5][k] = N == 0 ? bcy(g, S.p[7], ci, cj) : S.p[5][c];
     a[6][k] = N == 0 ? S.p[5][c] : bcx(g, S.p[6], ci, cj);
   }
-  double ql[7], qr[7];
 #pragma unroll
   for (int v = 0; v < 2; ++v) face_states<LIM>(a[v], f, n, ql[v], qr[v]);
-  const bool bad = (ql[0] <= 1e-12) || (qr[0] <= 1e-12) || (ql[1] <= 1e-14) || (qr[1] <= 1e-14);
+  bad = (ql[0] <= 1e-12) || (qr[0] <= 1e-12) || (ql[1] <= 1e-14) || (qr[1] <= 1e-14);
 #pragma unroll
   for (int v = 0; v < 7; ++v) {
     if (bad) {
@@ -426,9 +425,28 @@ __global__ __launch_bounds__(BLK) void k_adj_sweep(G2 g, St2 S, Pr2 P, srmhd_eos
       face_states<LIM>(a[v], f, n, ql[v], qr[v]);
     }
   }
-  const double Bn =
-      N == 0 ? S.p[6][(int64_t)i * g.nyt + j] : S.p[7][(int64_t)i * (g.nyt + 1) + j];
+  Bn = N == 0 ? S.p[6][(int64_t)i * g.nyt + j] : S.p[7][(int64_t)i * (g.nyt + 1) + j];
+}
 
+// Per face of the sweep along N, the adjoint of the flux (sweep.py:30-119)
+// with respect to its 15 face inputs: LQ[m * nf + t] for the 14 face-state
+// inputs (left 0..6, right 7..13) and lBn[t] for the normal field.  lamF: the
+// seven flux adjoints of the face.  HLLD in reverse mode; HLLE, HLLC by one
+// dual pass per input.
+template <int N, int LIM, int SOL>
+__global__ __launch_bounds__(BLK) void k_adj_flux(G2 g, St2 S, Pr2 P, srmhd_eos e,
+                                                  const double* pstar, CFlx lamF, double* LQ,
+                                                  double* lBn) {
+  const int64_t nf = n_faces<N>(g);
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (t >= nf) return;
+  double ql[7], qr[7], Bn;
+  {
+    double a[7][8];
+    int f, n;
+    bool bad;
+    adj_window<N, LIM>(g, S, P, t, a, f, n, bad, ql, qr, Bn);
+  }
   // flux adjoints in flux-vector order (the normal-B flux is zeroed: none)
   double lf[8];
   lf[CD] = lamF.p[0][t];
@@ -439,18 +457,37 @@ __global__ __launch_bounds__(BLK) void k_adj_sweep(G2 g, St2 S, Pr2 P, srmhd_eos
   lf[CB + N] = 0.0;
   lf[CB + 2] = lamF.p[5][t];
   lf[N == 0 ? CB + 1 : CB + 0] = lamF.p[6][t];
-
-  // level 1: the flux with respect to the 15 face inputs (HLLD in reverse
-  // mode; HLLE, HLLC by one dual pass per input)
   const double ps = SOL == SOLVER_HLLD ? pstar[t] : -1.0;
   double lq[15];
   if (SOL == SOLVER_HLLD)
     flux_vjp_hlld<N>(ql, qr, Bn, e, ps, lf, lq);
   else
     flux_vjp_dual<N, SOL>(ql, qr, Bn, e, ps, lf, lq);
+#pragma unroll
+  for (int m = 0; m < 14; ++m) LQ[(int64_t)m * nf + t] = lq[m];
+  lBn[t] = lq[14];
+}
 
-  // level 2: the reconstruction with respect to the window (all slots at once)
-#pragma unroll 1
+// Per face of the sweep along N: the adjoint of reconstruct_prims
+// (reconstruction.py:272-338) from the face-state adjoints LQ -- the face
+// states with respect to the stencil window, every limiter branch as the
+// forward took it (all slots at once); writes W[(v*KW + s)*nf + t], the
+// adjoint of variable v of (rho, p, z_x, z_y, z_z, B_t1, B_t2) at window slot
+// k0 + s.
+template <int N, int LIM>
+__global__ __launch_bounds__(BLK) void k_adj_recon(G2 g, St2 S, Pr2 P, const double* LQ,
+                                                   double* W) {
+  constexpr int hw = stencil_halfwidth(LIM);
+  constexpr int k0 = 3 - hw, k1 = 4 + hw, KW = k1 - k0 + 1;
+  const int64_t nf = n_faces<N>(g);
+  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  if (t >= nf) return;
+  double a[7][8], ql[7], qr[7], Bn;
+  int f, n;
+  bool bad;
+  adj_window<N, LIM>(g, S, P, t, a, f, n, bad, ql, qr, Bn);
+  // unrolled: a[v][.] with a run-time v would put the window in scratch
+#pragma unroll
   for (int v = 0; v < 7; ++v) {
     Dual<KW> aa[8];
 #pragma unroll
@@ -462,13 +499,10 @@ __global__ __launch_bounds__(BLK) void k_adj_sweep(G2 g, St2 S, Pr2 P, srmhd_eos
     } else {
       face_states<LIM>(aa, f, n, dl, dr);
     }
-    const double wl = lq[v], wr = lq[7 + v];
+    const double wl = LQ[(int64_t)v * nf + t], wr = LQ[(int64_t)(7 + v) * nf + t];
 #pragma unroll
     for (int s = 0; s < KW; ++s) W[(int64_t)(v * KW + s) * nf + t] = wl * dl.d[s] + wr * dr.d[s];
   }
-  lBn[t] = lq[14];
-  (void)T1;
-  (void)T2;
 }
 
 // Per cell: gather the window adjoints of both sweeps, turn the z = W v
@@ -600,7 +634,7 @@ int window_width(int lim) {
 }
 
 struct WsA {
-  double *lBcx, *lBcy, *lEz, *lBnx, *lBny, *Wx, *Wy;
+  double *lBcx, *lBcy, *lEz, *lBnx, *lBny, *Wx, *Wy, *LQ;
   double* LE[8];
   double* lFx[7];
   double* lFy[7];
@@ -625,19 +659,22 @@ WsA carve_adj(const G2& g, int lim, void* work) {
   for (int k = 0; k < 7; ++k) w.lFy[k] = c.take<double>(nfy);
   w.Wx = c.take<double>(7 * KW * nfx);
   w.Wy = c.take<double>(7 * KW * nfy);
+  w.LQ = c.take<double>(14 * (nfx > nfy ? nfx : nfy));   // the face-state adjoints of one sweep
   w.bytes = c.off;
   return w;
 }
 
 template <int N, int LIM>
 void launch_sweep(int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
-                  const double* ps, const CFlx& lF, double* W, double* lBn, hipStream_t s) {
+                  const double* ps, const CFlx& lF, double* LQ, double* W, double* lBn,
+                  hipStream_t s) {
   const int nb = blocks_for(N == 0 ? (int64_t)(g.nxt + 1) * g.nyt : (int64_t)g.nxt * (g.nyt + 1));
   switch (sol) {
-    case SOLVER_HLLE: k_adj_sweep<N, LIM, SOLVER_HLLE><<<nb, BLK, 0, s>>>(g, S, P, e, ps, lF, W, lBn); break;
-    case SOLVER_HLLC: k_adj_sweep<N, LIM, SOLVER_HLLC><<<nb, BLK, 0, s>>>(g, S, P, e, ps, lF, W, lBn); break;
-    default: k_adj_sweep<N, LIM, SOLVER_HLLD><<<nb, BLK, 0, s>>>(g, S, P, e, ps, lF, W, lBn);
+    case SOLVER_HLLE: k_adj_flux<N, LIM, SOLVER_HLLE><<<nb, BLK, 0, s>>>(g, S, P, e, ps, lF, LQ, lBn); break;
+    case SOLVER_HLLC: k_adj_flux<N, LIM, SOLVER_HLLC><<<nb, BLK, 0, s>>>(g, S, P, e, ps, lF, LQ, lBn); break;
+    default: k_adj_flux<N, LIM, SOLVER_HLLD><<<nb, BLK, 0, s>>>(g, S, P, e, ps, lF, LQ, lBn);
   }
+  k_adj_recon<N, LIM><<<nb, BLK, 0, s>>>(g, S, P, LQ, W);
 }
 
 template <int LIM>
@@ -650,9 +687,9 @@ void sweeps_adj(int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eo
     ly.p[k] = w.lFy[k];
   }
   timing_begin(SRMHD_K_ADJ_SWEEP_X, s);
-  launch_sweep<0, LIM>(sol, g, S, P, e, tape->pstar_x, lx, w.Wx, w.lBnx, s);
+  launch_sweep<0, LIM>(sol, g, S, P, e, tape->pstar_x, lx, w.LQ, w.Wx, w.lBnx, s);
   timing_end(SRMHD_K_ADJ_SWEEP_X, s);
-  launch_sweep<1, LIM>(sol, g, S, P, e, tape->pstar_y, ly, w.Wy, w.lBny, s);
+  launch_sweep<1, LIM>(sol, g, S, P, e, tape->pstar_y, ly, w.LQ, w.Wy, w.lBny, s);
   k_adj_gather<LIM><<<blocks_for((int64_t)g.nxt * g.nyt), BLK, 0, s>>>(g, P, w.Wx, w.Wy, lamP,
                                                                        lamBz, w.lBcx, w.lBcy);
 }

@@ -329,18 +329,24 @@ SRMHD_DEV T hll_state(const T& fl, const T& fr, const T& ul, const T& ur, const 
 }
 
 // Both sides plus the clamped HLL speeds   hlld.py:267-276
+// (cps, cms: each side's speeds c+, c-, left first)
 template <int N, class T>
 SRMHD_DEV void srmhd_fluxes(const PrimT<T>& L, const PrimT<T>& R, const srmhd_eos& e, T uL[8],
-                            T uR[8], T fL[8], T fR[8], T& cmax, T& cmin) {
-  T cpl, cml, cpr, cmr;
-  side_state<N>(L, e, uL, fL, cpl, cml);
-  side_state<N>(R, e, uR, fR, cpr, cmr);
-  cmax = clo(tmax(cpl, cpr), 0.0);
-  cmin = clo(tmax(-cml, -cmr), 0.0);
+                            T uR[8], T fL[8], T fR[8], T& cmax, T& cmin, T cps[2], T cms[2]) {
+  side_state<N>(L, e, uL, fL, cps[0], cms[0]);
+  side_state<N>(R, e, uR, fR, cps[1], cms[1]);
+  cmax = clo(tmax(cps[0], cps[1]), 0.0);
+  cmin = clo(tmax(-cms[0], -cms[1]), 0.0);
   if (cmax < 1e-12 && cmin < 1e-12) {
     cmax = T(1.0);
     cmin = T(1.0);
   }
+}
+template <int N, class T>
+SRMHD_DEV void srmhd_fluxes(const PrimT<T>& L, const PrimT<T>& R, const srmhd_eos& e, T uL[8],
+                            T uR[8], T fL[8], T fR[8], T& cmax, T& cmin) {
+  T cps[2], cms[2];
+  srmhd_fluxes<N>(L, R, e, uL, uR, fL, fR, cmax, cmin, cps, cms);
 }
 
 // primitive_to_conserved   hlld.py:185-217

@@ -7,41 +7,39 @@
 //  * flux_vjp_dual: one forward-mode dual pass per input through the whole
 //    solver (every solver; HLLD's p* by implicit differentiation of the
 //    residual at the taped root).
-//  * flux_vjp_hlld: HLLD in reverse mode.  The side states and speeds of each
-//    side (a small map of 8 inputs) are differentiated forward, one dual pass
-//    per input; everything after them -- the HLL flux, the R-vectors, the
-//    Alfven and contact states at p*, the region choice -- is run backwards
-//    once, with p*(inputs) defined by residual(p*) = 0:
+//  * flux_vjp_hlld: HLLD in reverse mode, run backwards once -- the HLL
+//    flux, the R-vectors, the Alfven and contact states at p*, the region
+//    choice, then each side's state and speeds (side_rev) -- with p*(inputs)
+//    defined by residual(p*) = 0:
 //        lambda^T dF/dz = lambda^T A_z + (o_bar + c Res_o)^T o_z + c Res_z,
 //        o_bar = A_o^T lambda,   c = -(o_bar . o_p + lambda^T A_p) / Res_p',
 //    where A is the flux assembly, o the star states (star_solve), Res the
 //    residual and Res_p' its total derivative in p (one dual pass in p).
 //    The branches are the forward's: the region at xi = 0, the sdiv guards,
 //    the clamps, the HLL speed maxima (a tie splits the adjoint evenly, as
-//    torch autograd does).  About a fifth of the dual form's arithmetic.
+//    torch autograd does).
 #pragma once
 
 #include "srmhd_riemann.h"
 
 namespace srmhd {
 
-template <class T>
-SRMHD_DEV T seeded(double x, bool on);
-template <>
-SRMHD_DEV double seeded<double>(double x, bool) { return x; }
-template <>
-SRMHD_DEV Dual<1> seeded<Dual<1>>(double x, bool on) { return Dual<1>::seed(x, on ? 0 : -1); }
+// x as a T; a dual gets a unit tangent in component `slot` (none when slot
+// is out of range)
+SRMHD_DEV double seeded(double x, int, double*) { return x; }
+template <int K>
+SRMHD_DEV Dual<K> seeded(double x, int slot, Dual<K>*) { return Dual<K>::seed(x, slot); }
 
 // The face primitives of one side from its reconstructed variables (rho, p,
-// z, B_t1, B_t2) and the face normal field, as face_prims2d forms them; the
-// tangent is seeded on q[dir] (dir 0..6) or on B_n (dir 7).
+// z, B_t1, B_t2) and the face normal field, as face_prims2d forms them.  With
+// duals, input m of (q[0..6], B_n) gets the tangent component m - base.
 template <int N, class T>
-SRMHD_DEV void side_prims_q(const double q[7], double Bn, const srmhd_eos& e, int dir,
+SRMHD_DEV void side_prims_q(const double q[7], double Bn, const srmhd_eos& e, int base,
                             PrimT<T>& Q) {
   constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
   T x[7];
 #pragma unroll
-  for (int m = 0; m < 7; ++m) x[m] = seeded<T>(q[m], dir == m);
+  for (int m = 0; m < 7; ++m) x[m] = seeded(q[m], m - base, (T*)nullptr);
   const T z2 = sq(x[2]) + sq(x[3]) + sq(x[4]);
   const T fac = 1.0 / csqrt(1.0 + z2);
   Q.v[0] = x[2] * fac;
@@ -50,7 +48,7 @@ SRMHD_DEV void side_prims_q(const double q[7], double Bn, const srmhd_eos& e, in
   Q.rho = clo(x[0], 1e-12);
   Q.p = clo(x[1], 1e-14);
   Q.eps = eos_eps(e, Q.p, Q.rho);
-  Q.B[N] = seeded<T>(Bn, dir == 7);
+  Q.B[N] = seeded(Bn, 7 - base, (T*)nullptr);
   Q.B[T1] = x[5];
   Q.B[T2] = x[6];
 }
@@ -64,8 +62,8 @@ SRMHD_DEV void flux_vjp_dual(const double ql[7], const double qr[7], double Bn,
 #pragma unroll 1
   for (int d = (SOL == SOLVER_HLLD && ps >= 0.0) ? -1 : 0; d < 15; ++d) {
     PrimT<D1> L, R;
-    side_prims_q<N, D1>(ql, Bn, e, d == 14 ? 7 : (d < 7 ? d : -1), L);
-    side_prims_q<N, D1>(qr, Bn, e, d == 14 ? 7 : (d >= 7 ? d - 7 : -1), R);
+    side_prims_q<N, D1>(ql, Bn, e, d == 14 ? 7 : (d >= 0 && d < 7 ? d : -100), L);
+    side_prims_q<N, D1>(qr, Bn, e, d == 14 ? 7 : (d >= 7 ? d - 7 : -100), R);
     D1 F[8];
     if (SOL == SOLVER_HLLE) {
       D1 U[8], pso;
@@ -656,15 +654,231 @@ SRMHD_DEV void rvec_zero(StarR& r) {
   r.cx = 0.0;
 }
 
+// ── reverse mode, one side ──────────────────────────────────────────────────
+// The cold part of the EOS (eos_cold) with the derivatives of pc and ec in
+// rho along the branch the value takes.
+SRMHD_DEV void eos_cold_d(const srmhd_eos& e, double rho, double& pc, double& ec, double& dpc,
+                          double& dec) {
+  eos_cold(e, rho, pc, ec);
+  const bool zero_k = e.K == 0.0 && e.gamma >= 0.0 && e.gamma <= 3.0;
+  // d (K rho^gamma) / d rho = gamma pc / rho (rho > 0: the callers clamp it)
+  dpc = (zero_k && rho >= 0.0 && rho <= 1e100) ? 0.0 : e.gamma * pc / rho;
+  const double den = rho * (e.gamma - 1.0);
+  dec = (pc == 0.0 && den > 0.0 && den <= DBL_MAX) ? 0.0 : (dpc - ec * (e.gamma - 1.0)) / den;
+}
+
+// The vector-Jacobian product of one side's state, flux and speeds
+// (side_prims_q then side_state) with respect to its 8 inputs (q[0..6],
+// B_n): lq8[m] = sum_k (lu[k] dU_k + lf[k] dF_k) / dx_m + lcp dc+/dx_m +
+// lcn dc-/dx_m.  The forward's arithmetic once, then its reverse; the
+// branches (the clamps, the EOS cases, the order of the two speeds, a tie
+// splitting the adjoint evenly) as the forward takes them.
+template <int N>
+SRMHD_DEV void side_rev(const double q[7], double Bn, const srmhd_eos& e, const double lu[8],
+                        const double lf[8], double lcp, double lcn, double lq8[8]) {
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+  // forward: the face primitives
+  const double z[3] = {q[2], q[3], q[4]};
+  const double z2 = sq(z[0]) + sq(z[1]) + sq(z[2]);
+  const double s1 = csqrt(1.0 + z2);
+  const double fac = 1.0 / s1;
+  double v[3], B[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] = z[i] * fac;
+  B[N] = Bn;
+  B[T1] = q[5];
+  B[T2] = q[6];
+  const double rho = clo(q[0], 1e-12), pin = clo(q[1], 1e-14);
+  double pc, ec, dpc, dec;
+  eos_cold_d(e, rho, pc, ec, dpc, dec);
+  const double g1 = e.gamma_th - 1.0;
+  const double eps = e.is_cold ? ec : ec + (pin - pc) / (rho * g1);
+  // forward: side_state
+  const double vn = v[N];
+  const double v2 = sq(v[0]) + sq(v[1]) + sq(v[2]);
+  const double sW = csqrt(1.0 - chi_(v2, 1.0 - 1e-10));
+  const double W = 1.0 / sW, W2 = W * W;
+  const double epsm = tmax(eps, ec);
+  const double p = e.is_cold ? pc : pc + (epsm - ec) * rho * g1;
+  const double hh = 1.0 + eps + p / rho;
+  const double rh = rho * hh;
+  const double cs2 = (e.is_cold ? e.gamma * p : e.gamma * pc + e.gamma_th * (p - pc)) / rh;
+  const double B2 = sq(B[0]) + sq(B[1]) + sq(B[2]);
+  const double vdB = v[0] * B[0] + v[1] * B[1] + v[2] * B[2];
+  const double b2 = B2 / W2 + sq(vdB);
+  const double bt = W * vdB;
+  double b[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) b[i] = B[i] / W + vdB * v[i] * W;
+  const double rhb2 = rh + b2;
+  const double fr = b2 / rhb2;
+  const double v02 = cs2 + fr * (1.0 - cs2);
+  const double om = 1.0 - v02;
+  const double a = W2 * om + v02;
+  const double bq = -2.0 * (W2 * vn * om);
+  const double c = W2 * sq(vn) * om - v02;
+  const double disc = bq * bq - 4.0 * a * c;
+  const double det = csqrt(clo(disc, 0.0));
+  const double ac = clo(a, TINY);
+  const double r1 = 0.5 * (det - bq) / ac, r2 = -0.5 * (det + bq) / ac;
+  const double d = rho * W;
+  const double bn = b[N];
+  // reverse: the conserved state and flux
+  double gW = 0.0, gvn = 0.0, grhb2 = 0.0, gbt = 0.0, gbn = 0.0, gPb2 = 0.0, gd = 0.0;
+  double gb[3], gv[3], gB[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    gb[i] = 0.0;
+    gv[i] = 0.0;
+    gB[i] = lu[CB + i] + lf[CB + i] * vn;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {   // F_B = vn B - v B_n
+    gvn += lf[CB + i] * B[i];
+    gv[i] -= lf[CB + i] * B[N];
+    gB[N] -= lf[CB + i] * v[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {   // U_S = rhb2 W u_i - bt b_i, F_S likewise
+    const double ui = W * v[i], ls = lu[CS + i], lfs = lf[CS + i];
+    grhb2 += (ls * W + lfs * (W * vn)) * ui;
+    gW += (ls + lfs * vn) * rhb2 * ui;
+    gvn += lfs * rhb2 * W * ui;
+    const double gui = (ls + lfs * vn) * rhb2 * W;
+    gbt -= ls * b[i];
+    gb[i] -= ls * bt + lfs * bn;
+    gbn -= lfs * b[i];
+    if (i == N) gPb2 += lfs;
+    gW += gui * v[i];
+    gv[i] += gui * W;
+  }
+  {   // tau and its flux
+    const double l = lf[CT], m = lu[CT];
+    grhb2 += (l * vn + m) * W2;
+    gW += (l * vn + m) * rhb2 * 2.0 * W;
+    gvn += l * (rhb2 * W2 - d);
+    gbt -= l * bn + 2.0 * m * bt;
+    gbn -= l * bt;
+    gd -= l * vn + m;
+    gPb2 -= m;
+  }
+  gd += lu[CD] + lf[CD] * vn;
+  gvn += lf[CD] * d;
+  gb[N] += gbn;
+  double gp = gPb2, gb2 = 0.5 * gPb2;
+  double grho = gd * W;
+  gW += gd * rho;
+  // reverse: the characteristic speeds (char_speeds)
+  double gr1 = 0.0, gr2 = 0.0;
+  if (r1 == r2) {
+    gr1 = gr2 = 0.5 * (lcp + lcn);
+  } else {
+    if (r1 > r2 || r1 != r1) gr1 += lcp; else gr2 += lcp;
+    if (r1 < r2 || r1 != r1) gr1 += lcn; else gr2 += lcn;
+  }
+  const double gdet = 0.5 * (gr1 - gr2) / ac;
+  double gbq = -0.5 * (gr1 + gr2) / ac;
+  double ga = a < TINY ? 0.0 : -(gr1 * r1 + gr2 * r2) / ac;
+  const double gdisc = disc < 0.0 ? 0.0 : gdet * (0.5 / det);
+  gbq += gdisc * 2.0 * bq;
+  ga -= 4.0 * c * gdisc;
+  const double gc = -4.0 * a * gdisc;
+  const double gW2 = gc * sq(vn) * om - 2.0 * gbq * vn * om + ga * om;
+  gvn += (gc * 2.0 * vn - 2.0 * gbq) * W2 * om;
+  const double gom = gc * W2 * sq(vn) - 2.0 * gbq * W2 * vn + ga * W2;
+  const double gv02 = -gc + ga - gom;
+  gW += gW2 * 2.0 * W;
+  // reverse: v02, rhb2, cs2
+  const double gcs2 = gv02 * (1.0 - fr);
+  const double gfr = gv02 * (1.0 - cs2);
+  gb2 += gfr / rhb2;
+  grhb2 -= gfr * fr / rhb2;
+  gb2 += grhb2;
+  const double grh = grhb2 - gcs2 * cs2 / rh, gnum = gcs2 / rh;
+  double gpc = 0.0, gec = 0.0;
+  if (e.is_cold) {
+    gp += gnum * e.gamma;
+  } else {
+    gpc += gnum * (e.gamma - e.gamma_th);
+    gp += gnum * e.gamma_th;
+  }
+  grho += grh * hh;
+  const double ghh = grh * rho;
+  // reverse: b, bt, b2
+  double gvdB = gbt * W;
+  gW += gbt * vdB;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    gB[i] += gb[i] / W;
+    gW += gb[i] * (vdB * v[i] - B[i] / W2);
+    gvdB += gb[i] * v[i] * W;
+    gv[i] += gb[i] * vdB * W;
+  }
+  const double gB2 = gb2 / W2;
+  gW -= 2.0 * gb2 * B2 / (W2 * W);
+  gvdB += 2.0 * vdB * gb2;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    gB[i] += 2.0 * B[i] * gB2 + gvdB * v[i];
+    gv[i] += gvdB * B[i];
+  }
+  // reverse: h = 1 + eps + p / rho, then p (eos_press)
+  double geps = ghh;
+  gp += ghh / rho;
+  grho -= ghh * p / (rho * rho);
+  gpc += gp;
+  if (!e.is_cold) {
+    const double ge = gp * rho * g1;
+    gec -= ge;
+    grho += gp * (epsm - ec) * g1;
+    if (eps == ec) {
+      geps += 0.5 * ge;
+      gec += 0.5 * ge;
+    } else if (eps > ec || eps != eps) {
+      geps += ge;
+    } else {
+      gec += ge;
+    }
+  }
+  {   // W = 1 / sqrt(1 - min(v2, 1 - 1e-10))
+    const double gv2 = v2 > 1.0 - 1e-10 ? 0.0 : gW * W2 * (0.5 / sW);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) gv[i] += 2.0 * v[i] * gv2;
+  }
+  gv[N] += gvn;
+  // reverse: eps (eos_eps), the cold EOS, the face primitives
+  double gpin = 0.0;
+  gec += geps;
+  if (!e.is_cold) {
+    const double dd = rho * g1;
+    gpin += geps / dd;
+    gpc -= geps / dd;
+    grho -= geps * (pin - pc) / (dd * rho);
+  }
+  grho += gpc * dpc + gec * dec;
+  double gfac = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) gfac += gv[i] * z[i];
+  const double gz2 = -gfac * fac * fac * (0.5 / s1);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) lq8[2 + i] = gv[i] * fac + 2.0 * z[i] * gz2;
+  lq8[0] = q[0] < 1e-12 ? 0.0 : grho;
+  lq8[1] = q[1] < 1e-14 ? 0.0 : gpin;
+  lq8[5] = gB[T1];
+  lq8[6] = gB[T2];
+  lq8[7] = gB[N];
+}
+
 // ── reverse mode, HLLD ──────────────────────────────────────────────────────
 template <int N>
 SRMHD_DEV void flux_vjp_hlld(const double ql[7], const double qr[7], double Bn,
                              const srmhd_eos& e, double ps, const double lf[8], double lq[15]) {
   Prim L, R;
-  side_prims_q<N, double>(ql, Bn, e, -1, L);
-  side_prims_q<N, double>(qr, Bn, e, -1, R);
+  side_prims_q<N, double>(ql, Bn, e, 0, L);
+  side_prims_q<N, double>(qr, Bn, e, 0, R);
   HlldPrep<N> h;
-  h.run(L, R, e);
+  double cp[2], cmn[2];
+  h.run(L, R, e, cp, cmn);
   double fH[8];
   hll_fluxes<N>(h, fH);
   double luL[8], lfL[8], luR[8], lfR[8], lfH[8];
@@ -713,13 +927,6 @@ SRMHD_DEV void flux_vjp_hlld(const double ql[7], const double qr[7], double Bn,
   // the HLL speeds from the side speeds (srmhd_fluxes)
   double lcpl = 0.0, lcpr = 0.0, lcml = 0.0, lcmr = 0.0;
   {
-    Prim P2[2] = {L, R};
-    double cp[2], cmn[2];
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      double u[8], f[8];
-      side_state<N>(P2[s], e, u, f, cp[s], cmn[s]);
-    }
     const double ax = tmax(cp[0], cp[1]), am = tmax(-cmn[0], -cmn[1]);
     const double cxr = clo(ax, 0.0), cmr = clo(am, 0.0);
     if (!(cxr < 1e-12 && cmr < 1e-12)) {
@@ -743,30 +950,16 @@ SRMHD_DEV void flux_vjp_hlld(const double ql[7], const double qr[7], double Bn,
       }
     }
   }
-  // the side states, one dual pass per input of the side
+  // the side states, reverse mode
+  double l8[8];
+  side_rev<N>(ql, Bn, e, luL, lfL, lcpl, lcml, l8);
 #pragma unroll
-  for (int m = 0; m < 15; ++m) lq[m] = 0.0;
-#pragma unroll 1
-  for (int s = 0; s < 2; ++s) {
-    const double* q = s ? qr : ql;
-    const double* lu = s ? luR : luL;
-    const double* lfs = s ? lfR : lfL;
-    const double lcp = s ? lcpr : lcpl, lcn = s ? lcmr : lcml;
-#pragma unroll 1
-    for (int m = 0; m < 8; ++m) {
-      PrimT<Dual<1>> P;
-      side_prims_q<N, Dual<1>>(q, Bn, e, m, P);
-      Dual<1> u[8], f[8], cp, cmn;
-      side_state<N>(P, e, u, f, cp, cmn);
-      double acc = lcp * cp.d[0] + lcn * cmn.d[0];
+  for (int m = 0; m < 7; ++m) lq[m] = l8[m];
+  lq[14] = l8[7];
+  side_rev<N>(qr, Bn, e, luR, lfR, lcpr, lcmr, l8);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc += lu[k] * u[k].d[0] + lfs[k] * f[k].d[0];
-      if (m < 7)
-        lq[s * 7 + m] = acc;
-      else
-        lq[14] += acc;
-    }
-  }
+  for (int m = 0; m < 7; ++m) lq[7 + m] = l8[m];
+  lq[14] += l8[7];
 }
 
 }  // namespace srmhd

@@ -244,7 +244,13 @@ template <int N, class T = double>
 struct HlldPrep {
   T uL[8], uR[8], fL[8], fR[8], cmax, cmin, uH[8];
   SRMHD_DEV void run(const PrimT<T>& L, const PrimT<T>& R, const srmhd_eos& e) {
-    srmhd_fluxes<N>(L, R, e, uL, uR, fL, fR, cmax, cmin);
+    T cps[2], cms[2];
+    run(L, R, e, cps, cms);
+  }
+  // ... keeping each side's speeds c+ (cps) and c- (cms), left first
+  SRMHD_DEV void run(const PrimT<T>& L, const PrimT<T>& R, const srmhd_eos& e, T cps[2],
+                     T cms[2]) {
+    srmhd_fluxes<N>(L, R, e, uL, uR, fL, fR, cmax, cmin, cps, cms);
 #pragma unroll
     for (int k = 0; k < 8; ++k) uH[k] = hll_state(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
   }

@@ -11,9 +11,11 @@ reconstruction and the Riemann solver, HLLD's p* by implicit differentiation
 at the recorded root).
 
 Memory: the forward keeps a checkpoint (state + primitives) every
-``checkpoint_every`` steps; the backward re-runs one segment at a time with
-the stage records (``srmhd2d_stage_taped``) and walks it backwards, so the
-resident set is one segment of stage records plus the checkpoints.
+``checkpoint_every`` steps and runs the last segment with the stage records
+(``srmhd2d_stage_taped``); the backward walks that segment's records, then
+re-runs every earlier segment with records and walks it backwards, so the
+resident set is two segments of stage records plus the checkpoints (at 512^2,
+about 77 MB of records per RK2 step).
 
 ``OTRollout`` is the differentiable relativistic Orszag-Tang problem: the
 initial density field (physical cells) and the magnetic amplitude A are the
@@ -62,6 +64,7 @@ class Rollout2D:
             raise ValueError("grid / scheme rejected by srmhd2d_adjoint_workspace_bytes")
         self.aws = _lib.workspace(nb, self.device)
         self.ckpt = {}
+        self.tail, self.tail_start = [], 0
         self.dts = None
 
     # ── buffers ─────────────────────────────────────────────────────────────
@@ -70,6 +73,26 @@ class Rollout2D:
 
     def _zeros_prims(self):
         return [torch.zeros_like(t) for t in self.st.new_prims()]
+
+    def _zeros_group(self, n_states, n_prims):
+        """n_states state lists and n_prims primitive lists of zeros, cut
+        from one buffer so that one fill clears them all."""
+        g = self.grid
+        cell = (g.nxt, g.nyt)
+        state = [cell] * len(EVOLVED_KEYS) + [(g.nxt + 1, g.nyt), (g.nxt, g.nyt + 1)]
+        prims = [cell] * len(PRIM6)
+        shapes = state * n_states + prims * n_prims
+        flat = torch.zeros(sum(a * b for a, b in shapes), dtype=torch.float64,
+                           device=self.device)
+        views, o = [], 0
+        for a, b in shapes:
+            views.append(flat[o:o + a * b].view(a, b))
+            o += a * b
+        ns = len(state)
+        states = [views[i * ns:(i + 1) * ns] for i in range(n_states)]
+        base = n_states * ns
+        np_ = len(prims)
+        return states, [views[base + i * np_:base + (i + 1) * np_] for i in range(n_prims)]
 
     def _new_tape(self):
         g, d = self.grid, self.device
@@ -110,14 +133,24 @@ class Rollout2D:
 
     def forward(self, U, P, dts):
         """Run len(dts) steps from the synced state (U: 8 tensors, P: 6),
-        keeping checkpoints; returns the final (U, P)."""
+        keeping checkpoints and the last segment's stage records; returns
+        the final (U, P)."""
         self.dts = [float(x) for x in dts]
+        N = len(self.dts)
+        # the last segment starts at the last multiple of K below N
+        self.tail_start = ((N - 1) // self.K) * self.K if N else 0
+        self.tail = []
         U = [t.clone() for t in U]
         P = [t.clone() for t in P]
         self.ckpt = {0: ([t.clone() for t in U], [t.clone() for t in P])}
         for n, dt in enumerate(self.dts):
-            U, P = self._step(U, P, dt)
-            if (n + 1) % self.K == 0 and n + 1 < len(self.dts):
+            if n >= self.tail_start:
+                rec = []
+                U, P = self._step(U, P, dt, record=rec)
+                self.tail.append(rec)
+            else:
+                U, P = self._step(U, P, dt)
+            if (n + 1) % self.K == 0 and n + 1 < self.tail_start:
                 self.ckpt[n + 1] = ([t.clone() for t in U], [t.clone() for t in P])
         return U, P
 
@@ -141,23 +174,30 @@ class Rollout2D:
         stages = RK[self.scheme]
         lamU = [t.contiguous() for t in lamU]
         lamP = [t.contiguous() for t in lamP]
+        S = len(stages)
         for s0 in reversed(range(0, N, self.K)):
             s1 = min(s0 + self.K, N)
-            U, P = self.ckpt[s0]
-            recs = []
-            for n in range(s0, s1):
-                rec = []
-                U, P = self._step(U, P, self.dts[n], record=rec)
-                recs.append(rec)
+            if s0 == self.tail_start:
+                recs = self.tail     # recorded by forward()
+            else:
+                U, P = self.ckpt[s0]
+                recs = []
+                for n in range(s0, s1):
+                    rec = []
+                    U, P = self._step(U, P, self.dts[n], record=rec)
+                    recs.append(rec)
             for n in reversed(range(s0, s1)):
                 self.st.set_dt(self.dts[n])
                 rec = recs[n - s0]
-                acc = self._zeros_state()
-                for k in reversed(range(len(stages))):
+                # acc (the step input's adjoint), the adjoint of stage k > 0's
+                # input state and every stage's primitive adjoint
+                states, prims = self._zeros_group(S, S)
+                acc = states[0]
+                for k in reversed(range(S)):
                     a, b, c = stages[k]
                     Uk, Pk, Uout, _bufs, tape = rec[k]
-                    newU = acc if k == 0 else self._zeros_state()
-                    newP = self._zeros_prims()
+                    newU = acc if k == 0 else states[k]
+                    newP = prims[k]
                     self._stage_adjoint(a, b, c, Uk, Pk, Uout, tape, lamU, lamP, acc, newU, newP)
                     lamU, lamP = newU, newP
                 lamU = acc

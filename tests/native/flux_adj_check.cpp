@@ -1,10 +1,15 @@
 // Host-side check that the reverse-mode HLLD flux VJP (flux_vjp_hlld) equals
 // the forward-mode one (flux_vjp_dual, one dual pass per input) on random
 // face states of both normal directions, at the root p* the forward solver
-// finds (faces it sends to the HLLE fallback included).
+// finds (faces it sends to the HLLE fallback included), for three EOS
+// (ideal gas, hybrid with a cold polytrope, cold polytrope alone).
 // Usage: flux_adj_check N SEED -> prints "faces N fallback F worst W"
 //        (W: the largest |lq_rev - lq_dual| / max_m |lq_dual|); exit 1 if
-//        W > 1e-8.
+//        W > 1e-7.  The two forms round differently; where the implicit
+//        derivative of p* is ill-conditioned the difference reaches ~2e-8
+//        (one face in 20 000 with the hybrid EOS, the same for a form whose
+//        side states are differentiated in forward mode); a dropped or
+//        wrong term gives 1e-4 and more.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -60,19 +65,29 @@ static double one(std::mt19937_64& rng, const srmhd_eos& e, bool* fellback) {
 int main(int argc, char** argv) {
   const long n = argc > 1 ? atol(argv[1]) : 2000;
   std::mt19937_64 rng(argc > 2 ? atol(argv[2]) : 1);
-  srmhd_eos e{};
-  e.K = 0.0;
-  e.gamma = 4.0 / 3.0;
-  e.gamma_th = 4.0 / 3.0;
+  // the ideal gas (K = 0, gamma = 4/3), a hybrid EOS with a cold polytrope
+  // (K = 0.5, gamma = 2, gamma_th = 5/3) and the cold polytrope alone
+  srmhd_eos eos[3] = {};
+  eos[0].K = 0.0;
+  eos[0].gamma = eos[0].gamma_th = 4.0 / 3.0;
+  eos[1].K = 0.5;
+  eos[1].gamma = 2.0;
+  eos[1].gamma_th = 5.0 / 3.0;
+  eos[2].K = 0.5;
+  eos[2].gamma = 2.0;
+  eos[2].gamma_th = 1.0;
+  eos[2].is_cold = 1;
   double worst = 0.0;
   long fb = 0, bad = 0;
   for (long q = 0; q < n; ++q) {
     bool f;
+    const srmhd_eos& e = eos[(q / 2) % 3];
     const double w = (q % 2) ? one<1>(rng, e, &f) : one<0>(rng, e, &f);
     fb += f;
-    if (!(w <= 1e-8)) {
+    if (!(w <= 1e-7)) {
       ++bad;
-      if (bad <= 5) std::printf("face %ld: rel err %.3e (fallback %d)\n", q, w, (int)f);
+      if (bad <= 5)
+        std::printf("face %ld (eos %ld): rel err %.3e (fallback %d)\n", q, (q / 2) % 3, w, (int)f);
     }
     if (w > worst || w != w) worst = w;
   }
