@@ -459,10 +459,36 @@ __global__ __launch_bounds__(BLK) void k_adj_flux(G2 g, St2 S, Pr2 P, srmhd_eos 
   lf[N == 0 ? CB + 1 : CB + 0] = lamF.p[6][t];
   const double ps = SOL == SOLVER_HLLD ? pstar[t] : -1.0;
   double lq[15];
-  if (SOL == SOLVER_HLLD)
-    flux_vjp_hlld<N>(ql, qr, Bn, e, ps, lf, lq);
-  else
+  if constexpr (SOL == SOLVER_HLLD) {
+    // the face states wait in LDS (each lane its own slots) while the core
+    // runs, so that they hold no registers there
+    __shared__ double qs[15][BLK];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      qs[m][threadIdx.x] = ql[m];
+      qs[7 + m][threadIdx.x] = qr[m];
+    }
+    qs[14][threadIdx.x] = Bn;
+    double luL[8], lfL[8], luR[8], lfR[8], lc[4], l8[8];
+    hlld_core_rev<N>(ql, qr, Bn, e, ps, lf, luL, lfL, luR, lfR, lc);
+    __asm__ volatile("" ::: "memory");   // reload below, no forwarding
+#pragma unroll
+    for (int m = 0; m < 7; ++m) {
+      ql[m] = qs[m][threadIdx.x];
+      qr[m] = qs[7 + m][threadIdx.x];
+    }
+    Bn = qs[14][threadIdx.x];
+    side_rev<N>(ql, Bn, e, luL, lfL, lc[0], lc[1], l8);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) lq[m] = l8[m];
+    lq[14] = l8[7];
+    side_rev<N>(qr, Bn, e, luR, lfR, lc[2], lc[3], l8);
+#pragma unroll
+    for (int m = 0; m < 7; ++m) lq[7 + m] = l8[m];
+    lq[14] += l8[7];
+  } else {
     flux_vjp_dual<N, SOL>(ql, qr, Bn, e, ps, lf, lq);
+  }
 #pragma unroll
   for (int m = 0; m < 14; ++m) LQ[(int64_t)m * nf + t] = lq[m];
   lBn[t] = lq[14];

@@ -444,52 +444,23 @@ SRMHD_DEV void rvec_rev(const HlldPrep<N>& h, const StarR& lR, double luL[8], do
 }
 
 // hlld_assemble's flux (not failed) backwards for flux adjoint lF: into the
-// side-state, speed, HLL-flux, star-state and R-vector adjoints and the
-// direct adjoint of p*
+// side-state, speed, star-state and R-vector adjoints and the direct adjoint
+// of p*.  Returns the region at xi = 0 (hlld.py:1448-1480; the last true
+// test wins): -1 none (the flux is the HLL flux: nothing is added), 0 / 5
+// the left / right state, 1 / 4 the left / right Alfven state, 2 / 3 the
+// left / right contact state.  Only the side s of the region (s = 1 for
+// regions 3, 4) is formed: the other side's states reach no adjoint.
 template <int N>
-SRMHD_DEV void assemble_rev(const HlldPrep<N>& h, const StarR& R, const StarOut& o, double ps,
-                            const double lF[8], double luL[8], double lfL[8], double luR[8],
-                            double lfR[8], double& lcm, double& lcx, double lfH[8], StarOut& lo,
-                            double& lps) {
+SRMHD_DEV int assemble_rev(const HlldPrep<N>& h, const StarR& R, const StarOut& o, double ps,
+                           const double lF[8], double luL[8], double lfL[8], double luR[8],
+                           double lfR[8], double& lcm, double& lcx, StarOut& lo, double& lps) {
   constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
   const double cmin = h.cmin, cmax = h.cmax;
   const double BnR = h.uR[CB + N];
-  double uaL[8], uaR[8], faL[8], faR[8];
-  uaL[CB + N] = h.uL[CB + N]; uaL[CB + T1] = o.B1[0]; uaL[CB + T2] = o.B2[0];
-  uaL[CD] = o.Da[0]; uaL[CT] = o.Ta[0];
-  uaL[CS + N] = o.Sa[0]; uaL[CS + T1] = o.S1[0]; uaL[CS + T2] = o.S2[0];
-  uaR[CB + N] = BnR; uaR[CB + T1] = o.B1[1]; uaR[CB + T2] = o.B2[1];
-  uaR[CD] = o.Da[1]; uaR[CT] = o.Ta[1];
-  uaR[CS + N] = o.Sa[1]; uaR[CS + T1] = o.S1[1]; uaR[CS + T2] = o.S2[1];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    faL[k] = h.fL[k] + (-cmin) * (uaL[k] - h.uL[k]);
-    faR[k] = h.fR[k] + cmax * (uaR[k] - h.uR[k]);
-  }
   const double dfac = -R.Sg[0] * o.sq[0] - o.KBL;
   const double fac = sdiv(1.0 - o.KL2, dfac);
   const double laL = o.la[0], laR = o.la[1];
   const double vc = laL - BnR * fac;
-  const double vt1 = o.K1L - o.Bc1 * fac;
-  const double vt2 = o.K2L - o.Bc2 * fac;
-  const double vBc = vc * BnR + vt1 * o.Bc1 + vt2 * o.Bc2;
-  const double sL = sdiv(laL - o.va[0], laL - vc), sR = sdiv(laR - o.va[1], laR - vc);
-  double ucL[8], ucR[8];
-  ucL[CB + N] = BnR; ucL[CB + T1] = o.Bc1; ucL[CB + T2] = o.Bc2;
-  ucR[CB + N] = BnR; ucR[CB + T1] = o.Bc1; ucR[CB + T2] = o.Bc2;
-  ucL[CD] = uaL[CD] * sL;
-  ucR[CD] = uaR[CD] * sR;
-  const double nTL = laL * uaL[CT] - faL[CT] + ps * vc - vBc * BnR;
-  const double nTR = laR * uaR[CT] - faR[CT] + ps * vc - vBc * BnR;
-  ucL[CT] = sdiv(nTL, laL - vc);
-  ucR[CT] = sdiv(nTR, laR - vc);
-  const double gl = ucL[CT] + ps + ucL[CD], gr = ucR[CT] + ps + ucR[CD];
-  ucL[CS + N] = gl * vc - vBc * BnR; ucL[CS + T1] = gl * vt1 - vBc * o.Bc1;
-  ucL[CS + T2] = gl * vt2 - vBc * o.Bc2;
-  ucR[CS + N] = gr * vc - vBc * BnR; ucR[CS + T1] = gr * vt1 - vBc * o.Bc1;
-  ucR[CS + T2] = gr * vt2 - vBc * o.Bc2;
-
-  // the region at xi = 0 (hlld.py:1448-1480; the last true test wins)
   const bool mL = 0.0 <= -cmin, maL = (-cmin < 0.0) && (0.0 <= laL);
   const bool mcL = (laL < 0.0) && (0.0 < vc), mcR = (vc <= 0.0) && (0.0 < laR);
   const bool maR = (laR <= 0.0) && (0.0 < cmax), mR = 0.0 >= cmax;
@@ -500,41 +471,62 @@ SRMHD_DEV void assemble_rev(const HlldPrep<N>& h, const StarR& R, const StarOut&
   if (mcR) reg = 3;
   if (maR) reg = 4;
   if (mR) reg = 5;
-
-  double luaL[8], luaR[8], lfaL[8], lfaR[8], lucL[8], lucR[8];
+  if (reg == 0 || reg == 5) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      lfL[k] += reg == 0 ? lF[k] : 0.0;
+      lfR[k] += reg == 5 ? lF[k] : 0.0;
+    }
+  }
+  if (!(reg >= 1 && reg <= 4)) return reg;
+  // the side of the region: its Alfven state (ua), flux (fa), fast speed
+  const bool s = reg >= 3, contact = reg == 2 || reg == 3;
+  const double lam = s ? cmax : -cmin, la = s ? laR : laL, va = s ? o.va[1] : o.va[0];
+  double hu[8], ua[8], fa[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) hu[k] = s ? h.uR[k] : h.uL[k];
+  ua[CB + N] = hu[CB + N];
+  ua[CB + T1] = s ? o.B1[1] : o.B1[0];
+  ua[CB + T2] = s ? o.B2[1] : o.B2[0];
+  ua[CD] = s ? o.Da[1] : o.Da[0];
+  ua[CT] = s ? o.Ta[1] : o.Ta[0];
+  ua[CS + N] = s ? o.Sa[1] : o.Sa[0];
+  ua[CS + T1] = s ? o.S1[1] : o.S1[0];
+  ua[CS + T2] = s ? o.S2[1] : o.S2[0];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) fa[k] = (s ? h.fR[k] : h.fL[k]) + lam * (ua[k] - hu[k]);
+  double lua[8], lfa[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    luaL[k] = 0.0; luaR[k] = 0.0; lfaL[k] = 0.0; lfaR[k] = 0.0; lucL[k] = 0.0; lucR[k] = 0.0;
+    lfa[k] = lF[k];
+    lua[k] = 0.0;
   }
-  double llaL = 0.0, llaR = 0.0;
+  if (contact) {
+    // the contact state of side s (forward)
+    const double vt1 = o.K1L - o.Bc1 * fac;
+    const double vt2 = o.K2L - o.Bc2 * fac;
+    const double vBc = vc * BnR + vt1 * o.Bc1 + vt2 * o.Bc2;
+    const double sD = sdiv(la - va, la - vc);
+    double uc[8];
+    uc[CB + N] = BnR; uc[CB + T1] = o.Bc1; uc[CB + T2] = o.Bc2;
+    uc[CD] = ua[CD] * sD;
+    const double nT = la * ua[CT] - fa[CT] + ps * vc - vBc * BnR;
+    uc[CT] = sdiv(nT, la - vc);
+    const double g = uc[CT] + ps + uc[CD];
+    uc[CS + N] = g * vc - vBc * BnR;
+    uc[CS + T1] = g * vt1 - vBc * o.Bc1;
+    uc[CS + T2] = g * vt2 - vBc * o.Bc2;
+    // F = fa + la (uc - ua)
+    double lla = 0.0, luc[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const double l = lF[k];
-    if (reg < 0) lfH[k] += l;
-    else if (reg == 0) lfL[k] += l;
-    else if (reg == 1) lfaL[k] += l;
-    else if (reg == 2) {
-      lfaL[k] += l; llaL += l * (ucL[k] - uaL[k]); lucL[k] += laL * l; luaL[k] -= laL * l;
-    } else if (reg == 3) {
-      lfaR[k] += l; llaR += l * (ucR[k] - uaR[k]); lucR[k] += laR * l; luaR[k] -= laR * l;
-    } else if (reg == 4) lfaR[k] += l;
-    else lfR[k] += l;
-  }
-
-  // contact states
-  double lvc = 0.0, lvt1 = 0.0, lvt2 = 0.0, lvBc = 0.0, lBnR = 0.0, lfac = 0.0;
-  double lBc1 = 0.0, lBc2 = 0.0;
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const double* uc = s ? ucR : ucL;
-    double* luc = s ? lucR : lucL;
-    const double* ua = s ? uaR : uaL;
-    double* lua = s ? luaR : luaL;
-    double* lfa = s ? lfaR : lfaL;
-    const double la = s ? laR : laL, va = s ? o.va[1] : o.va[0];
-    const double g = s ? gr : gl, sD = s ? sR : sL, nT = s ? nTR : nTL;
-    double& lla = s ? llaR : llaL;
-    double lg = 0.0;
+    for (int k = 0; k < 8; ++k) {
+      const double l = lF[k];
+      lla += l * (uc[k] - ua[k]);
+      luc[k] = la * l;
+      lua[k] -= la * l;
+    }
+    double lvc = 0.0, lvt1 = 0.0, lvt2 = 0.0, lvBc = 0.0, lBnR = 0.0, lfac = 0.0;
+    double lBc1 = 0.0, lBc2 = 0.0, lg = 0.0;
     lg += luc[CS + N] * vc; lvc += luc[CS + N] * g; lvBc -= luc[CS + N] * BnR;
     lBnR -= luc[CS + N] * vBc;
     lg += luc[CS + T1] * vt1; lvt1 += luc[CS + T1] * g; lvBc -= luc[CS + T1] * o.Bc1;
@@ -548,49 +540,65 @@ SRMHD_DEV void assemble_rev(const HlldPrep<N>& h, const StarR& R, const StarOut&
       lla += ln * ua[CT]; lua[CT] += ln * la; lfa[CT] -= ln; lps += ln * vc; lvc += ln * ps;
       lvBc -= ln * BnR; lBnR -= ln * vBc;
       lla += ld; lvc -= ld;
-      (void)nT;
     }
     {
       lua[CD] += luc[CD] * sD;
       const double lsD = luc[CD] * ua[CD];
       double ln = 0.0, ld = 0.0;
       sdiv_rev(la - vc, sD, lsD, ln, ld);
-      lla += ln; (s ? lo.va[1] : lo.va[0]) -= ln;
+      lla += ln;
+      lo.va[0] -= s ? 0.0 : ln;
+      lo.va[1] -= s ? ln : 0.0;
       lla += ld; lvc -= ld;
     }
     lBnR += luc[CB + N];
     lBc1 += luc[CB + T1];
     lBc2 += luc[CB + T2];
+    lvc += lvBc * BnR; lBnR += lvBc * vc; lvt1 += lvBc * o.Bc1; lBc1 += lvBc * vt1;
+    lvt2 += lvBc * o.Bc2; lBc2 += lvBc * vt2;
+    lo.K2L += lvt2; lBc2 -= lvt2 * fac; lfac -= lvt2 * o.Bc2;
+    lo.K1L += lvt1; lBc1 -= lvt1 * fac; lfac -= lvt1 * o.Bc1;
+    // vc = la_L - B_nR fac
+    double llaL = lvc;
+    lBnR -= lvc * fac; lfac -= lvc * BnR;
+    {
+      double ln = 0.0, ld = 0.0;
+      sdiv_rev(dfac, fac, lfac, ln, ld);
+      lo.KL2 -= ln; lo.sq[0] -= ld * R.Sg[0]; lo.KBL -= ld;
+    }
+    lo.Bc1 += lBc1;
+    lo.Bc2 += lBc2;
+    llaL += s ? 0.0 : lla;
+    lo.la[0] += llaL;
+    lo.la[1] += s ? lla : 0.0;
+    luR[CB + N] += lBnR;
   }
-  lvc += lvBc * BnR; lBnR += lvBc * vc; lvt1 += lvBc * o.Bc1; lBc1 += lvBc * vt1;
-  lvt2 += lvBc * o.Bc2; lBc2 += lvBc * vt2;
-  lo.K2L += lvt2; lBc2 -= lvt2 * fac; lfac -= lvt2 * o.Bc2;
-  lo.K1L += lvt1; lBc1 -= lvt1 * fac; lfac -= lvt1 * o.Bc1;
-  llaL += lvc; lBnR -= lvc * fac; lfac -= lvc * BnR;
-  {
-    double ln = 0.0, ld = 0.0;
-    sdiv_rev(dfac, fac, lfac, ln, ld);
-    lo.KL2 -= ln; lo.sq[0] -= ld * R.Sg[0]; lo.KBL -= ld;
-  }
-  lo.Bc1 += lBc1;
-  lo.Bc2 += lBc2;
-  lo.la[0] += llaL;
-  lo.la[1] += llaR;
-  // Alfven fluxes and states
+  // the Alfven flux fa = f + lam (ua - u) of side s
+  double ll = 0.0;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    lfL[k] += lfaL[k]; luaL[k] -= cmin * lfaL[k]; luL[k] += cmin * lfaL[k];
-    lcm -= (uaL[k] - h.uL[k]) * lfaL[k];
-    lfR[k] += lfaR[k]; luaR[k] += cmax * lfaR[k]; luR[k] -= cmax * lfaR[k];
-    lcx += (uaR[k] - h.uR[k]) * lfaR[k];
+    const double lu = -lam * lfa[k];
+    lua[k] += lam * lfa[k];
+    ll += (ua[k] - hu[k]) * lfa[k];
+    lfL[k] += s ? 0.0 : lfa[k];
+    lfR[k] += s ? lfa[k] : 0.0;
+    luL[k] += s ? 0.0 : lu;
+    luR[k] += s ? lu : 0.0;
   }
-  luL[CB + N] += luaL[CB + N]; lo.B1[0] += luaL[CB + T1]; lo.B2[0] += luaL[CB + T2];
-  lo.Da[0] += luaL[CD]; lo.Ta[0] += luaL[CT];
-  lo.Sa[0] += luaL[CS + N]; lo.S1[0] += luaL[CS + T1]; lo.S2[0] += luaL[CS + T2];
-  lBnR += luaR[CB + N]; lo.B1[1] += luaR[CB + T1]; lo.B2[1] += luaR[CB + T2];
-  lo.Da[1] += luaR[CD]; lo.Ta[1] += luaR[CT];
-  lo.Sa[1] += luaR[CS + N]; lo.S1[1] += luaR[CS + T1]; lo.S2[1] += luaR[CS + T2];
-  luR[CB + N] += lBnR;
+  lcm -= s ? 0.0 : ll;   // lam = -cmin
+  lcx += s ? ll : 0.0;   // lam = cmax
+  // the Alfven state of side s from the star states
+  luL[CB + N] += s ? 0.0 : lua[CB + N];
+  luR[CB + N] += s ? lua[CB + N] : 0.0;
+  const double w0 = s ? 0.0 : 1.0, w1 = s ? 1.0 : 0.0;
+  lo.B1[0] += w0 * lua[CB + T1]; lo.B1[1] += w1 * lua[CB + T1];
+  lo.B2[0] += w0 * lua[CB + T2]; lo.B2[1] += w1 * lua[CB + T2];
+  lo.Da[0] += w0 * lua[CD]; lo.Da[1] += w1 * lua[CD];
+  lo.Ta[0] += w0 * lua[CT]; lo.Ta[1] += w1 * lua[CT];
+  lo.Sa[0] += w0 * lua[CS + N]; lo.Sa[1] += w1 * lua[CS + N];
+  lo.S1[0] += w0 * lua[CS + T1]; lo.S1[1] += w1 * lua[CS + T1];
+  lo.S2[0] += w0 * lua[CS + T2]; lo.S2[1] += w1 * lua[CS + T2];
+  return reg;
 }
 
 // d/dp of the star states and the residual's total derivative at p, by one
@@ -870,52 +878,47 @@ SRMHD_DEV void side_rev(const double q[7], double Bn, const srmhd_eos& e, const 
 }
 
 // ── reverse mode, HLLD ──────────────────────────────────────────────────────
+// Everything of flux_vjp_hlld but the side states' reverse: the adjoints of
+// each side's state (luL, luR), flux (lfL, lfR) and speeds (lc: c+ left, c-
+// left, c+ right, c- right).
 template <int N>
-SRMHD_DEV void flux_vjp_hlld(const double ql[7], const double qr[7], double Bn,
-                             const srmhd_eos& e, double ps, const double lf[8], double lq[15]) {
+SRMHD_DEV void hlld_core_rev(const double ql[7], const double qr[7], double Bn,
+                             const srmhd_eos& e, double ps, const double lf[8], double luL[8],
+                             double lfL[8], double luR[8], double lfR[8], double lc[4]) {
   Prim L, R;
   side_prims_q<N, double>(ql, Bn, e, 0, L);
   side_prims_q<N, double>(qr, Bn, e, 0, R);
   HlldPrep<N> h;
   double cp[2], cmn[2];
   h.run(L, R, e, cp, cmn);
-  double fH[8];
-  hll_fluxes<N>(h, fH);
-  double luL[8], lfL[8], luR[8], lfR[8], lfH[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    luL[k] = 0.0; lfL[k] = 0.0; luR[k] = 0.0; lfR[k] = 0.0; lfH[k] = 0.0;
+    luL[k] = 0.0; lfL[k] = 0.0; luR[k] = 0.0; lfR[k] = 0.0;
   }
   double lcm = 0.0, lcx = 0.0;
-  if (!(ps >= 0.0)) {   // the forward fell back to the HLLE flux
-#pragma unroll
-    for (int k = 0; k < 8; ++k) lfH[k] = lf[k];
-  } else {
-    StarR Rv;
+  // the flux is the HLL flux where the forward fell back to it, or where no
+  // region test holds
+  bool hll = !(ps >= 0.0);
+  StarR Rv;
+  StarOut o, lo;
+  double lps = 0.0;
+  int reg = -1;
+  if (!hll) {
     hlld_rvec<N>(h, Rv);
-    StarOut o;
     star_solve(Rv, ps, o);
-    StarOut lo;
     star_zero(lo);
-    StarR lR;
-    rvec_zero(lR);
-    double lps = 0.0;
-    assemble_rev<N>(h, Rv, o, ps, lf, luL, lfL, luR, lfR, lcm, lcx, lfH, lo, lps);
-    StarOut op;
-    double resp;
-    star_dp(Rv, ps, op, resp);
-    const double c = -(star_dot(lo, op) + lps) / resp;
-    residual_rev(Rv, o, c, lo, lR);
-    star_solve_rev(Rv, ps, lo, lR);
-    rvec_rev<N>(h, lR, luL, lfL, luR, lfR, lcm, lcx);
+    reg = assemble_rev<N>(h, Rv, o, ps, lf, luL, lfL, luR, lfR, lcm, lcx, lo, lps);
+    hll = reg < 0;
   }
   // the HLL fluxes (hlld.py:161-168)
-  {
+  if (hll) {
+    double fH[8];
+    hll_fluxes<N>(h, fH);
     const double cm = h.cmin, cx = h.cmax, den = cx + cm;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const double du = h.uR[k] - h.uL[k];
-      const double ln = lfH[k] / den;
+      const double ln = lf[k] / den;
       lfL[k] += cx * ln;
       lfR[k] += cm * ln;
       luR[k] -= cx * cm * ln;
@@ -923,6 +926,18 @@ SRMHD_DEV void flux_vjp_hlld(const double ql[7], const double qr[7], double Bn,
       lcx += (h.fL[k] - cm * du) * ln - ln * fH[k];
       lcm += (h.fR[k] - cx * du) * ln - ln * fH[k];
     }
+  }
+  // p* (regions 1 to 4; elsewhere lo and lps are 0: nothing reaches it)
+  if (reg >= 1 && reg <= 4) {
+    StarOut op;
+    double resp;
+    star_dp(Rv, ps, op, resp);
+    const double c = -(star_dot(lo, op) + lps) / resp;
+    StarR lR;
+    rvec_zero(lR);
+    residual_rev(Rv, o, c, lo, lR);
+    star_solve_rev(Rv, ps, lo, lR);
+    rvec_rev<N>(h, lR, luL, lfL, luR, lfR, lcm, lcx);
   }
   // the HLL speeds from the side speeds (srmhd_fluxes)
   double lcpl = 0.0, lcpr = 0.0, lcml = 0.0, lcmr = 0.0;
@@ -950,13 +965,24 @@ SRMHD_DEV void flux_vjp_hlld(const double ql[7], const double qr[7], double Bn,
       }
     }
   }
-  // the side states, reverse mode
-  double l8[8];
-  side_rev<N>(ql, Bn, e, luL, lfL, lcpl, lcml, l8);
+  lc[0] = lcpl;
+  lc[1] = lcml;
+  lc[2] = lcpr;
+  lc[3] = lcmr;
+}
+
+// The VJP of the face flux with respect to the 15 face inputs: the core,
+// then each side's state backwards (side_rev).
+template <int N>
+SRMHD_DEV void flux_vjp_hlld(const double ql[7], const double qr[7], double Bn,
+                             const srmhd_eos& e, double ps, const double lf[8], double lq[15]) {
+  double luL[8], lfL[8], luR[8], lfR[8], lc[4], l8[8];
+  hlld_core_rev<N>(ql, qr, Bn, e, ps, lf, luL, lfL, luR, lfR, lc);
+  side_rev<N>(ql, Bn, e, luL, lfL, lc[0], lc[1], l8);
 #pragma unroll
   for (int m = 0; m < 7; ++m) lq[m] = l8[m];
   lq[14] = l8[7];
-  side_rev<N>(qr, Bn, e, luR, lfR, lcpr, lcmr, l8);
+  side_rev<N>(qr, Bn, e, luR, lfR, lc[2], lc[3], l8);
 #pragma unroll
   for (int m = 0; m < 7; ++m) lq[7 + m] = l8[m];
   lq[14] += l8[7];
