@@ -59,6 +59,14 @@ VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 1.836e8, ("fp64", "sweep_
 # issued a vector instruction every quad-cycle of the launch
 VALU_BUSY = {("fp64", "sweep_x_a", 2048): 0.973}
 
+# the adjoint's x sweep at 512^2 (k_adj_flux<0, 1, 2> + k_adj_recon<0, 1>),
+# rocprofv3 --pmc on the Orszag-Tang gradient (profiles/adjoint_ot512_counters.md):
+# FP64 flops per face (64 lanes x (add + mul + 2 fma + trans) per wave over
+# the launch's faces) and HBM bytes per launch pair (FETCH_SIZE scaled by the
+# copy kernel's 2.0, plus WRITE_SIZE)
+ADJ_SWEEP_FLOPS_PER_FACE = {512: 6783.0}
+ADJ_SWEEP_TRAFFIC = {512: 269.3e6}
+
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
 # (form, kernel class, n)
@@ -493,10 +501,19 @@ def bench_adjoint(a):
         "grad_rho_l2": float(g_rho.norm()),
         "peak_device_memory_GB": peak / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_adj_flux<0, 1, 2> + k_adj_recon<0, 1> (x-sweep adjoint)", "avg_launch_ms": avg_ms,
-                     "launches": cnt.value, "algorithmic_bytes_per_launch": kb},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": ADJ_SWEEP_TRAFFIC.get(n),
+                     "kernel": "k_adj_flux<0, 1, 2> + k_adj_recon<0, 1> (x-sweep adjoint)",
+                     "avg_launch_ms": avg_ms, "launches": cnt.value,
+                     "algorithmic_bytes_per_launch": kb},
     }
+    flops = ADJ_SWEEP_FLOPS_PER_FACE.get(n)
+    if flops is not None:
+        tf = flops * nfx / (avg_ms * 1e-3) / 1e12
+        out["roofline"]["fp64_valu"] = {
+            "flops_per_face": flops, "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tf / FP64_PEAK_TFLOPS,
+            "source": "rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 "
+                      "(profiles/adjoint_ot512_counters.md)"}
     if not a.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline_adjoint()

@@ -438,7 +438,7 @@ __global__ __launch_bounds__(BLK) void k_adj_flux(G2 g, St2 S, Pr2 P, srmhd_eos 
                                                   const double* pstar, CFlx lamF, double* LQ,
                                                   double* lBn) {
   const int64_t nf = n_faces<N>(g);
-  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const int64_t t = xcd_block() * BLK + threadIdx.x;
   if (t >= nf) return;
   double ql[7], qr[7], Bn;
   {
@@ -506,7 +506,7 @@ __global__ __launch_bounds__(BLK) void k_adj_recon(G2 g, St2 S, Pr2 P, const dou
   constexpr int hw = stencil_halfwidth(LIM);
   constexpr int k0 = 3 - hw, k1 = 4 + hw, KW = k1 - k0 + 1;
   const int64_t nf = n_faces<N>(g);
-  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const int64_t t = xcd_block() * BLK + threadIdx.x;
   if (t >= nf) return;
   double a[7][8], ql[7], qr[7], Bn;
   int f, n;
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(BLK) void k_adj_gather(G2 g, Pr2 P, const double* W
                                                     double* lBcx, double* lBcy) {
   constexpr int hw = stencil_halfwidth(LIM);
   constexpr int k0 = 3 - hw, k1 = 4 + hw, KW = k1 - k0 + 1;
-  const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
+  const int64_t t = xcd_block() * BLK + threadIdx.x;
   if (t >= (int64_t)g.nxt * g.nyt) return;
   const int ci = (int)(t / g.nyt), cj = (int)(t - (int64_t)ci * g.nyt);
   double lq[2][7];

@@ -35,6 +35,19 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
 }
 __device__ inline bool lane0() { return (threadIdx.x & 63) == 0; }
 
+// The workgroup's index in an order that gives each of the 8 XCDs one
+// contiguous run of the grid: the dispatcher deals workgroup b to XCD b % 8,
+// so XCD x holds b = x, x + 8, ...; they are renumbered to the x-th run of
+// the grid (runs of nb / 8, the first nb % 8 one longer).  A bijection on
+// [0, nb) for any nb, so correctness does not depend on the dealing order;
+// the gain does: neighbouring workgroups, whose stencils share rows, then
+// share an L2.
+__device__ inline int64_t xcd_block() {
+  const int64_t b = blockIdx.x, nb = gridDim.x;
+  const int64_t x = b & 7, q = b >> 3, per = nb >> 3, rem = nb & 7;
+  return x * per + (x < rem ? x : rem) + q;
+}
+
 // Appends idx to the work list q for the lanes with want set: one atomic per
 // wave on the count *n.  Every lane of the wave that is still running calls
 // it.  The order of the entries is unspecified.

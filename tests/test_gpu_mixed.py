@@ -56,17 +56,34 @@ def _phys(S, g, ng):
     return S.P[0][sl], S.P[4][sl], Bm, float(div.abs().max()) / (bmax / g.dx)
 
 
+# Bounds on max(L1(rho), L1(p), L1(|B|)) vs the FP64 form: about 1.6x the
+# values measured on the MI355X with this build (in the comments), so that a
+# change which makes a form less accurate fails.  HLLD's p* is only defined
+# to its residual tolerance, so a change of a root's path (the seeds of the
+# FP32 roots, the HLL-state con2prim bracket) moves these within that
+# freedom: such a change re-measures them.
+ROTOR_BOUND = {"mixed": 6e-5,        # 3.81e-5, 2.79e-5, 7.80e-6
+               "mixed_r64": 1.2e-5,  # 6.69e-6, 6.99e-6, 1.08e-6
+               "fp64_local": 1e-13}  # 3.98e-15
+LIMITER_BOUND = {                    # 96^2, 8 RK3 steps: mixed / mixed_r64 / fp64_local
+    ("mc", "hlld"): (3.5e-5, 3e-5, 1e-13),        # 2.20e-5 / 1.85e-5 / 3.2e-15
+    ("mp5", "hlle"): (1e-7, 1e-7, 1e-13),         # 6.19e-8 / 5.01e-8 / 3.8e-15
+    ("pcm", "hllc"): (2.6e-6, 4e-6, 1e-10),       # 1.64e-6 / 2.40e-6 / 5.2e-11
+    ("minmod", "hlld"): (1.05e-4, 8e-5, 1e-13),   # 6.45e-5 / 4.87e-5 / 3.4e-15
+    ("mp7", "hlld"): (1.5e-5, 2e-5, 1e-13),       # 9.27e-6 / 1.27e-5 / 3.9e-15
+}
+FORMS = ("mixed", "mixed_r64", "fp64_local")
+EMF_HLL_BOUND = {"mixed": 1.8e-4, "mixed_r64": 1.8e-4, "fp64_local": 1e-13}  # 1.15e-4 x2, 4.5e-15
+
+
 @pytest.mark.parametrize("form", ["mixed", "mixed_r64", "fp64_local"])
 def test_mixed_rotor_close_to_fp64_divb_and_determinism(form):
     """Tapered rotor 256^2, WENO5-Z + HLLD, 20 RK3 steps: L1(rho), L1(p),
-    L1(|B|) vs FP64 below 1e-4.  Measured L1(rho) for the mixed form: 7e-6 to
-    3.8e-5 as the seeds of the FP32 roots changed (HLL-state con2prim
-    bracket, mu_+ source); HLLD's p* is only defined to its residual
-    tolerance, so any change of the root's path moves the result within that
-    freedom, and the rotor amplifies it (fp64_local, FP64 throughout with
-    per-element con2prims: 3e-15 here but 2.3e-6 at 2048^2 after 200 steps).
-    div B at round-off, and a rerun bitwise identical (the fix-up list order
-    varies, each face's flux does not)."""
+    L1(|B|) vs FP64 within ROTOR_BOUND (the rotor amplifies root-path
+    differences: fp64_local, FP64 throughout with per-element con2prims, is
+    4e-15 here but 2.3e-6 at 2048^2 after 200 steps).  div B at round-off,
+    and a rerun bitwise identical (the fix-up list order varies, each face's
+    flux does not)."""
     _dev()
     print("mixed rotor 256^2 vs fp64", flush=True)
     A, g, ng = _run(256, "weno5z", "hlld", "fp64", 20)
@@ -76,7 +93,7 @@ def test_mixed_rotor_close_to_fp64_divb_and_determinism(form):
     rb, pb, bb, divb = _phys(B, g, ng)
     l1 = (_l1(ra, rb), _l1(pa, pb), _l1(ba, bb))
     print(f"L1 rho, p, |B| = {l1}", flush=True)
-    assert max(l1) < 1e-4, l1
+    assert max(l1) < ROTOR_BOUND[form], l1
     assert divb < 1e-13
     for u, v in zip(B.U + B.P, C.U + C.P):
         assert torch.equal(u, v)
@@ -95,14 +112,15 @@ def test_mixed_every_limiter_and_solver(limiter, solver, form):
     assert torch.isfinite(rb).all() and torch.isfinite(pb).all()
     l1 = (_l1(ra, rb), _l1(pa, pb), _l1(ba, bb))
     print(f"L1 rho, p, |B| = {l1}", flush=True)
-    assert max(l1) < 1e-4, l1
+    assert max(l1) < LIMITER_BOUND[(limiter, solver)][FORMS.index(form)], l1
     assert divb < 1e-13
 
 
 @pytest.mark.parametrize("form", ["mixed", "mixed_r64", "fp64_local"])
 def test_emf_hll_forms_close_to_fp64(form):
     """emf_mode "hll" in every form: WENO5-Z + HLLD rotor 128^2, 10 steps,
-    L1 vs the FP64 form with the same EMF below 3e-4, div B at round-off.
+    L1 vs the FP64 form with the same EMF within EMF_HLL_BOUND, div B at
+    round-off.
     Measured L1(rho, p, |B|): mixed 4.9e-5, 1.1e-4, 9.9e-6; mixed_r64 (whose
     EMF is FP64) the same to two digits; fp64_local 4.5e-15.  With the
     solver's EMF the same problem gives 8.4e-6, 1.8e-5, 2.3e-6: the
@@ -115,7 +133,7 @@ def test_emf_hll_forms_close_to_fp64(form):
     rb, pb, bb, divb = _phys(B, g, ng)
     l1 = (_l1(ra, rb), _l1(pa, pb), _l1(ba, bb))
     print(f"L1 rho, p, |B| = {l1}", flush=True)
-    assert max(l1) < 3e-4, l1
+    assert max(l1) < EMF_HLL_BOUND[form], l1
     assert divb < 1e-13
 
 
@@ -140,7 +158,9 @@ def test_mixed_orszag_tang_periodic():
         torch.cuda.synchronize()
         out.append(S.P[0][2:66, 2:66].clone())
     assert torch.isfinite(out[1]).all()
-    assert _l1(out[0], out[1]) < 1e-4
+    l1 = _l1(out[0], out[1])
+    print(f"L1 rho = {l1}", flush=True)
+    assert l1 < 4e-7   # measured 2.36e-7
 
 
 @pytest.mark.xfail(strict=True, reason=(
