@@ -30,6 +30,7 @@ import math
 import os
 import socket
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -79,11 +80,52 @@ def symmetry_error(prims, g):
     return worst
 
 
-def _write_meta(out, meta):
-    tmp = os.path.join(out, "run_meta.json.tmp")
-    with open(tmp, "w") as fh:
-        json.dump(meta, fh, indent=1, sort_keys=True)
-    os.replace(tmp, os.path.join(out, "run_meta.json"))
+class RunDir:
+    """The bookkeeping of one run directory: run_meta.json (each update
+    rewrites the whole file through a temporary one, so a reader never sees
+    a partial file), the check that the directory does not already hold a
+    different run, and when snapshots and restart files fall due."""
+
+    # the keys that make two runs different runs (run_2d.py:240-261)
+    IDENTITY = ("limiter", "n", "solver", "problem")
+
+    def __init__(self, out, meta, tend, nsnap, restart_every):
+        self.out, self.meta = out, meta
+        self.path = os.path.join(out, "run_meta.json")
+        # snapshot k is due once t reaches k tend / nsnap (to 1e-14)
+        self.snap_times = [k * tend / nsnap for k in range(1, nsnap + 1)]
+        self.restart_every = restart_every
+
+    def refuse_other_run(self):
+        if not os.path.exists(self.path):
+            return
+        with open(self.path) as fh:
+            held = json.load(fh)
+        diff = [f"{k} {held.get(k)!r} != {self.meta[k]!r}" for k in self.IDENTITY
+                if held.get(k) != self.meta[k]]
+        if diff:
+            raise SystemExit(f"refusing to write into {self.out}: it holds a different run "
+                             f"({', '.join(diff)}).  Pass --out, or --restart to continue "
+                             f"that one.")
+
+    def record(self, **fields):
+        self.meta.update(fields)
+        fd, tmp = tempfile.mkstemp(prefix=".run_meta.", suffix=".json", dir=self.out)
+        with os.fdopen(fd, "w") as fh:
+            json.dump(self.meta, fh, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)
+
+    def record_cost(self, step, steps_here, wall, **fields):
+        self.record(steps=step, steps_here=steps_here, wall_s=round(wall, 1),
+                    s_per_step=round(wall / max(steps_here, 1), 3),
+                    finished=time.strftime("%Y-%m-%dT%H:%M:%S"), **fields)
+
+    def snap_due(self, t, next_snap):
+        """Snapshot number next_snap (1-based) is due at time t."""
+        return next_snap <= len(self.snap_times) and t >= self.snap_times[next_snap - 1] - 1e-14
+
+    def restart_due(self, step, done):
+        return bool(self.restart_every) and not done and step % self.restart_every == 0
 
 
 def _parser():
@@ -127,16 +169,9 @@ def main(argv=None):
                 bc=[bc_x, bc_y], gamma=P["gamma"], host=socket.gethostname(),
                 device=torch.cuda.get_device_name(dev), precision=a.precision,
                 started=time.strftime("%Y-%m-%dT%H:%M:%S"))
-    prev = os.path.join(out, "run_meta.json")
-    if os.path.exists(prev) and not a.restart:
-        old = json.load(open(prev))
-        clash = {k: (old.get(k), meta[k]) for k in ("limiter", "n", "solver", "problem")
-                 if old.get(k) != meta[k]}
-        if clash:
-            raise SystemExit("refusing to write into %s: it holds a different run (%s).  Pass "
-                             "--out, or --restart to continue that one."
-                             % (out, ", ".join("%s %r != %r" % (k, o, n)
-                                               for k, (o, n) in clash.items())))
+    rd = RunDir(out, meta, tend, a.nsnap, a.restart_every)
+    if not a.restart:
+        rd.refuse_other_run()
     eos = E.hybrid_eos(K=0.0, gamma=P["gamma"], gamma_th=P["gamma"])
     lo, hi = P["box"]
     g = G.Grid2D(lo, hi, a.n, lo, hi, a.n, ng=ghosts_needed(a.limiter), device=dev)
@@ -164,7 +199,7 @@ def main(argv=None):
                      torch.from_numpy(R["Byf"].copy()).to(dev))
         st = D2.sync_state(st, g, eos, bc_x, bc_y, limiter=a.limiter)
         gen = int(R["gen"]) + 1
-    _write_meta(out, meta)
+    rd.record()
 
     S = D2.Stepper2D(g, eos, solver=a.solver, limiter=a.limiter, bc_x=bc_x, bc_y=bc_y,
                      upwind=not a.no_upwind_emf, precision=a.precision,
@@ -260,20 +295,17 @@ def main(argv=None):
                   f"{extra}rho=[{float(pr['rho'][ph].min()):.3f},{float(pr['rho'][ph].max()):.3f}]"
                   f" c2p_bad={c2p_bad} ({time.time() - t0:.0f}s)", flush=True)
 
-        if t >= next_snap * tend / a.nsnap - 1e-14 and next_snap <= a.nsnap:
+        if rd.snap_due(t, next_snap):
             save(f"{next_snap:03d}", t)
             next_snap += 1
         done = t >= tend - 1e-14
-        if a.restart_every and step % a.restart_every == 0 and not done:
+        if rd.restart_due(step, done):
             save_restart()
         if a.max_steps is not None and steps_here >= a.max_steps and not done:
             save_restart()
             log.close()
             wall = time.time() - t0
-            meta.update(steps=step, steps_here=steps_here, wall_s=round(wall, 1),
-                        s_per_step=round(wall / max(steps_here, 1), 3), stopped_early=True,
-                        finished=time.strftime("%Y-%m-%dT%H:%M:%S"))
-            _write_meta(out, meta)
+            rd.record_cost(step, steps_here, wall, stopped_early=True)
             print(f"stopped after {steps_here} steps at t={t:.6f} (step {step}); restart file "
                   f"written ({wall:.0f}s)", flush=True)
             return out
@@ -281,10 +313,7 @@ def main(argv=None):
     save("fin", t)
     log.close()
     wall = time.time() - t0
-    meta.update(steps=step, steps_here=steps_here, wall_s=round(wall, 1),
-                s_per_step=round(wall / max(steps_here, 1), 3),
-                finished=time.strftime("%Y-%m-%dT%H:%M:%S"))
-    _write_meta(out, meta)
+    rd.record_cost(step, steps_here, wall)
     print(f"done in {wall:.0f}s, {step} steps -> {out}", flush=True)
     return out
 
