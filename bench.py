@@ -453,19 +453,25 @@ def bench_adjoint(a):
         gradient(dts[:2])
     torch.cuda.synchronize()
     lib = roll.lib
-    _lib.check(lib.srmhd_timing_enable(_lib.KERNEL_CLASSES["adj_sweep_x"], 2 * steps),
-               "srmhd_timing_enable")
     torch.cuda.reset_peak_memory_stats()
     t0 = time.perf_counter()
     L, g_rho, g_A = gradient(dts)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    peak = torch.cuda.max_memory_allocated()
+    print(f"gradient in {wall:.2f} s", file=sys.stderr, flush=True)
+    # the x-sweep adjoint's own launch time, from the gradient of one
+    # checkpoint segment (its records come from the forward, so no re-run
+    # shares the device with the walk, as it does in the run above)
+    seg = min(K, steps)
+    _lib.check(lib.srmhd_timing_enable(_lib.KERNEL_CLASSES["adj_sweep_x"], 2 * seg),
+               "srmhd_timing_enable")
+    gradient(dts[:seg])
+    torch.cuda.synchronize()
     ms = ctypes.c_double()
     cnt = ctypes.c_int64()
     _lib.check(lib.srmhd_timing_read(ctypes.byref(ms), ctypes.byref(cnt)), "srmhd_timing_read")
     lib.srmhd_timing_enable(-1, 0)
-    peak = torch.cuda.max_memory_allocated()
-    print(f"gradient in {wall:.2f} s", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
     h = 1e-6
 

@@ -13,9 +13,12 @@ at the recorded root).
 Memory: the forward keeps a checkpoint (state + primitives) every
 ``checkpoint_every`` steps and runs the last segment with the stage records
 (``srmhd2d_stage_taped``); the backward walks that segment's records, then
-re-runs every earlier segment with records and walks it backwards, so the
-resident set is two segments of stage records plus the checkpoints (at 512^2,
-about 77 MB of records per RK2 step).
+re-runs every earlier segment with records and walks it backwards.  The
+re-run of the next segment (a second stepper, on a side stream) goes on
+while the current one is walked, so the two share the device; the resident
+set is three segments of stage records plus the checkpoints (at 512^2,
+about 77 MB of records per RK2 step).  ``overlap=False`` runs the re-runs
+in line on the caller's stream instead (two segments resident).
 
 ``OTRollout`` is the differentiable relativistic Orszag-Tang problem: the
 initial density field (physical cells) and the magnetic amplitude A are the
@@ -43,12 +46,18 @@ class Rollout2D:
 
     def __init__(self, grid, eos, *, solver="hlld", limiter="mc", bc_x="outflow",
                  bc_y="outflow", upwind=True, scheme="rk2", checkpoint_every=20,
-                 atmo_rho=1e-10, atmo_eps=1e-10, device=None):
+                 atmo_rho=1e-10, atmo_eps=1e-10, device=None, overlap=True):
         # the adjoint differentiates the reference's default p* root, so the
         # rollout keeps the default options whatever srmhd.hlld's switches say
-        self.st = Stepper2D(grid, eos, solver=solver, limiter=limiter, bc_x=bc_x, bc_y=bc_y,
-                            upwind=upwind, atmo_rho=atmo_rho, atmo_eps=atmo_eps, device=device,
-                            precision="fp64", hlld_opts_from_globals=False)
+        kw = dict(solver=solver, limiter=limiter, bc_x=bc_x, bc_y=bc_y, upwind=upwind,
+                  atmo_rho=atmo_rho, atmo_eps=atmo_eps, device=device, precision="fp64",
+                  hlld_opts_from_globals=False)
+        self.st = Stepper2D(grid, eos, **kw)
+        # the backward re-runs the next segment while it walks the current
+        # one: that re-run has its own stepper (workspace, dt) and stream
+        self.st_rerun = Stepper2D(grid, eos, **kw)
+        self.overlap = bool(overlap)
+        self.side = torch.cuda.Stream(device=self.st.device) if self.overlap else None
         self.grid = grid
         self.device = self.st.device
         self.lib = self.st.lib
@@ -105,10 +114,11 @@ class Rollout2D:
         return bufs, _lib.Tape2D(*[b.data_ptr() for b in bufs])
 
     # ── forward ─────────────────────────────────────────────────────────────
-    def _step(self, U, P, dt, record=None):
+    def _step(self, U, P, dt, record=None, st=None):
         """One RK step from (U, P) -> new (U, P); with ``record`` (a list) the
-        stage inputs and tapes are appended to it."""
-        st = self.st
+        stage inputs and tapes are appended to it.  ``st``: the stepper (the
+        rollout's own by default)."""
+        st = self.st if st is None else st
         st.set_dt(dt)
         U0 = U
         for a, b, c in RK[self.scheme]:
@@ -175,17 +185,25 @@ class Rollout2D:
         lamU = [t.contiguous() for t in lamU]
         lamP = [t.contiguous() for t in lamP]
         S = len(stages)
-        for s0 in reversed(range(0, N, self.K)):
+        main = torch.cuda.current_stream(self.device)
+        side = self.side
+        if side is not None:
+            side.wait_stream(main)   # the checkpoints and the tail are written
+        starts = list(reversed(range(0, N, self.K)))
+        pending = None
+        for idx, s0 in enumerate(starts):
             s1 = min(s0 + self.K, N)
             if s0 == self.tail_start:
                 recs = self.tail     # recorded by forward()
+            elif pending is not None:
+                recs, done = pending   # re-run on the side stream
+                main.wait_event(done)
             else:
-                U, P = self.ckpt[s0]
-                recs = []
-                for n in range(s0, s1):
-                    rec = []
-                    U, P = self._step(U, P, self.dts[n], record=rec)
-                    recs.append(rec)
+                recs = self._rerun(s0)
+            # the next segment's re-run goes on beside this segment's walk
+            pending = None
+            if side is not None and idx + 1 < len(starts):
+                pending = self._rerun_side(starts[idx + 1], main)
             for n in reversed(range(s0, s1)):
                 self.st.set_dt(self.dts[n])
                 rec = recs[n - s0]
@@ -202,7 +220,35 @@ class Rollout2D:
                     lamU, lamP = newU, newP
                 lamU = acc
             del recs
+        if side is not None:
+            main.wait_stream(side)
         return lamU, lamP
+
+    def _rerun(self, s0, st=None):
+        """The stage records of the segment starting at step s0, re-run from
+        its checkpoint."""
+        U, P = self.ckpt[s0]
+        recs = []
+        for n in range(s0, min(s0 + self.K, len(self.dts))):
+            rec = []
+            U, P = self._step(U, P, self.dts[n], record=rec, st=st)
+            recs.append(rec)
+        return recs
+
+    def _rerun_side(self, s0, main):
+        """_rerun on the side stream with the second stepper; returns the
+        records and the event that marks them written.  Every record tensor
+        is marked as used on the main stream, so that its memory is not
+        handed out again before the walk that reads it has run."""
+        with torch.cuda.stream(self.side):
+            recs = self._rerun(s0, st=self.st_rerun)
+            done = torch.cuda.Event()
+            done.record(self.side)
+        for rec in recs:
+            for Uk, Pk, Uout, bufs, _tape in rec:
+                for t in (*Uk, *Pk, *Uout, *bufs):
+                    t.record_stream(main)
+        return recs, done
 
     def sync_adjoint(self, U, lamU, lamP):
         """Adjoint of sync_state: -> the adjoint of the state before the
