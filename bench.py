@@ -251,9 +251,10 @@ def bench_tubes(a):
         "per_tube_loop": {"value": loop_zc, "unit": "zone-cycles/s", "tubes_timed": n_loop,
                           "speedup_of_batch": zc / loop_zc},
         "autotune_ms": tune,
-        # MP5 + HLLD blows up on a few of the random Riemann problems; the CPU
-        # oracle's per-tube loop does the same on them (e.g. tubes 1794 and
-        # 1873 of seed 0 are non-finite after 100 steps on both)
+        # MP5 + HLLD blows up on 11 of the random Riemann problems of seed 0
+        # (500 steps); each blows up at the same step on the oracle's
+        # per-tube step, continued bit for bit from the device state
+        # (tests/test_gpu_tubes.py)
         "nonfinite_tubes": {"count": len(nonfinite), "first": nonfinite[:16]},
     }
     if not a.no_cpu:

@@ -195,3 +195,74 @@ def test_shared_dt_other_solvers_bitwise_vs_oracle(golden_dir, lim, flux):
     for k in PRIM9:
         s = _same(prof[k].cpu().numpy(), po[k].numpy())
         assert s.all(), f"{k}: {int((~s).sum())} differ from the oracle"
+
+
+def _oracle_continue(U, P, Bn, ncells, ng, eos, limiter, solver, scheme):
+    """One step of the oracle's per-tube loop (oracle/tubes.py:run_tube) from
+    a given conserved state U (D, Sx, Sy, Sz, tau, By, Bz) and its
+    primitives P (rho, vx, vy, vz, p, eps, Bx, By, Bz), ghosts included."""
+    import oracle.tubes as OT
+    dx = 1.0 / ncells
+    cons = dict(U)
+    Bx = P["Bx"]
+    dt = float(OT.compute_dt(P, eos, dx, 0.4, ng, ncells, speed_of_light=False))
+    U0, Uk = cons, cons
+    for a, b, c in OT.RK[scheme]:
+        L = OT._rhs(P, eos, dx, limiter, solver)
+        w = c * dt
+        Uk = {k: a * U0[k] + b * Uk[k] + w * L[k] for k in U0}
+        P = OT._sync(Uk, Bx, eos, ng)
+    return Uk, P
+
+
+def test_nonfinite_tubes_blow_up_on_the_oracle_too():
+    """The tubes of the benchmark workload (4096 tubes of seed 0, MP5 + HLLD +
+    SSP-RK3, per-tube dt, 500 steps) that end non-finite: each is run alone
+    (the per-tube dt makes it independent of the batch), one step at a time,
+    to its first non-finite step S; from its device state three steps
+    before S, the oracle's per-tube step continues in lockstep, equal bit for
+    bit after steps S - 2 and S - 1 and non-finite after step S too.  The
+    blow-ups are the algorithm's, not the device path's."""
+    _dev()
+    import oracle.eos as OE
+    from srmhd import tubes as T
+    lengths, left, right, Bn = _workload(4096, seed=0, lmin=128, lmax=2048)
+    b = _batch(lengths, left, right, Bn, "mp5", "hlld")
+    b.run(500, cfl=0.4, block=0)   # the phased form (bit-identical, faster here)
+    torch.cuda.synchronize()
+    fin = torch.ones(b.total_cells, dtype=torch.bool, device=b.U[0].device)
+    for p in b.P:
+        fin &= torch.isfinite(p)
+    off = b.off_host
+    bad = [i for i in range(b.n_tubes) if not bool(fin[int(off[i]):int(off[i + 1])].all())]
+    print(f"non-finite tubes after 500 steps: {bad}", flush=True)
+    assert 0 < len(bad) <= 64
+    eos = OE.ideal(5.0 / 3.0)
+    for i in bad:
+        one = _batch(lengths[i:i + 1], {k: v[i:i + 1] for k, v in left.items()},
+                     {k: v[i:i + 1] for k, v in right.items()}, Bn[i:i + 1], "mp5", "hlld")
+        ng, L = one.ng, int(lengths[i])
+        hist = []   # (U, P) on the host after the last few steps
+        S = None
+        for s in range(1, 501):
+            one.run(1, cfl=0.4)
+            U = {k: u.cpu() for k, u in zip(T.U_KEYS, one.U)}
+            tb = one.tube(0)
+            P = {k: v.cpu().clone() for k, v in tb.items()}
+            if not all(bool(torch.isfinite(P[k]).all()) for k in T.P_KEYS):
+                S = s
+                break
+            hist = (hist + [(U, P)])[-3:]
+        assert S is not None and S >= 4, (i, S)
+        Uo, Po = hist[0]                      # after step S - 3
+        for j, s in enumerate((S - 2, S - 1, S)):
+            Uo, Po = _oracle_continue(Uo, Po, float(Bn[i]), L, ng, eos, "mp5", "hlld", "rk3")
+            if s < S:
+                Ug, Pg = hist[j + 1]
+                for k in T.P_KEYS:
+                    same = _same(Pg[k].numpy(), Po[k].numpy())
+                    assert same.all(), f"tube {i} step {s} {k}: {int((~same).sum())} differ"
+            else:
+                assert not all(bool(torch.isfinite(Po[k]).all()) for k in T.P_KEYS), \
+                    f"tube {i}: the oracle stays finite at step {S}"
+        print(f"tube {i} ({L} cells): non-finite from step {S} on both", flush=True)
