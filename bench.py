@@ -35,6 +35,9 @@ for _p in (ROOT, os.path.join(ROOT, "hlld-ai_amd")):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBS = 8000.0
+# steps (from the initial state, sweeps in sequence) over which the rotor
+# forms' dominant kernel class is timed with HIP events
+KERNEL_TIMING_STEPS = 20
 FP64_PEAK_TFLOPS = 78.6
 
 # FP64 VALU work per element of the dominant kernels, from rocprofv3 --pmc
@@ -283,22 +286,32 @@ def _rotor_form(a, precision, kernel, U_init, P_init):
     step.load([t.clone() for t in U_init], [t.clone() for t in P_init])
     for _ in range(a.warmup):
         one_step()
+    # the kernel class's own launch time: KERNEL_TIMING_STEPS steps from the
+    # initial state with the sweeps one after the other (in the timed run
+    # the FP64 form's y sweep shares the device with its x sweep)
+    lib = step.lib
+    step.load([t.clone() for t in U_init], [t.clone() for t in P_init])
+    step.cs.flags = _lib.STAGE_SERIAL_SWEEPS
+    ktime_steps = min(KERNEL_TIMING_STEPS, a.steps)
+    _lib.check(lib.srmhd_timing_enable(_lib.KERNEL_CLASSES[kernel], 3 * ktime_steps),
+               "srmhd_timing_enable")
+    for _ in range(ktime_steps):
+        one_step()
+    torch.cuda.synchronize()
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    _lib.check(lib.srmhd_timing_read(ctypes.byref(ms), ctypes.byref(cnt)), "srmhd_timing_read")
+    lib.srmhd_timing_enable(-1, 0)
+    step.cs.flags = 0
     step.load([t.clone() for t in U_init], [t.clone() for t in P_init])
     torch.cuda.synchronize()
-    print(f"{precision}: warmup done ({a.warmup} steps)", file=sys.stderr, flush=True)
-    lib = step.lib
-    _lib.check(lib.srmhd_timing_enable(_lib.KERNEL_CLASSES[kernel], 3 * a.steps),
-               "srmhd_timing_enable")
-    torch.cuda.synchronize()
+    print(f"{precision}: warmup done ({a.warmup} steps, {ktime_steps} kernel-timing steps)",
+          file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(a.steps):
         one_step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = ctypes.c_double()
-    cnt = ctypes.c_int64()
-    _lib.check(lib.srmhd_timing_read(ctypes.byref(ms), ctypes.byref(cnt)), "srmhd_timing_read")
-    lib.srmhd_timing_enable(-1, 0)
     print(f"{precision}: {a.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
     avg_ms = ms.value / max(cnt.value, 1)
     n, ng = a.n, a.ng

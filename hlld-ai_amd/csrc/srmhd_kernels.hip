@@ -752,42 +752,93 @@ static_assert(N_WORDS <= DIAG_ZERO_BLOCKS * 256, "the stage's zeroing grid cover
 // workgroups of k_sweep_a_rescue (grid-stride over the faces)
 constexpr int RESCUE_BLOCKS = 1024;
 
-// the two sweeps of ct_stage for one (limiter, solver)
+// A second stream per device for work that can run beside the sweeps (the
+// FP64 form's y sweep beside its x sweep; the mixed form's FP64 fix-up of the
+// x faces, a few latency-bound waves, beside the y sweep), forked and joined
+// with events.  Created at first use, at the
+// default priority.
+struct AuxStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  bool failed = false;   // creation failed once: this device stays on one stream
+};
+constexpr int AUX_MAX_DEVICES = 64;
+AuxStream g_aux[AUX_MAX_DEVICES];
+std::mutex g_aux_mu;   // held while a stage enqueues its fork/join
+
+// The current device's auxiliary stream, created at first use (nullptr when
+// unavailable: the caller then stays on its own stream).  Caller holds
+// g_aux_mu.
+AuxStream* aux_stream() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= AUX_MAX_DEVICES) return nullptr;
+  AuxStream& a = g_aux[dev];
+  if (!a.s && !a.failed) {
+    hipStream_t st = nullptr;
+    hipEvent_t f = nullptr, j = nullptr;
+    const bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+                    hipEventCreateWithFlags(&f, hipEventDisableTiming) == hipSuccess &&
+                    hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      if (j) (void)hipEventDestroy(j);
+      if (f) (void)hipEventDestroy(f);
+      if (st) (void)hipStreamDestroy(st);
+      (void)hipGetLastError();   // the failure is handled here, not by the next launch check
+      a.failed = true;
+      return nullptr;
+    }
+    a.fork = f;
+    a.join = j;
+    a.s = st;
+  }
+  return a.s ? &a : nullptr;
+}
+
+
+// the sweep along N of ct_stage for one (limiter, solver), on stream s.
+// HLLD: pass A with the inert faces deferred, the batch mask folded, the
+// deferred faces' pass A only if that mask is not full, then pass B.  Each
+// direction has its own shards, records, masks and fluxes.
+template <int N, int LIM, int SOL>
+void sweep_dir(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const Ws2& w,
+               hipStream_t s) {
+  const int nb = blocks_for(N == 0 ? (int64_t)(g.nxt + 1) * g.nyt : (int64_t)g.nxt * (g.nyt + 1));
+  srmhd_flux_diag* sh = (srmhd_flux_diag*)((char*)w.dsh + N * DIAG_SHARDS * DIAG_SHARD_BYTES);
+  C2PRec* rec = N == 0 ? w.recx : w.recy;
+  const bool hd = SOL == SOLVER_HLLD;
+  unsigned long long* mres = hd ? w.words + 8 + N : nullptr;
+  if (hd) {
+    tbegin(N == 0 ? SRMHD_K_SWEEP_X_A : SRMHD_K_SWEEP_Y_A, s);
+    k_sweep_a<N, LIM, true><<<nb, BLK, 0, s>>>(g, S, P, e, rec, sh);
+    tend(N == 0 ? SRMHD_K_SWEEP_X_A : SRMHD_K_SWEEP_Y_A, s);
+    k_fold_diag<<<1, 64, 0, s>>>(sh, w.diag + N);   // the batch mask for pass B
+    k_sweep_a_rescue<N, LIM><<<nb < RESCUE_BLOCKS ? nb : RESCUE_BLOCKS, BLK, 0, s>>>(
+        g, S, P, e, rec, w.diag + N, mres);
+  }
+  tbegin(N == 0 ? SRMHD_K_SWEEP_X_B : SRMHD_K_SWEEP_Y_B, s);
+  k_sweep_b<N, LIM, SOL><<<nb, BLK, 0, s>>>(g, S, P, e, rec, w.diag + N, sh,
+                                             N == 0 ? w.Fx : w.Fy, N == 0 ? w.psx : w.psy,
+                                             w.emf_hll, w.o.tol, mres);
+  tend(N == 0 ? SRMHD_K_SWEEP_X_B : SRMHD_K_SWEEP_Y_B, s);
+}
+
+// the two sweeps of ct_stage for one (limiter, solver).  ax (non-NULL: the
+// caller holds g_aux_mu): the y sweep runs on the auxiliary stream beside
+// the x sweep, joined before the fold of both; the results are the same
+// bit for bit either way.
 template <int LIM, int SOL>
 void sweeps(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, const Ws2& w,
-            hipStream_t s) {
-  const int bx = blocks_for((int64_t)(g.nxt + 1) * g.nyt);
-  const int by = blocks_for((int64_t)g.nxt * (g.nyt + 1));
-  srmhd_flux_diag* shx = w.dsh;
-  srmhd_flux_diag* shy = (srmhd_flux_diag*)((char*)w.dsh + DIAG_SHARDS * DIAG_SHARD_BYTES);
-  // HLLD: pass A with the inert faces deferred, the batch mask folded, the
-  // deferred faces' pass A only if that mask is not full, then pass B
-  const bool hd = SOL == SOLVER_HLLD;
-  unsigned long long* mres = hd ? w.words + 8 : nullptr;
-  if (hd) {
-    tbegin(SRMHD_K_SWEEP_X_A, s);
-    k_sweep_a<0, LIM, true><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, shx);
-    tend(SRMHD_K_SWEEP_X_A, s);
-    k_fold_diag<<<1, 64, 0, s>>>(shx, w.diag + 0);   // the batch mask for pass B
-    k_sweep_a_rescue<0, LIM><<<bx < RESCUE_BLOCKS ? bx : RESCUE_BLOCKS, BLK, 0, s>>>(
-        g, S, P, e, w.recx, w.diag + 0, mres);
+            hipStream_t s, AuxStream* ax = nullptr) {
+  const bool forked = ax && hipEventRecord(ax->fork, s) == hipSuccess &&
+                      hipStreamWaitEvent(ax->s, ax->fork, 0) == hipSuccess;
+  sweep_dir<0, LIM, SOL>(g, S, P, e, w, s);
+  sweep_dir<1, LIM, SOL>(g, S, P, e, w, forked ? ax->s : s);
+  if (forked) {
+    if (hipEventRecord(ax->join, ax->s) == hipSuccess)
+      (void)hipStreamWaitEvent(s, ax->join, 0);
+    else
+      (void)hipStreamSynchronize(ax->s);   // no join event: wait on the host
   }
-  tbegin(SRMHD_K_SWEEP_X_B, s);
-  k_sweep_b<0, LIM, SOL><<<bx, BLK, 0, s>>>(g, S, P, e, w.recx, w.diag + 0, shx, w.Fx, w.psx,
-                                             w.emf_hll, w.o.tol, mres);
-  tend(SRMHD_K_SWEEP_X_B, s);
-  if (hd) {
-    tbegin(SRMHD_K_SWEEP_Y_A, s);
-    k_sweep_a<1, LIM, true><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, shy);
-    tend(SRMHD_K_SWEEP_Y_A, s);
-    k_fold_diag<<<1, 64, 0, s>>>(shy, w.diag + 1);
-    k_sweep_a_rescue<1, LIM><<<by < RESCUE_BLOCKS ? by : RESCUE_BLOCKS, BLK, 0, s>>>(
-        g, S, P, e, w.recy, w.diag + 1, mres + 1);
-  }
-  tbegin(SRMHD_K_SWEEP_Y_B, s);
-  k_sweep_b<1, LIM, SOL><<<by, BLK, 0, s>>>(g, S, P, e, w.recy, w.diag + 1, shy, w.Fy, w.psy,
-                                             w.emf_hll, w.o.tol, hd ? mres + 1 : nullptr);
-  tend(SRMHD_K_SWEEP_Y_B, s);
   k_fold_diag<<<1, 128, 0, s>>>(w.dsh, w.diag);
 }
 
@@ -841,17 +892,17 @@ void sweeps_local(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, c
 
 template <int LIM>
 void sweeps_solver(int sol, bool local, const G2& g, const St2& S, const Pr2& P,
-                   const srmhd_eos& e, const Ws2& w, hipStream_t s) {
+                   const srmhd_eos& e, const Ws2& w, hipStream_t s, AuxStream* ax) {
   switch (sol) {
-    case SOLVER_HLLE: sweeps<LIM, SOLVER_HLLE>(g, S, P, e, w, s); break;
-    case SOLVER_HLLC: sweeps<LIM, SOLVER_HLLC>(g, S, P, e, w, s); break;
+    case SOLVER_HLLE: sweeps<LIM, SOLVER_HLLE>(g, S, P, e, w, s, ax); break;
+    case SOLVER_HLLC: sweeps<LIM, SOLVER_HLLC>(g, S, P, e, w, s, ax); break;
     default:
       if (local)
         sweeps_local<LIM>(g, S, P, e, w, s);
       else if (!hlld_opts_fused(w.o))
         sweeps_opts<LIM>(g, S, P, e, w, s);
       else
-        sweeps<LIM, SOLVER_HLLD>(g, S, P, e, w, s);
+        sweeps<LIM, SOLVER_HLLD>(g, S, P, e, w, s, ax);
   }
 }
 
@@ -875,47 +926,6 @@ void fixup_dispatch(int lim, const G2& g, const St2& S, const Pr2& P, const srmh
     case LIM_MP5: fixup_lim<N, LIM_MP5>(g, S, P, e, fix, F, emf_hll, s); break;
     default: fixup_lim<N, LIM_MP7>(g, S, P, e, fix, F, emf_hll, s);
   }
-}
-
-// A second stream per device for work that can run beside the sweeps (the
-// mixed form's FP64 fix-up of the x faces, a few latency-bound waves, beside
-// the y sweep), forked and joined with events.  Created at first use, at the
-// default priority.
-struct AuxStream {
-  hipStream_t s = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-  bool failed = false;   // creation failed once: this device stays on one stream
-};
-constexpr int AUX_MAX_DEVICES = 64;
-AuxStream g_aux[AUX_MAX_DEVICES];
-std::mutex g_aux_mu;   // held while a stage enqueues its fork/join
-
-// The current device's auxiliary stream, created at first use (nullptr when
-// unavailable: the caller then stays on its own stream).  Caller holds
-// g_aux_mu.
-AuxStream* aux_stream() {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= AUX_MAX_DEVICES) return nullptr;
-  AuxStream& a = g_aux[dev];
-  if (!a.s && !a.failed) {
-    hipStream_t st = nullptr;
-    hipEvent_t f = nullptr, j = nullptr;
-    const bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
-                    hipEventCreateWithFlags(&f, hipEventDisableTiming) == hipSuccess &&
-                    hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
-    if (!ok) {
-      if (j) (void)hipEventDestroy(j);
-      if (f) (void)hipEventDestroy(f);
-      if (st) (void)hipStreamDestroy(st);
-      (void)hipGetLastError();   // the failure is handled here, not by the next launch check
-      a.failed = true;
-      return nullptr;
-    }
-    a.fork = f;
-    a.join = j;
-    a.s = st;
-  }
-  return a.s ? &a : nullptr;
 }
 
 int sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const Pr2& P,
@@ -956,14 +966,14 @@ int sweeps_mixed(int lim, int sol, bool r64, const G2& g, const St2& S, const Pr
 }
 
 void sweeps_dispatch(int lim, int sol, bool local, const G2& g, const St2& S, const Pr2& P,
-                     const srmhd_eos& e, const Ws2& w, hipStream_t s) {
+                     const srmhd_eos& e, const Ws2& w, hipStream_t s, AuxStream* ax = nullptr) {
   switch (lim) {
-    case LIM_PCM: sweeps_solver<LIM_PCM>(sol, local, g, S, P, e, w, s); break;
-    case LIM_MC: sweeps_solver<LIM_MC>(sol, local, g, S, P, e, w, s); break;
-    case LIM_MINMOD: sweeps_solver<LIM_MINMOD>(sol, local, g, S, P, e, w, s); break;
-    case LIM_WENO5Z: sweeps_solver<LIM_WENO5Z>(sol, local, g, S, P, e, w, s); break;
-    case LIM_MP5: sweeps_solver<LIM_MP5>(sol, local, g, S, P, e, w, s); break;
-    default: sweeps_solver<LIM_MP7>(sol, local, g, S, P, e, w, s);
+    case LIM_PCM: sweeps_solver<LIM_PCM>(sol, local, g, S, P, e, w, s, ax); break;
+    case LIM_MC: sweeps_solver<LIM_MC>(sol, local, g, S, P, e, w, s, ax); break;
+    case LIM_MINMOD: sweeps_solver<LIM_MINMOD>(sol, local, g, S, P, e, w, s, ax); break;
+    case LIM_WENO5Z: sweeps_solver<LIM_WENO5Z>(sol, local, g, S, P, e, w, s, ax); break;
+    case LIM_MP5: sweeps_solver<LIM_MP5>(sol, local, g, S, P, e, w, s, ax); break;
+    default: sweeps_solver<LIM_MP7>(sol, local, g, S, P, e, w, s, ax);
   }
 }
 
@@ -1385,6 +1395,7 @@ static bool hlld_default(const HlldOpts& o) {
 
 static bool scheme_ok(const srmhd_scheme2d* s) {
   if (!(s && s->solver >= 0 && s->solver <= 3 && s->limiter >= 0 && s->limiter <= 5 &&
+        (s->flags & ~SRMHD_STAGE_SERIAL_SWEEPS) == 0 &&
         (s->precision == SRMHD_PREC_FP64 || s->precision == SRMHD_PREC_MIXED ||
          s->precision == SRMHD_PREC_MIXED_R64 || s->precision == SRMHD_PREC_FP64_LOCAL) &&
         (s->emf_mode == SRMHD_EMF_SOLVER || s->emf_mode == SRMHD_EMF_HLL)))
@@ -1474,8 +1485,15 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
     const int rc = launch_linear_sweeps(sp->limiter, g, s1, pr, *eos, LinOpts(), w.Fx, w.Fy,
                                         w.emf_hll, w.diag, w.words + 4, st);
     if (rc) return rc;
-  } else {
+  } else if (local || (sp->flags & SRMHD_STAGE_SERIAL_SWEEPS)) {
     sweeps_dispatch(sp->limiter, sp->solver, local, g, s1, pr, *eos, w, st);
+  } else {
+    // the y sweep beside the x sweep on the device's auxiliary stream; a
+    // launch error so far is reported first (creating that stream may clear
+    // the error state when it fails)
+    if (const int rc = launch_status()) return rc;
+    std::lock_guard<std::mutex> lock(g_aux_mu);
+    sweeps_dispatch(sp->limiter, sp->solver, local, g, s1, pr, *eos, w, st, aux_stream());
   }
   if (tape) {
     const int64_t nfx = (int64_t)(g.nxt + 1) * g.nyt, nfy = (int64_t)g.nxt * (g.nyt + 1);

@@ -64,10 +64,15 @@ PSTAR_METHOD_IDS = {"secant": 0, "newton": 1, "newton_fd": 2, "newton_fd1": 3, "
 PSTAR_BRACKET_IDS = {"legacy": 0, "hllc": 1}
 
 
+# srmhd_scheme2d.flags: the FP64 form's y sweep after its x sweep, on the
+# caller's stream (default: beside it, on the device's second stream)
+STAGE_SERIAL_SWEEPS = 1
+
+
 class Scheme2D(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int32), ("limiter", ctypes.c_int32), ("upwind", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("emf_mode", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("atmo_rho", ctypes.c_double),
+                ("flags", ctypes.c_int32), ("atmo_rho", ctypes.c_double),
                 ("atmo_eps", ctypes.c_double), ("hlld", HlldOpts)]
 
 

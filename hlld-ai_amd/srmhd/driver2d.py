@@ -35,7 +35,8 @@ class Stepper2D:
 
     def __init__(self, grid, eos, *, solver="hlld", limiter="mc", bc_x="outflow",
                  bc_y="outflow", upwind=True, atmo_rho=1e-10, atmo_eps=1e-10, device=None,
-                 precision="fp64", emf_mode="solver", hlld_opts_from_globals=True):
+                 precision="fp64", emf_mode="solver", hlld_opts_from_globals=True,
+                 serial_sweeps=False):
         self.grid = grid
         self.device = torch.device(device) if device is not None else grid.device
         if self.device.type != "cuda":
@@ -57,8 +58,9 @@ class Stepper2D:
         self.emf_mode = emf_mode
         self.cs = _lib.Scheme2D(_lib.SOLVER_IDS[self.solver], _lib.LIMITER_IDS[limiter],
                                 1 if upwind else 0, _lib.PRECISION_IDS[precision],
-                                _lib.EMF_MODE_IDS[emf_mode], 0, float(atmo_rho),
-                                float(atmo_eps))
+                                _lib.EMF_MODE_IDS[emf_mode],
+                                _lib.STAGE_SERIAL_SWEEPS if serial_sweeps else 0,
+                                float(atmo_rho), float(atmo_eps))
         # HLLD's p* switches: srmhd.hlld's module globals, read at every stage
         # like the reference's hlld_flux does (False: the default root, fixed)
         self.hlld_opts_from_globals = bool(hlld_opts_from_globals)

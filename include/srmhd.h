@@ -228,9 +228,15 @@ enum { SRMHD_EMF_SOLVER = 0, SRMHD_EMF_HLL = 1 };
 /* hlld: the HLLD p* options (zeroed: the reference's default); non-default
  * methods and the hllc bracket are taken by the FP64 precision only (the
  * other forms and the adjoint entry points refuse them, SRMHD_EARG). */
+/* flags: SRMHD_STAGE_SERIAL_SWEEPS runs the FP64 form's y sweep after its x
+ * sweep on the caller's stream; without it the y sweep runs beside the x
+ * sweep on a second stream of the device (created at first use, default
+ * priority; forked from and joined back into the caller's stream with
+ * events).  The results are the same bit for bit. */
+enum { SRMHD_STAGE_SERIAL_SWEEPS = 1 };
 typedef struct srmhd_scheme2d {
   int32_t solver, limiter, upwind, precision;
-  int32_t emf_mode, reserved;   /* reserved: 0 */
+  int32_t emf_mode, flags;   /* flags: SRMHD_STAGE_* bits, 0 by default */
   double atmo_rho, atmo_eps;
   srmhd_hlld_opts hlld;
 } srmhd_scheme2d;
