@@ -7,13 +7,14 @@
  * whose size the matching *_workspace_bytes() returns (srmhd_flux_noalloc
  * takes none: it uses a fixed device array of the library's); no device
  * memory is allocated inside.  One exception holds host-side HIP objects: the first
- * mixed-precision srmhd2d_stage (SRMHD_PREC_MIXED / _MIXED_R64, HLLD) on a
- * device creates one non-blocking stream and two events for that device
- * (default priority), kept until the process ends, to run the FP64 fix-up
- * of the x faces beside the y sweep; if their creation fails the stage runs
- * on the caller's stream alone.  All calls are asynchronous on the stream
- * (the mixed stage's second stream is forked from and joined back into it
- * with events).  Return value: 0 on
+ * srmhd2d_stage on a device that uses a second stream (the FP64 form, whose
+ * y sweep runs beside its x sweep unless SRMHD_STAGE_SERIAL_SWEEPS is set;
+ * the mixed-precision forms with HLLD, whose FP64 fix-up of the x faces runs
+ * beside the y sweep) creates one non-blocking stream and two events for
+ * that device (default priority), kept until the process ends; if their
+ * creation fails the stage runs on the caller's stream alone.  All calls are
+ * asynchronous on the stream (the second stream is forked from and joined
+ * back into it with events).  Return value: 0 on
  * success, SRMHD_EARG for a bad argument, otherwise SRMHD_EHIP + the
  * hipError_t of the failing launch.  Calls that share a workspace must not
  * overlap.
