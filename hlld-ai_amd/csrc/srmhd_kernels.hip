@@ -179,14 +179,59 @@ __global__ __launch_bounds__(BLK) void k_p2c(int64_t n, PrimPtrs P, ConsPtrs C) 
 }
 
 
+// The face states of a sweep as pass A forms them (face_prims2d), kept for
+// pass B: FQ_N arrays of the sweep's faces -- rho, v (3), eps, B_t1, B_t2 of
+// the left state, then of the right one.  B_n is the face field; the
+// pressure is not kept: the solvers form it again from eps (side_state) and
+// read no other.
+constexpr int FQ_N = 14;
+template <int N>
+__device__ inline void face_q_store(double* fq, int64_t nf, int64_t t, const Prim& l,
+                                    const Prim& r) {
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const Prim& q = s ? r : l;
+    double* o = fq + (int64_t)(7 * s) * nf + t;
+    o[0] = q.rho;
+    o[nf] = q.v[0];
+    o[2 * nf] = q.v[1];
+    o[3 * nf] = q.v[2];
+    o[4 * nf] = q.eps;
+    o[5 * nf] = q.B[T1];
+    o[6 * nf] = q.B[T2];
+  }
+}
+template <int N>
+__device__ inline void face_q_load(const G2& g, const St2& S, const double* fq, int64_t nf,
+                                   int64_t t, int i, int j, Prim& l, Prim& r) {
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+  const double Bn = N == 0 ? S.p[6][(int64_t)i * g.nyt + j] : S.p[7][(int64_t)i * (g.nyt + 1) + j];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    Prim& q = s ? r : l;
+    const double* o = fq + (int64_t)(7 * s) * nf + t;
+    q.rho = o[0];
+    q.v[0] = o[nf];
+    q.v[1] = o[2 * nf];
+    q.v[2] = o[3 * nf];
+    q.p = 0.0;   // not read by the solvers (see above)
+    q.eps = o[4 * nf];
+    q.B[N] = Bn;
+    q.B[T1] = o[5 * nf];
+    q.B[T2] = o[6 * nf];
+  }
+}
+
 // Pass A of the sweep's HLL-state con2prims.  DEFER: faces whose con2prim
 // is inert (hll_c2p_inert: B_n = 0 on both sides, so the flux is the HLLE
 // one) are left out; their result matters only when the other faces'
 // masks leave the batch's break iteration below 60, and k_sweep_a_rescue
 // then runs them.  On the rotor ~95 % of the y faces are such faces.
+// fq (non-NULL): the face states are kept there for pass B (face_q_store).
 template <int N, int LIM, bool DEFER = false>
 __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e, C2PRec* rec,
-                                                 srmhd_flux_diag* dsh) {
+                                                 srmhd_flux_diag* dsh, double* fq = nullptr) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   uint64_t m = 0;
   if (t < n_faces<N>(g)) {
@@ -194,6 +239,7 @@ __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e
     face_ij<N>(g, t, i, j);
     Prim l, r;
     face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
+    if (fq) face_q_store<N>(fq, n_faces<N>(g), t, l, r);
     HlldPrep<N> h;
     h.run(l, r, e);
     if (!(DEFER && hll_c2p_inert<N>(h))) {
@@ -241,12 +287,15 @@ __global__ __launch_bounds__(BLK) void k_sweep_a_rescue(G2 g, St2 S, Pr2 P, srmh
 // no pass-A record and skip their con2prim: p_hll then reaches only the
 // fallback sentinel of p* (written as -1 - 1e-10 there: negative, as every
 // fallback's), and the rest of the solver is unchanged.
-template <int N, int LIM, int SOL, bool LOCAL = false>
+// FQ: the face states come from pass A's copy fq (face_q_load) instead of
+// the reconstruction.
+template <int N, int LIM, int SOL, bool LOCAL = false, bool FQ = false>
 __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e,
                                                  const C2PRec* rec, const srmhd_flux_diag* d,
                                                  srmhd_flux_diag* dsh, Flx F, double* pstar_out,
                                                  int emf_hll, double tol,
-                                                 const unsigned long long* mres = nullptr) {
+                                                 const unsigned long long* mres = nullptr,
+                                                 const double* fq = nullptr) {
   static_assert(!LOCAL || SOL == SOLVER_HLLD, "the local con2prim concerns HLLD only");
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = t < n_faces<N>(g);
@@ -256,7 +305,10 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
     int i, j;
     face_ij<N>(g, t, i, j);
     Prim l, r;
-    face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
+    if (FQ)
+      face_q_load<N>(g, S, fq, n_faces<N>(g), t, i, j, l, r);
+    else
+      face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
     // the HLL face EMF first, while only the face states are live
     const double emf_h = emf_hll ? hll_emf_flux<N>(l, r, e) : 0.0;
     double f[8], u[8], ps;
@@ -718,13 +770,14 @@ struct Ws2 {
                                 // [4..7] the linear solver's |F_D| maxima and counts,
                                 // [8..9] the deferred faces' con2prim masks (x, y)
   Cells32 cf{};                 // FP32 cell variables of the MIXED form (carved for it only)
+  double* fq[2] = {nullptr, nullptr};   // pass A's face states for pass B (FP64 form only)
   srmhd_flux_diag* dsh;         // DIAG_SHARDS copies of diag[0], then of diag[1]
   int emf_hll = 0;              // the scheme's emf_mode is SRMHD_EMF_HLL
   HlldOpts o;                   // the scheme's HLLD p* options
   int64_t bytes;
 };
 
-Ws2 carve2d(const G2& g, void* work, bool cells32 = false) {
+Ws2 carve2d(const G2& g, void* work, bool cells32 = false, bool faceq = false) {
   Carver c{(char*)work};
   Ws2 w;
   const int64_t nfx = (int64_t)(g.nxt + 1) * g.nyt, nfy = (int64_t)g.nxt * (g.nyt + 1);
@@ -740,6 +793,10 @@ Ws2 carve2d(const G2& g, void* work, bool cells32 = false) {
   w.dsh = (srmhd_flux_diag*)c.take<char>(2 * DIAG_SHARDS * DIAG_SHARD_BYTES);
   if (cells32)
     for (int k = 0; k < 8; ++k) w.cf.p[k] = c.take<float>(ncell);
+  if (faceq) {
+    w.fq[0] = c.take<double>(FQ_N * nfx);
+    w.fq[1] = c.take<double>(FQ_N * nfy);
+  }
   w.bytes = c.off;
   return w;
 }
@@ -807,18 +864,26 @@ void sweep_dir(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, cons
   C2PRec* rec = N == 0 ? w.recx : w.recy;
   const bool hd = SOL == SOLVER_HLLD;
   unsigned long long* mres = hd ? w.words + 8 + N : nullptr;
+  // HLLD: pass B takes the face states pass A formed (when the workspace
+  // holds them) instead of reconstructing them again
+  double* fq = hd ? w.fq[N] : nullptr;
   if (hd) {
     tbegin(N == 0 ? SRMHD_K_SWEEP_X_A : SRMHD_K_SWEEP_Y_A, s);
-    k_sweep_a<N, LIM, true><<<nb, BLK, 0, s>>>(g, S, P, e, rec, sh);
+    k_sweep_a<N, LIM, true><<<nb, BLK, 0, s>>>(g, S, P, e, rec, sh, fq);
     tend(N == 0 ? SRMHD_K_SWEEP_X_A : SRMHD_K_SWEEP_Y_A, s);
     k_fold_diag<<<1, 64, 0, s>>>(sh, w.diag + N);   // the batch mask for pass B
     k_sweep_a_rescue<N, LIM><<<nb < RESCUE_BLOCKS ? nb : RESCUE_BLOCKS, BLK, 0, s>>>(
         g, S, P, e, rec, w.diag + N, mres);
   }
   tbegin(N == 0 ? SRMHD_K_SWEEP_X_B : SRMHD_K_SWEEP_Y_B, s);
-  k_sweep_b<N, LIM, SOL><<<nb, BLK, 0, s>>>(g, S, P, e, rec, w.diag + N, sh,
-                                             N == 0 ? w.Fx : w.Fy, N == 0 ? w.psx : w.psy,
-                                             w.emf_hll, w.o.tol, mres);
+  if (fq)
+    k_sweep_b<N, LIM, SOL, false, true><<<nb, BLK, 0, s>>>(
+        g, S, P, e, rec, w.diag + N, sh, N == 0 ? w.Fx : w.Fy, N == 0 ? w.psx : w.psy,
+        w.emf_hll, w.o.tol, mres, fq);
+  else
+    k_sweep_b<N, LIM, SOL><<<nb, BLK, 0, s>>>(g, S, P, e, rec, w.diag + N, sh,
+                                               N == 0 ? w.Fx : w.Fy, N == 0 ? w.psx : w.psy,
+                                               w.emf_hll, w.o.tol, mres);
   tend(N == 0 ? SRMHD_K_SWEEP_X_B : SRMHD_K_SWEEP_Y_B, s);
 }
 
@@ -1384,7 +1449,8 @@ int srmhd_p2c(int64_t n, const double* const* prim, double* const* cons, void* s
 
 int64_t srmhd2d_workspace_bytes(const srmhd_grid2d* g, const srmhd_scheme2d* s) {
   if (!g || !s || !grid_ok(g, s->limiter)) return -1;
-  return carve2d(make_g2(g), nullptr, s->precision == SRMHD_PREC_MIXED).bytes;
+  return carve2d(make_g2(g), nullptr, s->precision == SRMHD_PREC_MIXED,
+                 s->precision == SRMHD_PREC_FP64).bytes;
 }
 
 // the HLLD p* options are the reference's default
@@ -1429,7 +1495,8 @@ static int stage_impl(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, const sr
   if (tape && (!hlld_default(hlld_opts(&sp->hlld)) || sp->solver == SRMHD_LINEAR))
     return SRMHD_EARG;
   const G2 g = make_g2(gp);
-  Ws2 w = carve2d(g, work, sp->precision == SRMHD_PREC_MIXED);
+  Ws2 w = carve2d(g, work, sp->precision == SRMHD_PREC_MIXED,
+                  sp->precision == SRMHD_PREC_FP64);
   if (work_bytes < w.bytes) return SRMHD_EARG;
   w.emf_hll = sp->emf_mode == SRMHD_EMF_HLL;
   w.o = hlld_opts(&sp->hlld);
