@@ -179,48 +179,13 @@ __global__ __launch_bounds__(BLK) void k_p2c(int64_t n, PrimPtrs P, ConsPtrs C) 
 }
 
 
-// The face states of a sweep as pass A forms them (face_prims2d), kept for
-// pass B: FQ_N arrays of the sweep's faces -- rho, v (3), eps, B_t1, B_t2 of
-// the left state, then of the right one.  B_n is the face field; the
-// pressure is not kept: the solvers form it again from eps (side_state) and
-// read no other.
-constexpr int FQ_N = 14;
+// pass B of the FP64 HLLD sweeps reads the face states pass A kept (the
+// face_q_* helpers of srmhd_riemann.h); B_n is the face field
 template <int N>
-__device__ inline void face_q_store(double* fq, int64_t nf, int64_t t, const Prim& l,
-                                    const Prim& r) {
-  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const Prim& q = s ? r : l;
-    double* o = fq + (int64_t)(7 * s) * nf + t;
-    o[0] = q.rho;
-    o[nf] = q.v[0];
-    o[2 * nf] = q.v[1];
-    o[3 * nf] = q.v[2];
-    o[4 * nf] = q.eps;
-    o[5 * nf] = q.B[T1];
-    o[6 * nf] = q.B[T2];
-  }
-}
-template <int N>
-__device__ inline void face_q_load(const G2& g, const St2& S, const double* fq, int64_t nf,
-                                   int64_t t, int i, int j, Prim& l, Prim& r) {
-  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+__device__ inline void face_q_load2d(const G2& g, const St2& S, const double* fq, int64_t nf,
+                                     int64_t t, int i, int j, Prim& l, Prim& r) {
   const double Bn = N == 0 ? S.p[6][(int64_t)i * g.nyt + j] : S.p[7][(int64_t)i * (g.nyt + 1) + j];
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    Prim& q = s ? r : l;
-    const double* o = fq + (int64_t)(7 * s) * nf + t;
-    q.rho = o[0];
-    q.v[0] = o[nf];
-    q.v[1] = o[2 * nf];
-    q.v[2] = o[3 * nf];
-    q.p = 0.0;   // not read by the solvers (see above)
-    q.eps = o[4 * nf];
-    q.B[N] = Bn;
-    q.B[T1] = o[5 * nf];
-    q.B[T2] = o[6 * nf];
-  }
+  face_q_load<N>(fq, nf, t, Bn, l, r);
 }
 
 // Pass A of the sweep's HLL-state con2prims.  DEFER: faces whose con2prim
@@ -306,7 +271,7 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
     face_ij<N>(g, t, i, j);
     Prim l, r;
     if (FQ)
-      face_q_load<N>(g, S, fq, n_faces<N>(g), t, i, j, l, r);
+      face_q_load2d<N>(g, S, fq, n_faces<N>(g), t, i, j, l, r);
     else
       face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
     // the HLL face EMF first, while only the face states are live

@@ -239,6 +239,48 @@ SRMHD_DEV T star_residual(const StarRT<T>& R, const T& p) {
   return (o.la[0] - o.la[1]) + R.Bn[0] * (sdiv(1.0 - o.KR2, dR) + sdiv(1.0 - o.KL2, dL));
 }
 
+// The face states of a sweep as its HLLD pass A forms them, kept for its
+// pass B (which would otherwise reconstruct them again): FQ_N arrays of the
+// sweep's nf faces -- rho, v (3), eps, B_t1, B_t2 of the left state, then
+// of the right one.  B_n is the caller's (the face field); the pressure is
+// not kept: the solvers form it again from eps (side_state) and read no
+// other.
+constexpr int FQ_N = 14;
+template <int N>
+SRMHD_DEV void face_q_store(double* fq, int64_t nf, int64_t t, const Prim& l, const Prim& r) {
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const Prim& q = s ? r : l;
+    double* o = fq + (int64_t)(7 * s) * nf + t;
+    o[0] = q.rho;
+    o[nf] = q.v[0];
+    o[2 * nf] = q.v[1];
+    o[3 * nf] = q.v[2];
+    o[4 * nf] = q.eps;
+    o[5 * nf] = q.B[T1];
+    o[6 * nf] = q.B[T2];
+  }
+}
+template <int N>
+SRMHD_DEV void face_q_load(const double* fq, int64_t nf, int64_t t, double Bn, Prim& l, Prim& r) {
+  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    Prim& q = s ? r : l;
+    const double* o = fq + (int64_t)(7 * s) * nf + t;
+    q.rho = o[0];
+    q.v[0] = o[nf];
+    q.v[1] = o[2 * nf];
+    q.v[2] = o[3 * nf];
+    q.p = 0.0;   // not read by the solvers (see above)
+    q.eps = o[4 * nf];
+    q.B[N] = Bn;
+    q.B[T1] = o[5 * nf];
+    q.B[T2] = o[6 * nf];
+  }
+}
+
 // Everything of hlld_flux that precedes the con2prim of the HLL state.
 template <int N, class T = double>
 struct HlldPrep {

@@ -5,6 +5,7 @@
 #include "srmhd_launch.h"
 #include "srmhd_c2p.h"
 #include "srmhd_recon.h"
+#include "srmhd_riemann.h"
 
 namespace srmhd {
 
@@ -32,6 +33,7 @@ struct TubeArgs {
   double* F[7];
   C2PRecK* recf;
   C2PRecK* recc;
+  double* fq;             // the phased form's face states, pass A to pass B (FQ_N x faces)
   TubeState ts;           // the phased form's per-tube words
   int64_t cells;          // all tubes' cells
 };

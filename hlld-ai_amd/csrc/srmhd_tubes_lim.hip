@@ -460,6 +460,7 @@ __global__ __launch_bounds__(BLK) void k_tp_flux_a(TubeArgs A, TubeState S, int 
     tube_prims(A, oc, P);
     Prim l, r;
     tube_face<LIM>(P, A.U[bu][5] + oc, A.U[bu][6] + oc, A.Bn[tb], f, nt, e, l, r);
+    face_q_store<0>(A.fq, nf, i, l, r);   // for pass B and the redo
     HlldPrep<0> h;
     h.run(l, r, e);
     Kastaun k;
@@ -505,8 +506,14 @@ __global__ __launch_bounds__(BLK) void k_tp_flux_b(TubeArgs A, TubeState S, int 
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i >= nf) return;
   Prim l, r;
-  int f;
-  const int tb = tp_face<LIM>(A, n, i, bu, e, l, r, f);
+  int f, tb;
+  if (SOL == SOLVER_HLLD) {   // the face states pass A kept
+    tb = tube_of<true>(A.off_c, n, i);
+    f = (int)(i - A.off_c[tb] - tb);
+    face_q_load<0>(A.fq, nf, i, A.Bn[tb], l, r);
+  } else {
+    tb = tp_face<LIM>(A, n, i, bu, e, l, r, f);
+  }
   if (f == 0) S.mc[tb] = 0ull;
   double fl[8], u[8], ps;
   if (SOL == SOLVER_HLLE) {
@@ -541,9 +548,9 @@ __global__ __launch_bounds__(BLK) void k_tp_flux_redo(TubeArgs A, TubeState S, i
   for (int64_t q = (int64_t)blockIdx.x * BLK + threadIdx.x; q < cnt;
        q += (int64_t)gridDim.x * BLK) {
     const int64_t i = S.qf[q];
+    const int tb = tube_of<true>(A.off_c, n, i);
     Prim l, r;
-    int f;
-    tp_face<LIM>(A, n, i, bu, e, l, r, f);
+    face_q_load<0>(A.fq, A.cells + n, i, A.Bn[tb], l, r);   // kept by pass A
     HlldPrep<0> h;
     h.run(l, r, e);
     Kastaun k;
