@@ -181,8 +181,10 @@ def cpu_baseline_tubes(lengths, left, right, Bn, steps=2, limiter="mp5"):
 
 
 def bench_tubes(a):
-    """4096 ragged Riemann tubes, MP5 + HLLD + SSP-RK3, per-tube dt, all
-    steps in one persistent graph-captured launch.  Returns the result dict."""
+    """4096 ragged Riemann tubes, MP5 + HLLD + SSP-RK3, per-tube dt, the
+    timed steps captured as one HIP graph of the autotuned form (the phased
+    form's per-phase launches over every tube, or one persistent launch with
+    a workgroup per tube).  Returns the result dict."""
     import torch
     from srmhd import eos as E, tubes as T
     dev = torch.device("cuda:0")
@@ -231,11 +233,12 @@ def bench_tubes(a):
     loop_wall = time.perf_counter() - t0
     loop_zc = int(lengths[:n_loop].sum()) * a.steps / loop_wall
     # compulsory bytes per cell and RK stage, each phase reading and writing
-    # its arrays once: HLLD pass A 64 in (P, By, Bz) + 24 out (record); pass B
-    # 88 in + 56 out (7 fluxes); update 168 in (F, U0, U) + 56 out; cell
-    # con2prim pass A 56 in + 24 out; pass B 80 in + 48 out (P) -- 664 B; plus
-    # 64 B per cell and step for the CFL reduction
-    tube_bytes = float(zones) * a.steps * (3 * 664 + 64)
+    # its arrays once: HLLD pass A 64 in (P, By, Bz) + 24 (record) + 112 (the
+    # face states kept for pass B) out; pass B 136 in (face states, record) +
+    # 56 out (7 fluxes); update 168 in (F, U0, U) + 56 out; cell con2prim pass
+    # A 56 in + 24 out; pass B 80 in + 48 out (P) -- 824 B; plus 64 B per cell
+    # and step for the CFL reduction
+    tube_bytes = float(zones) * a.steps * (3 * 824 + 64)
     out = {
         "metric": "zone-cycles/s, batched 1D Riemann tubes (4096 ragged tubes, MP5/HLLD/RK3)",
         "value": zc, "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -243,14 +246,16 @@ def bench_tubes(a):
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{a.tubes} Riemann tubes, lengths U[128, 2048] (seed 0, "
                                f"{zones} cells), {a.limiter.upper()} + {a.solver.upper()} + "
-                               f"SSP-RK3, per-tube dt, CFL {a.cfl}, float64, one graph-captured "
-                               "persistent launch",
+                               f"SSP-RK3, per-tube dt, CFL {a.cfl}, float64, the steps "
+                               "captured as one HIP graph",
                    "tubes": a.tubes, "cells": zones, "block": best},
         "kernel_ms": kernel_ms,
         "roofline": {"bound": "hbm", "achieved": tube_bytes / (kernel_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tube_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_tubes (the persistent launch, every step)",
+                     "traffic": None,
+                     "kernel": ("the phased form's launches, every step" if best == 0 else
+                                f"k_tubes (the persistent launch, block {best}, every step)"),
                      "avg_launch_ms": kernel_ms, "launches": 1,
                      "algorithmic_bytes_per_launch": tube_bytes},
         "per_tube_loop": {"value": loop_zc, "unit": "zone-cycles/s", "tubes_timed": n_loop,
