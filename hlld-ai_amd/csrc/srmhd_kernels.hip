@@ -179,24 +179,15 @@ __global__ __launch_bounds__(BLK) void k_p2c(int64_t n, PrimPtrs P, ConsPtrs C) 
 }
 
 
-// pass B of the FP64 HLLD sweeps reads the face states pass A kept (the
-// face_q_* helpers of srmhd_riemann.h); B_n is the face field
-template <int N>
-__device__ inline void face_q_load2d(const G2& g, const St2& S, const double* fq, int64_t nf,
-                                     int64_t t, int i, int j, Prim& l, Prim& r) {
-  const double Bn = N == 0 ? S.p[6][(int64_t)i * g.nyt + j] : S.p[7][(int64_t)i * (g.nyt + 1) + j];
-  face_q_load<N>(fq, nf, t, Bn, l, r);
-}
-
 // Pass A of the sweep's HLL-state con2prims.  DEFER: faces whose con2prim
 // is inert (hll_c2p_inert: B_n = 0 on both sides, so the flux is the HLLE
 // one) are left out; their result matters only when the other faces'
 // masks leave the batch's break iteration below 60, and k_sweep_a_rescue
 // then runs them.  On the rotor ~95 % of the y faces are such faces.
-// fq (non-NULL): the face states are kept there for pass B (face_q_store).
+// hq (non-NULL): the side states are kept there for pass B (side_q_store).
 template <int N, int LIM, bool DEFER = false>
 __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e, C2PRec* rec,
-                                                 srmhd_flux_diag* dsh, double* fq = nullptr) {
+                                                 srmhd_flux_diag* dsh, double* hq = nullptr) {
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   uint64_t m = 0;
   if (t < n_faces<N>(g)) {
@@ -204,9 +195,9 @@ __global__ __launch_bounds__(BLK) void k_sweep_a(G2 g, St2 S, Pr2 P, srmhd_eos e
     face_ij<N>(g, t, i, j);
     Prim l, r;
     face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
-    if (fq) face_q_store<N>(fq, n_faces<N>(g), t, l, r);
     HlldPrep<N> h;
     h.run(l, r, e);
+    if (hq) side_q_store<N>(hq, n_faces<N>(g), t, h);
     if (!(DEFER && hll_c2p_inert<N>(h))) {
       Kastaun k;
       k.init(h.uH, e);
@@ -252,15 +243,15 @@ __global__ __launch_bounds__(BLK) void k_sweep_a_rescue(G2 g, St2 S, Pr2 P, srmh
 // no pass-A record and skip their con2prim: p_hll then reaches only the
 // fallback sentinel of p* (written as -1 - 1e-10 there: negative, as every
 // fallback's), and the rest of the solver is unchanged.
-// FQ: the face states come from pass A's copy fq (face_q_load) instead of
-// the reconstruction.
-template <int N, int LIM, int SOL, bool LOCAL = false, bool FQ = false>
+// HQ: the side states come from pass A's copy hq (side_q_load) instead of
+// the reconstruction and HlldPrep::run; the HLL face EMF from them too.
+template <int N, int LIM, int SOL, bool LOCAL = false, bool HQ = false>
 __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e,
                                                  const C2PRec* rec, const srmhd_flux_diag* d,
                                                  srmhd_flux_diag* dsh, Flx F, double* pstar_out,
                                                  int emf_hll, double tol,
                                                  const unsigned long long* mres = nullptr,
-                                                 const double* fq = nullptr) {
+                                                 const double* hq = nullptr) {
   static_assert(!LOCAL || SOL == SOLVER_HLLD, "the local con2prim concerns HLLD only");
   const int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x;
   const bool active = t < n_faces<N>(g);
@@ -269,21 +260,27 @@ __global__ __launch_bounds__(BLK) void k_sweep_b(G2 g, St2 S, Pr2 P, srmhd_eos e
   if (active) {
     int i, j;
     face_ij<N>(g, t, i, j);
-    Prim l, r;
-    if (FQ)
-      face_q_load2d<N>(g, S, fq, n_faces<N>(g), t, i, j, l, r);
-    else
+    Prim l{}, r{};   // (HQ: not formed; the solver reads them not)
+    HlldPrep<N> h;
+    double emf_h = 0.0;
+    if (HQ) {
+      static_assert(!HQ || SOL == SOLVER_HLLD, "the side states come from HLLD's pass A");
+      side_q_load<N>(hq, n_faces<N>(g), t, h);
+      // hll_emf_flux's arithmetic on the same side states
+      constexpr int c = N == 0 ? CB + 1 : CB + 0;
+      if (emf_hll) emf_h = hll_flux(h.fL[c], h.fR[c], h.uL[c], h.uR[c], h.cmin, h.cmax);
+    } else {
       face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
-    // the HLL face EMF first, while only the face states are live
-    const double emf_h = emf_hll ? hll_emf_flux<N>(l, r, e) : 0.0;
+      // the HLL face EMF first, while only the face states are live
+      emf_h = emf_hll ? hll_emf_flux<N>(l, r, e) : 0.0;
+    }
     double f[8], u[8], ps;
     if (SOL == SOLVER_HLLE) {
       hlle<N>(l, r, e, f, u, ps);
     } else if (SOL == SOLVER_HLLC) {
       hllc<N>(l, r, e, f, u, ps);
     } else {
-      HlldPrep<N> h;
-      h.run(l, r, e);
+      if (!HQ) h.run(l, r, e);
       double p_hll = 1.0;
       int cst = 0;
       if (!(!LOCAL && mres && c2p_mask_full(d->c2p_mask) && hll_c2p_inert<N>(h))) {
@@ -735,14 +732,14 @@ struct Ws2 {
                                 // [4..7] the linear solver's |F_D| maxima and counts,
                                 // [8..9] the deferred faces' con2prim masks (x, y)
   Cells32 cf{};                 // FP32 cell variables of the MIXED form (carved for it only)
-  double* fq[2] = {nullptr, nullptr};   // pass A's face states for pass B (FP64 form only)
+  double* hq[2] = {nullptr, nullptr};   // pass A's side states for pass B (FP64 form only)
   srmhd_flux_diag* dsh;         // DIAG_SHARDS copies of diag[0], then of diag[1]
   int emf_hll = 0;              // the scheme's emf_mode is SRMHD_EMF_HLL
   HlldOpts o;                   // the scheme's HLLD p* options
   int64_t bytes;
 };
 
-Ws2 carve2d(const G2& g, void* work, bool cells32 = false, bool faceq = false) {
+Ws2 carve2d(const G2& g, void* work, bool cells32 = false, bool sideq = false) {
   Carver c{(char*)work};
   Ws2 w;
   const int64_t nfx = (int64_t)(g.nxt + 1) * g.nyt, nfy = (int64_t)g.nxt * (g.nyt + 1);
@@ -758,9 +755,9 @@ Ws2 carve2d(const G2& g, void* work, bool cells32 = false, bool faceq = false) {
   w.dsh = (srmhd_flux_diag*)c.take<char>(2 * DIAG_SHARDS * DIAG_SHARD_BYTES);
   if (cells32)
     for (int k = 0; k < 8; ++k) w.cf.p[k] = c.take<float>(ncell);
-  if (faceq) {
-    w.fq[0] = c.take<double>(FQ_N * nfx);
-    w.fq[1] = c.take<double>(FQ_N * nfy);
+  if (sideq) {
+    w.hq[0] = c.take<double>(HQ_N * nfx);
+    w.hq[1] = c.take<double>(HQ_N * nfy);
   }
   w.bytes = c.off;
   return w;
@@ -829,22 +826,22 @@ void sweep_dir(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, cons
   C2PRec* rec = N == 0 ? w.recx : w.recy;
   const bool hd = SOL == SOLVER_HLLD;
   unsigned long long* mres = hd ? w.words + 8 + N : nullptr;
-  // HLLD: pass B takes the face states pass A formed (when the workspace
-  // holds them) instead of reconstructing them again
-  double* fq = hd ? w.fq[N] : nullptr;
+  // HLLD: pass B takes the side states pass A formed (when the workspace
+  // holds them) instead of reconstructing the faces and forming them again
+  double* hq = hd ? w.hq[N] : nullptr;
   if (hd) {
     tbegin(N == 0 ? SRMHD_K_SWEEP_X_A : SRMHD_K_SWEEP_Y_A, s);
-    k_sweep_a<N, LIM, true><<<nb, BLK, 0, s>>>(g, S, P, e, rec, sh, fq);
+    k_sweep_a<N, LIM, true><<<nb, BLK, 0, s>>>(g, S, P, e, rec, sh, hq);
     tend(N == 0 ? SRMHD_K_SWEEP_X_A : SRMHD_K_SWEEP_Y_A, s);
     k_fold_diag<<<1, 64, 0, s>>>(sh, w.diag + N);   // the batch mask for pass B
     k_sweep_a_rescue<N, LIM><<<nb < RESCUE_BLOCKS ? nb : RESCUE_BLOCKS, BLK, 0, s>>>(
         g, S, P, e, rec, w.diag + N, mres);
   }
   tbegin(N == 0 ? SRMHD_K_SWEEP_X_B : SRMHD_K_SWEEP_Y_B, s);
-  if (fq)
-    k_sweep_b<N, LIM, SOL, false, true><<<nb, BLK, 0, s>>>(
+  if (hq)
+    k_sweep_b<N, LIM, SOLVER_HLLD, false, true><<<nb, BLK, 0, s>>>(
         g, S, P, e, rec, w.diag + N, sh, N == 0 ? w.Fx : w.Fy, N == 0 ? w.psx : w.psy,
-        w.emf_hll, w.o.tol, mres, fq);
+        w.emf_hll, w.o.tol, mres, hq);
   else
     k_sweep_b<N, LIM, SOL><<<nb, BLK, 0, s>>>(g, S, P, e, rec, w.diag + N, sh,
                                                N == 0 ? w.Fx : w.Fy, N == 0 ? w.psx : w.psy,

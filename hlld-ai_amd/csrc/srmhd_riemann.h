@@ -239,48 +239,6 @@ SRMHD_DEV T star_residual(const StarRT<T>& R, const T& p) {
   return (o.la[0] - o.la[1]) + R.Bn[0] * (sdiv(1.0 - o.KR2, dR) + sdiv(1.0 - o.KL2, dL));
 }
 
-// The face states of a sweep as its HLLD pass A forms them, kept for its
-// pass B (which would otherwise reconstruct them again): FQ_N arrays of the
-// sweep's nf faces -- rho, v (3), eps, B_t1, B_t2 of the left state, then
-// of the right one.  B_n is the caller's (the face field); the pressure is
-// not kept: the solvers form it again from eps (side_state) and read no
-// other.
-constexpr int FQ_N = 14;
-template <int N>
-SRMHD_DEV void face_q_store(double* fq, int64_t nf, int64_t t, const Prim& l, const Prim& r) {
-  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const Prim& q = s ? r : l;
-    double* o = fq + (int64_t)(7 * s) * nf + t;
-    o[0] = q.rho;
-    o[nf] = q.v[0];
-    o[2 * nf] = q.v[1];
-    o[3 * nf] = q.v[2];
-    o[4 * nf] = q.eps;
-    o[5 * nf] = q.B[T1];
-    o[6 * nf] = q.B[T2];
-  }
-}
-template <int N>
-SRMHD_DEV void face_q_load(const double* fq, int64_t nf, int64_t t, double Bn, Prim& l, Prim& r) {
-  constexpr int T1 = (N + 1) % 3, T2 = (N + 2) % 3;
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    Prim& q = s ? r : l;
-    const double* o = fq + (int64_t)(7 * s) * nf + t;
-    q.rho = o[0];
-    q.v[0] = o[nf];
-    q.v[1] = o[2 * nf];
-    q.v[2] = o[3 * nf];
-    q.p = 0.0;   // not read by the solvers (see above)
-    q.eps = o[4 * nf];
-    q.B[N] = Bn;
-    q.B[T1] = o[5 * nf];
-    q.B[T2] = o[6 * nf];
-  }
-}
-
 // Everything of hlld_flux that precedes the con2prim of the HLL state.
 template <int N, class T = double>
 struct HlldPrep {
@@ -297,6 +255,40 @@ struct HlldPrep {
     for (int k = 0; k < 8; ++k) uH[k] = hll_state(fL[k], fR[k], uL[k], uR[k], cmin, cmax);
   }
 };
+
+// The side states and HLL speeds of a sweep's faces as HLLD's pass A forms
+// them (HlldPrep::run), kept for its pass B, which would otherwise
+// reconstruct the face states and form them again: HQ_N arrays of the
+// sweep's nf faces -- uL, uR, fL, fR (8 each), cmax, cmin.  The HLL state
+// uH is formed again on load (the same arithmetic as HlldPrep::run).
+constexpr int HQ_N = 34;
+template <int N>
+SRMHD_DEV void side_q_store(double* hq, int64_t nf, int64_t t, const HlldPrep<N>& h) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    hq[(int64_t)k * nf + t] = h.uL[k];
+    hq[(int64_t)(8 + k) * nf + t] = h.uR[k];
+    hq[(int64_t)(16 + k) * nf + t] = h.fL[k];
+    hq[(int64_t)(24 + k) * nf + t] = h.fR[k];
+  }
+  hq[(int64_t)32 * nf + t] = h.cmax;
+  hq[(int64_t)33 * nf + t] = h.cmin;
+}
+template <int N>
+SRMHD_DEV void side_q_load(const double* hq, int64_t nf, int64_t t, HlldPrep<N>& h) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    h.uL[k] = hq[(int64_t)k * nf + t];
+    h.uR[k] = hq[(int64_t)(8 + k) * nf + t];
+    h.fL[k] = hq[(int64_t)(16 + k) * nf + t];
+    h.fR[k] = hq[(int64_t)(24 + k) * nf + t];
+  }
+  h.cmax = hq[(int64_t)32 * nf + t];
+  h.cmin = hq[(int64_t)33 * nf + t];
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    h.uH[k] = hll_state(h.fL[k], h.fR[k], h.uL[k], h.uR[k], h.cmin, h.cmax);
+}
 
 // The face EMF flux of emf_mode "hll" (ct.py:63-88): the HLL average of the
 // tangential-field flux the sweep keeps for the corner EMF (F^x(B_y) for

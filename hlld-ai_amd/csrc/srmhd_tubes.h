@@ -33,7 +33,7 @@ struct TubeArgs {
   double* F[7];
   C2PRecK* recf;
   C2PRecK* recc;
-  double* fq;             // the phased form's face states, pass A to pass B (FQ_N x faces)
+  double* hq;             // the phased form's side states, pass A to pass B (HQ_N x faces)
   TubeState ts;           // the phased form's per-tube words
   int64_t cells;          // all tubes' cells
 };

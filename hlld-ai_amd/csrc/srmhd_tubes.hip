@@ -37,7 +37,7 @@ struct WsT {
   double* U[2][7];
   double* F[7];
   C2PRecK *recf, *recc;
-  double* fq;
+  double* hq;
   TubeState ts;
   int64_t bytes;
 };
@@ -58,7 +58,7 @@ WsT carve_tubes(int64_t cells, int64_t faces, void* work) {
   w.ts.qf = c.take<int64_t>(faces);
   w.ts.qc = c.take<int64_t>(cells);
   w.ts.qn = c.take<unsigned long long>(2);
-  w.fq = c.take<double>(FQ_N * faces);
+  w.hq = c.take<double>(HQ_N * faces);
   w.bytes = c.off;
   return w;
 }
@@ -94,7 +94,7 @@ int tubes_args(int n_tubes, int64_t total_cells, const srmhd_tube_layout* lay,
   }
   A.recf = w.recf;
   A.recc = w.recc;
-  A.fq = w.fq;
+  A.hq = w.hq;
   A.ts = w.ts;
   A.cells = total_cells;
   return SRMHD_OK;

@@ -460,9 +460,9 @@ __global__ __launch_bounds__(BLK) void k_tp_flux_a(TubeArgs A, TubeState S, int 
     tube_prims(A, oc, P);
     Prim l, r;
     tube_face<LIM>(P, A.U[bu][5] + oc, A.U[bu][6] + oc, A.Bn[tb], f, nt, e, l, r);
-    face_q_store<0>(A.fq, nf, i, l, r);   // for pass B and the redo
     HlldPrep<0> h;
     h.run(l, r, e);
+    side_q_store<0>(A.hq, nf, i, h);   // for pass B and the redo
     Kastaun k;
     k.init(h.uH, e);
     m = c2p_pass_a(k, A.recf[i]);
@@ -505,12 +505,11 @@ __global__ __launch_bounds__(BLK) void k_tp_flux_b(TubeArgs A, TubeState S, int 
                                                    int bu, srmhd_eos e) {
   const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
   if (i >= nf) return;
-  Prim l, r;
+  Prim l{}, r{};   // (HLLD: not formed; its solver reads them not)
   int f, tb;
-  if (SOL == SOLVER_HLLD) {   // the face states pass A kept
+  if (SOL == SOLVER_HLLD) {   // the side states pass A kept
     tb = tube_of<true>(A.off_c, n, i);
     f = (int)(i - A.off_c[tb] - tb);
-    face_q_load<0>(A.fq, nf, i, A.Bn[tb], l, r);
   } else {
     tb = tp_face<LIM>(A, n, i, bu, e, l, r, f);
   }
@@ -522,7 +521,7 @@ __global__ __launch_bounds__(BLK) void k_tp_flux_b(TubeArgs A, TubeState S, int 
     hllc<0>(l, r, e, fl, u, ps);
   } else {
     HlldPrep<0> h;
-    h.run(l, r, e);
+    side_q_load<0>(A.hq, nf, i, h);
     Kastaun k;
     k.init(h.uH, e);
     const C2PRecK& rec = A.recf[i];
@@ -541,18 +540,17 @@ __global__ __launch_bounds__(BLK) void k_tp_flux_b(TubeArgs A, TubeState S, int 
 
 // the HLLD flux of the faces on the work list (the bisection redo of their
 // HLL-state con2prim, then the rest of the solver)
+// (two waves per SIMD: the compiler's own choice is 258 registers, one wave)
 template <int LIM>
-__global__ __launch_bounds__(BLK) void k_tp_flux_redo(TubeArgs A, TubeState S, int n, int bu,
-                                                      srmhd_eos e) {
+__global__ __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(2)))
+void k_tp_flux_redo(TubeArgs A, TubeState S, int n, int bu, srmhd_eos e) {
   const int64_t cnt = (int64_t)S.qn[0];
   for (int64_t q = (int64_t)blockIdx.x * BLK + threadIdx.x; q < cnt;
        q += (int64_t)gridDim.x * BLK) {
     const int64_t i = S.qf[q];
-    const int tb = tube_of<true>(A.off_c, n, i);
-    Prim l, r;
-    face_q_load<0>(A.fq, A.cells + n, i, A.Bn[tb], l, r);   // kept by pass A
+    Prim l{}, r{};   // not read by the solver
     HlldPrep<0> h;
-    h.run(l, r, e);
+    side_q_load<0>(A.hq, A.cells + n, i, h);   // kept by pass A
     Kastaun k;
     k.init(h.uH, e);
     const C2PRecK& rec = A.recf[i];
