@@ -54,7 +54,7 @@ DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "swee
 
 # wave-level VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU on the
 # 2048^2 rotor, profiles/rotor2048_counters.md)
-VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 1.836e8, ("fp64", "sweep_x_a", 2048): 1.024e9}
+VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 1.836e8, ("fp64", "sweep_x_a", 2048): 1.029e9}
 
 # measured VALU busy fraction of the launch (same profile): SQ_ACTIVE_INST_VALU
 # (quad-cycles in which a SIMD issued a VALU instruction, summed over the
@@ -73,7 +73,7 @@ ADJ_SWEEP_TRAFFIC = {512: 269.3e6}
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
 # scaled by a plain copy kernel; profiles/rotor2048_counters.md), keyed by
 # (form, kernel class, n)
-MEASURED_TRAFFIC = {("mixed", "sweep_x_b", 2048): 420.0e6, ("fp64", "sweep_x_a", 2048): 894.4e6,
+MEASURED_TRAFFIC = {("mixed", "sweep_x_b", 2048): 420.0e6, ("fp64", "sweep_x_a", 2048): 1592.6e6,
                     ("fp64", "c2p_a", 2048): 376.3e6, ("mixed", "c2p_a", 2048): 490.1e6}
 
 
@@ -93,13 +93,13 @@ def algorithmic_bytes(kernel, n, ng, precision="fp64"):
         # p, vx, vy, vz + cons Bz + both face-B arrays in, 7 flux arrays out
         return 8 * (6 * cells + 2 * faces) + 56 * faces
     if kernel in ("sweep_x_b", "sweep_y_b"):
-        # HLLD's pass B: pass A's face states (14 doubles/face), the normal
-        # face field, the pass-A record (3 doubles/face) in; 7 flux arrays out
-        return (14 * 8 + 8 + 24) * faces + 56 * faces
+        # HLLD's pass B: pass A's side states and HLL speeds (34 doubles/face)
+        # and record (3 doubles/face) in; 7 flux arrays out
+        return (34 * 8 + 24) * faces + 56 * faces
     if kernel in ("sweep_x_a", "sweep_y_a"):
         # prims rho, p, vx, vy, vz + cons Bz + both face-B arrays in; the
-        # record (3 doubles/face) and the face states (14 doubles/face) out
-        return 8 * (6 * cells + 2 * faces) + 24 * faces + 14 * 8 * faces
+        # record (3 doubles/face) and the side states (34 doubles/face) out
+        return 8 * (6 * cells + 2 * faces) + 24 * faces + 34 * 8 * faces
     if precision != "fp64" and kernel == "c2p_a":
         # the per-cell con2prim of the mixed forms: physical cells only, the
         # 6 cell + 2 face state arrays in, 6 primitives out
@@ -233,12 +233,12 @@ def bench_tubes(a):
     loop_wall = time.perf_counter() - t0
     loop_zc = int(lengths[:n_loop].sum()) * a.steps / loop_wall
     # compulsory bytes per cell and RK stage, each phase reading and writing
-    # its arrays once: HLLD pass A 64 in (P, By, Bz) + 24 (record) + 112 (the
-    # face states kept for pass B) out; pass B 136 in (face states, record) +
-    # 56 out (7 fluxes); update 168 in (F, U0, U) + 56 out; cell con2prim pass
-    # A 56 in + 24 out; pass B 80 in + 48 out (P) -- 824 B; plus 64 B per cell
-    # and step for the CFL reduction
-    tube_bytes = float(zones) * a.steps * (3 * 824 + 64)
+    # its arrays once: HLLD pass A 64 in (P, By, Bz) + 24 (record) + 272 (the
+    # side states and HLL speeds kept for pass B) out; pass B 296 in (side
+    # states, record) + 56 out (7 fluxes); update 168 in (F, U0, U) + 56 out;
+    # cell con2prim pass A 56 in + 24 out; pass B 80 in + 48 out (P) --
+    # 1144 B; plus 64 B per cell and step for the CFL reduction
+    tube_bytes = float(zones) * a.steps * (3 * 1144 + 64)
     out = {
         "metric": "zone-cycles/s, batched 1D Riemann tubes (4096 ragged tubes, MP5/HLLD/RK3)",
         "value": zc, "unit": "zone-cycles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
