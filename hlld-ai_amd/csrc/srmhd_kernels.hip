@@ -213,20 +213,26 @@ SRMHD_DEV bool c2p_mask_full(uint64_t m) { return (m & C2P_ALL) == C2P_ALL; }
 // faces' mask d->c2p_mask leaves the break iteration below 60 (otherwise
 // every workgroup returns at once); their mask bits go to *mres.
 template <int N, int LIM>
+// hq (non-NULL): the side states pass A kept, instead of forming them again.
 __global__ __launch_bounds__(BLK) void k_sweep_a_rescue(G2 g, St2 S, Pr2 P, srmhd_eos e,
                                                         C2PRec* rec, const srmhd_flux_diag* d,
-                                                        unsigned long long* mres) {
+                                                        unsigned long long* mres,
+                                                        const double* hq = nullptr) {
   if (c2p_mask_full(d->c2p_mask)) return;
   const int64_t nf = n_faces<N>(g);
   uint64_t m = 0;
   for (int64_t t = (int64_t)blockIdx.x * BLK + threadIdx.x; t < nf;
        t += (int64_t)gridDim.x * BLK) {
-    int i, j;
-    face_ij<N>(g, t, i, j);
-    Prim l, r;
-    face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
     HlldPrep<N> h;
-    h.run(l, r, e);
+    if (hq) {
+      side_q_load<N>(hq, nf, t, h);
+    } else {
+      int i, j;
+      face_ij<N>(g, t, i, j);
+      Prim l, r;
+      face_prims2d<N, LIM>(g, S, P, e, i, j, l, r);
+      h.run(l, r, e);
+    }
     if (hll_c2p_inert<N>(h)) {
       Kastaun k;
       k.init(h.uH, e);
@@ -835,7 +841,7 @@ void sweep_dir(const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e, cons
     tend(N == 0 ? SRMHD_K_SWEEP_X_A : SRMHD_K_SWEEP_Y_A, s);
     k_fold_diag<<<1, 64, 0, s>>>(sh, w.diag + N);   // the batch mask for pass B
     k_sweep_a_rescue<N, LIM><<<nb < RESCUE_BLOCKS ? nb : RESCUE_BLOCKS, BLK, 0, s>>>(
-        g, S, P, e, rec, w.diag + N, mres);
+        g, S, P, e, rec, w.diag + N, mres, hq);
   }
   tbegin(N == 0 ? SRMHD_K_SWEEP_X_B : SRMHD_K_SWEEP_Y_B, s);
   if (hq)
