@@ -482,9 +482,11 @@ def bench_adjoint(a):
     peak = torch.cuda.max_memory_allocated()
     print(f"gradient in {wall:.2f} s", file=sys.stderr, flush=True)
     # the x-sweep adjoint's own launch time, from the gradient of one
-    # checkpoint segment (its records come from the forward, so no re-run
-    # shares the device with the walk, as it does in the run above)
+    # checkpoint segment with the sweeps in sequence (its records come from
+    # the forward, so no re-run shares the device with the walk, and no
+    # y sweep runs beside it, as they do in the run above)
     seg = min(K, steps)
+    roll.st.cs.flags = _lib.STAGE_SERIAL_SWEEPS
     _lib.check(lib.srmhd_timing_enable(_lib.KERNEL_CLASSES["adj_sweep_x"], 2 * seg),
                "srmhd_timing_enable")
     gradient(dts[:seg])
@@ -493,6 +495,7 @@ def bench_adjoint(a):
     cnt = ctypes.c_int64()
     _lib.check(lib.srmhd_timing_read(ctypes.byref(ms), ctypes.byref(cnt)), "srmhd_timing_read")
     lib.srmhd_timing_enable(-1, 0)
+    roll.st.cs.flags = 0
     t1 = time.perf_counter()
     h = 1e-6
 

@@ -637,7 +637,8 @@ G2 make_g2a(const srmhd_grid2d* g) {
 bool grid_ok_a(const srmhd_grid2d* g, const srmhd_scheme2d* s) {
   if (!g || !s || g->nx < 1 || g->ny < 1 || g->ng < 1) return false;
   if (s->precision != SRMHD_PREC_FP64 || s->emf_mode != SRMHD_EMF_SOLVER || s->solver < 0 ||
-      s->solver > 2 || s->limiter < 0 || s->limiter > 5)
+      s->solver > 2 || s->limiter < 0 || s->limiter > 5 ||
+      (s->flags & ~SRMHD_STAGE_SERIAL_SWEEPS) != 0)
     return false;
   // the reference's default p* root (zeroed options): the one the forward taped
   if (s->hlld.pstar_method != SRMHD_PSTAR_SECANT ||
@@ -660,7 +661,7 @@ int window_width(int lim) {
 }
 
 struct WsA {
-  double *lBcx, *lBcy, *lEz, *lBnx, *lBny, *Wx, *Wy, *LQ;
+  double *lBcx, *lBcy, *lEz, *lBnx, *lBny, *Wx, *Wy, *LQx, *LQy;
   double* LE[8];
   double* lFx[7];
   double* lFy[7];
@@ -685,7 +686,9 @@ WsA carve_adj(const G2& g, int lim, void* work) {
   for (int k = 0; k < 7; ++k) w.lFy[k] = c.take<double>(nfy);
   w.Wx = c.take<double>(7 * KW * nfx);
   w.Wy = c.take<double>(7 * KW * nfy);
-  w.LQ = c.take<double>(14 * (nfx > nfy ? nfx : nfy));   // the face-state adjoints of one sweep
+  // the face-state adjoints of each sweep (the two sweeps may run at once)
+  w.LQx = c.take<double>(14 * nfx);
+  w.LQy = c.take<double>(14 * nfy);
   w.bytes = c.off;
   return w;
 }
@@ -706,30 +709,43 @@ void launch_sweep(int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_
 template <int LIM>
 void sweeps_adj(int sol, const G2& g, const St2& S, const Pr2& P, const srmhd_eos& e,
                 const srmhd2d_tape* tape, const WsA& w, Pr2Out lamP, double* lamBz,
-                hipStream_t s) {
+                hipStream_t s, bool par) {
   CFlx lx, ly;
   for (int k = 0; k < 7; ++k) {
     lx.p[k] = w.lFx[k];
     ly.p[k] = w.lFy[k];
   }
+  // par: the y sweep beside the x sweep on the device's second stream (they
+  // share no buffer), joined before the gather that reads both
+  std::lock_guard<std::mutex> lock(g_aux_mu);
+  AuxStream* ax = par ? aux_stream() : nullptr;
+  const bool forked = ax && hipEventRecord(ax->fork, s) == hipSuccess &&
+                      hipStreamWaitEvent(ax->s, ax->fork, 0) == hipSuccess;
   timing_begin(SRMHD_K_ADJ_SWEEP_X, s);
-  launch_sweep<0, LIM>(sol, g, S, P, e, tape->pstar_x, lx, w.LQ, w.Wx, w.lBnx, s);
+  launch_sweep<0, LIM>(sol, g, S, P, e, tape->pstar_x, lx, w.LQx, w.Wx, w.lBnx, s);
   timing_end(SRMHD_K_ADJ_SWEEP_X, s);
-  launch_sweep<1, LIM>(sol, g, S, P, e, tape->pstar_y, ly, w.LQ, w.Wy, w.lBny, s);
+  launch_sweep<1, LIM>(sol, g, S, P, e, tape->pstar_y, ly, w.LQy, w.Wy, w.lBny,
+                       forked ? ax->s : s);
+  if (forked) {
+    if (hipEventRecord(ax->join, ax->s) == hipSuccess)
+      (void)hipStreamWaitEvent(s, ax->join, 0);
+    else
+      (void)hipStreamSynchronize(ax->s);   // no join event: wait on the host
+  }
   k_adj_gather<LIM><<<blocks_for((int64_t)g.nxt * g.nyt), BLK, 0, s>>>(g, P, w.Wx, w.Wy, lamP,
                                                                        lamBz, w.lBcx, w.lBcy);
 }
 
 void sweeps_adj_dispatch(int lim, int sol, const G2& g, const St2& S, const Pr2& P,
                          const srmhd_eos& e, const srmhd2d_tape* tape, const WsA& w, Pr2Out lamP,
-                         double* lamBz, hipStream_t s) {
+                         double* lamBz, hipStream_t s, bool par) {
   switch (lim) {
-    case LIM_PCM: sweeps_adj<LIM_PCM>(sol, g, S, P, e, tape, w, lamP, lamBz, s); break;
-    case LIM_MC: sweeps_adj<LIM_MC>(sol, g, S, P, e, tape, w, lamP, lamBz, s); break;
-    case LIM_MINMOD: sweeps_adj<LIM_MINMOD>(sol, g, S, P, e, tape, w, lamP, lamBz, s); break;
-    case LIM_WENO5Z: sweeps_adj<LIM_WENO5Z>(sol, g, S, P, e, tape, w, lamP, lamBz, s); break;
-    case LIM_MP5: sweeps_adj<LIM_MP5>(sol, g, S, P, e, tape, w, lamP, lamBz, s); break;
-    default: sweeps_adj<LIM_MP7>(sol, g, S, P, e, tape, w, lamP, lamBz, s);
+    case LIM_PCM: sweeps_adj<LIM_PCM>(sol, g, S, P, e, tape, w, lamP, lamBz, s, par); break;
+    case LIM_MC: sweeps_adj<LIM_MC>(sol, g, S, P, e, tape, w, lamP, lamBz, s, par); break;
+    case LIM_MINMOD: sweeps_adj<LIM_MINMOD>(sol, g, S, P, e, tape, w, lamP, lamBz, s, par); break;
+    case LIM_WENO5Z: sweeps_adj<LIM_WENO5Z>(sol, g, S, P, e, tape, w, lamP, lamBz, s, par); break;
+    case LIM_MP5: sweeps_adj<LIM_MP5>(sol, g, S, P, e, tape, w, lamP, lamBz, s, par); break;
+    default: sweeps_adj<LIM_MP7>(sol, g, S, P, e, tape, w, lamP, lamBz, s, par);
   }
 }
 
@@ -814,8 +830,11 @@ int srmhd2d_stage_adjoint(const srmhd_grid2d* gp, const srmhd_scheme2d* sp, cons
                                            sp->upwind, dt_dev, c, fy);
   k_adj_ec<<<blocks_for((int64_t)g.nxt * g.nyt), BLK, 0, s>>>(
       g, su, sp6, w.lEz, tape->FD_x, tape->FD_y, tape->thresh, sp->upwind, lp, w.lBcx, w.lBcy);
-  // the sweeps
-  sweeps_adj_dispatch(sp->limiter, sp->solver, g, su, sp6, *eos, tape, w, lp, lamU[5], s);
+  // the sweeps; a launch error so far is reported first (creating the second
+  // stream they use may clear the error state when it fails)
+  if (const int rc = launch_status()) return rc;
+  sweeps_adj_dispatch(sp->limiter, sp->solver, g, su, sp6, *eos, tape, w, lp, lamU[5], s,
+                      !(sp->flags & SRMHD_STAGE_SERIAL_SWEEPS));
   k_adj_bc2face<<<blocks_for(ncor), BLK, 0, s>>>(g, w.lBcx, w.lBcy, w.lBnx, w.lBny, lamU[6],
                                                  lamU[7]);
   return launch_status();

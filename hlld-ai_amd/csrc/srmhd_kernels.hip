@@ -777,49 +777,6 @@ static_assert(N_WORDS <= DIAG_ZERO_BLOCKS * 256, "the stage's zeroing grid cover
 // workgroups of k_sweep_a_rescue (grid-stride over the faces)
 constexpr int RESCUE_BLOCKS = 1024;
 
-// A second stream per device for work that can run beside the sweeps (the
-// FP64 form's y sweep beside its x sweep; the mixed form's FP64 fix-up of the
-// x faces, a few latency-bound waves, beside the y sweep), forked and joined
-// with events.  Created at first use, at the
-// default priority.
-struct AuxStream {
-  hipStream_t s = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-  bool failed = false;   // creation failed once: this device stays on one stream
-};
-constexpr int AUX_MAX_DEVICES = 64;
-AuxStream g_aux[AUX_MAX_DEVICES];
-std::mutex g_aux_mu;   // held while a stage enqueues its fork/join
-
-// The current device's auxiliary stream, created at first use (nullptr when
-// unavailable: the caller then stays on its own stream).  Caller holds
-// g_aux_mu.
-AuxStream* aux_stream() {
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= AUX_MAX_DEVICES) return nullptr;
-  AuxStream& a = g_aux[dev];
-  if (!a.s && !a.failed) {
-    hipStream_t st = nullptr;
-    hipEvent_t f = nullptr, j = nullptr;
-    const bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
-                    hipEventCreateWithFlags(&f, hipEventDisableTiming) == hipSuccess &&
-                    hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
-    if (!ok) {
-      if (j) (void)hipEventDestroy(j);
-      if (f) (void)hipEventDestroy(f);
-      if (st) (void)hipStreamDestroy(st);
-      (void)hipGetLastError();   // the failure is handled here, not by the next launch check
-      a.failed = true;
-      return nullptr;
-    }
-    a.fork = f;
-    a.join = j;
-    a.s = st;
-  }
-  return a.s ? &a : nullptr;
-}
-
-
 // the sweep along N of ct_stage for one (limiter, solver), on stream s.
 // HLLD: pass A with the inert faces deferred, the batch mask folded, the
 // deferred faces' pass A only if that mask is not full, then pass B.  Each

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "srmhd_common.h"
 
 namespace srmhd {
@@ -14,6 +16,49 @@ inline int blocks_for(int64_t n) { return (int)((n + BLK - 1) / BLK); }
 inline int hip_status(hipError_t e) { return e == hipSuccess ? SRMHD_OK : SRMHD_EHIP + (int)e; }
 
 inline int launch_status() { return hip_status(hipGetLastError()); }
+
+// A second stream per device for work that can run beside the sweeps (the
+// FP64 form's y sweep beside its x sweep; the mixed form's FP64 fix-up of the
+// x faces, a few latency-bound waves, beside the y sweep; the adjoint's y
+// sweep beside its x sweep), forked and joined with events.  Created at
+// first use, at the default priority; one per device, shared by every
+// translation unit of the library (inline variables).
+struct AuxStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  bool failed = false;   // creation failed once: this device stays on one stream
+};
+constexpr int AUX_MAX_DEVICES = 64;
+inline AuxStream g_aux[AUX_MAX_DEVICES];
+inline std::mutex g_aux_mu;   // held while a stage enqueues its fork/join
+
+// The current device's auxiliary stream, created at first use (nullptr when
+// unavailable: the caller then stays on its own stream).  Caller holds
+// g_aux_mu.
+inline AuxStream* aux_stream() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= AUX_MAX_DEVICES) return nullptr;
+  AuxStream& a = g_aux[dev];
+  if (!a.s && !a.failed) {
+    hipStream_t st = nullptr;
+    hipEvent_t f = nullptr, j = nullptr;
+    const bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess &&
+                    hipEventCreateWithFlags(&f, hipEventDisableTiming) == hipSuccess &&
+                    hipEventCreateWithFlags(&j, hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      if (j) (void)hipEventDestroy(j);
+      if (f) (void)hipEventDestroy(f);
+      if (st) (void)hipStreamDestroy(st);
+      (void)hipGetLastError();   // the failure is handled here, not by the next launch check
+      a.failed = true;
+      return nullptr;
+    }
+    a.fork = f;
+    a.join = j;
+    a.s = st;
+  }
+  return a.s ? &a : nullptr;
+}
 
 __device__ inline unsigned long long wave_or(unsigned long long v) {
 #pragma unroll

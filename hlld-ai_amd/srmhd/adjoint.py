@@ -54,8 +54,10 @@ class Rollout2D:
                   hlld_opts_from_globals=False)
         self.st = Stepper2D(grid, eos, **kw)
         # the backward re-runs the next segment while it walks the current
-        # one: that re-run has its own stepper (workspace, dt) and stream
-        self.st_rerun = Stepper2D(grid, eos, **kw)
+        # one: that re-run has its own stepper (workspace, dt) and stream,
+        # and keeps its two sweeps in sequence on it (the device's second
+        # stream then serves the walk's sweeps alone)
+        self.st_rerun = Stepper2D(grid, eos, serial_sweeps=bool(overlap), **kw)
         self.overlap = bool(overlap)
         self.side = torch.cuda.Stream(device=self.st.device) if self.overlap else None
         self.grid = grid

@@ -7,14 +7,15 @@
  * whose size the matching *_workspace_bytes() returns (srmhd_flux_noalloc
  * takes none: it uses a fixed device array of the library's); no device
  * memory is allocated inside.  One exception holds host-side HIP objects: the first
- * srmhd2d_stage on a device that uses a second stream (the FP64 form, whose
- * y sweep runs beside its x sweep unless SRMHD_STAGE_SERIAL_SWEEPS is set;
+ * call on a device that uses a second stream -- srmhd2d_stage in the FP64
+ * form and srmhd2d_stage_adjoint, whose y sweep runs beside the x sweep
+ * unless the scheme's flags hold SRMHD_STAGE_SERIAL_SWEEPS; srmhd2d_stage in
  * the mixed-precision forms with HLLD, whose FP64 fix-up of the x faces runs
- * beside the y sweep) creates one non-blocking stream and two events for
- * that device (default priority), kept until the process ends; if their
- * creation fails the stage runs on the caller's stream alone.  All calls are
- * asynchronous on the stream (the second stream is forked from and joined
- * back into it with events).  Return value: 0 on
+ * beside the y sweep -- creates one non-blocking stream and two events for
+ * that device (default priority), shared by these calls and kept until the
+ * process ends; if their creation fails the call runs on the caller's
+ * stream alone.  All calls are asynchronous on the stream (the second stream
+ * is forked from and joined back into it with events).  Return value: 0 on
  * success, SRMHD_EARG for a bad argument, otherwise SRMHD_EHIP + the
  * hipError_t of the failing launch.  Calls that share a workspace must not
  * overlap.
@@ -229,11 +230,12 @@ enum { SRMHD_EMF_SOLVER = 0, SRMHD_EMF_HLL = 1 };
 /* hlld: the HLLD p* options (zeroed: the reference's default); non-default
  * methods and the hllc bracket are taken by the FP64 precision only (the
  * other forms and the adjoint entry points refuse them, SRMHD_EARG). */
-/* flags: SRMHD_STAGE_SERIAL_SWEEPS runs the FP64 form's y sweep after its x
- * sweep on the caller's stream; without it the y sweep runs beside the x
- * sweep on a second stream of the device (created at first use, default
- * priority; forked from and joined back into the caller's stream with
- * events).  The results are the same bit for bit. */
+/* flags: SRMHD_STAGE_SERIAL_SWEEPS runs the FP64 form's y sweep (in
+ * srmhd2d_stage and srmhd2d_stage_adjoint) after its x sweep on the caller's
+ * stream; without it the y sweep runs beside the x sweep on a second stream
+ * of the device (created at first use, default priority; forked from and
+ * joined back into the caller's stream with events).  The results are the
+ * same bit for bit. */
 enum { SRMHD_STAGE_SERIAL_SWEEPS = 1 };
 typedef struct srmhd_scheme2d {
   int32_t solver, limiter, upwind, precision;
