@@ -128,26 +128,36 @@ struct KastaunT {
   //   nu = max(nuA, nuB) >= nuA >= 1e-5 and |qb|, rbar^2 <= 1e61, so
   //   1 / (nu + mu rbar^2):             divisor in [1e-5, 1e62]
   // -- all inside div_nr's range, and the square root inside csqrt_mid's.
-  // The two divisions whose dividend can be arbitrarily small (vh^2 Wh^2 and
-  // p) stay IEEE divisions.
+  // The two divisions whose dividend can be arbitrarily small take div_lo:
+  //   vh^2 Wh^2 / (1 + Wh):             dividend in [+0, 1e10], divisor in [2, 1e5 + 1]
+  //   p / (rho (1 + eps) + tiny):       dividend in [+0, 1e21], divisor in [1e-12, 1e21]
+  // (neither dividend is ever -0: see below).
+  // Every value here is finite, so the clamps and maxima compare no NaN, and
+  // none of them meets zeros of opposite sign: mu, vh^2 (and v0^2) and the
+  // dividends are >= +0 and never -0 (mu = +0 or a sum of non-negative
+  // terms; mu^2 rbar^2 a product of such), the bounds 1e-12, 1e15, 1e-10
+  // and 1e5 are not zeros, eps0 is never -0 (its second term is >= +0, and
+  // -0 + +0 = +0), and nuA >= 1e-5.  So vmax / vmin return the values of
+  // tmax / tmin / clo / chi_.  With eps in [+0, 1e5] the EOS pressure
+  // 0 + (max(eps, 0) - 0) rho (g_th - 1) is eps rho (g_th - 1), bit for bit.
   SRMHD_DEV double master_fast(double mu, double* rho_out = nullptr,
                                double* eps_out = nullptr) const {
     const double c = div_nr(1.0, 1.0 + mu * bt2);
     const double mc = mu * c;
     const double qb = q - 0.5 * bt2 - 0.5 * (mc * mc) * bperp2;
     const double rb2 = rbar2(mu, c);
-    const double vh2 = chi_(mu * mu * rb2, v02);
-    const double Wh = div_nr(1.0, csqrt_mid(clo(1.0 - vh2, 1e-10)));
-    const double rho = tmin(tmax(div_nr(D, Wh), 1e-12), 1e15);
-    const double eps0 = Wh * (qb - mu * rb2) + vh2 * (Wh * Wh) / (1.0 + Wh);
+    const double vh2 = vmin(mu * mu * rb2, v02);
+    const double Wh = div_nr(1.0, csqrt_mid(vmax(1.0 - vh2, 1e-10)));
+    const double rho = vmin(vmax(div_nr(D, Wh), 1e-12), 1e15);
+    const double eps0 = Wh * (qb - mu * rb2) + div_lo(vh2 * (Wh * Wh), 1.0 + Wh);
     // the EOS at K = 0 with rho in [1e-12, 1e15]: cold pressure and energy
     // +0, so eps_lo = +0 and p = 0 + (eps - 0) rho (g_th - 1) (eos_press)
-    const double eps = tmin(tmax(eps0, 0.0), 1e5);
-    const double p = e.is_cold ? 0.0 : 0.0 + (tmax(eps, 0.0) - 0.0) * rho * (e.gamma_th - 1.0);
-    const double a = p / (rho * (1.0 + eps) + TINY);
+    const double eps = vmin(vmax(eps0, 0.0), 1e5);
+    const double p = e.is_cold ? 0.0 : eps * rho * (e.gamma_th - 1.0);
+    const double a = div_lo(p, rho * (1.0 + eps) + TINY);
     const double nuA = div_nr((1.0 + a) * (1.0 + eps), Wh);
     const double nuB = (1.0 + a) * (1.0 + qb - mu * rb2);
-    const double nu = tmax(nuA, nuB);
+    const double nu = vmax(nuA, nuB);
     if (rho_out) *rho_out = rho;
     if (eps_out) *eps_out = eps;
     return mu - div_nr(1.0, nu + mu * rb2);
@@ -234,12 +244,18 @@ struct KastaunT {
   }
 
   // one guarded secant step   c2p.py:219-232 (FAST: master_fast, tame
-  // elements only)
+  // elements only).  For a tame element |f| <= 1 + 1e5 (1 / (nu + mu rbar^2)
+  // <= 1e5), so the step's quotient has |dividend| <= 1 + 1e5 and divisor
+  // 1e-30 < |df| <= 2 + 2e5: div_lo's range.  A zero dividend may be -0
+  // (f1 = +0, x1 < x0); the quotient is then subtracted from x1, where a
+  // zero of either sign leaves x1 unchanged.
   template <bool FAST = false>
   SRMHD_DEV void secant_step(double& x0, double& f0, double& x1, double& f1) const {
     const double df = f1 - f0;
     const double mid = 0.5 * (x0 + x1);
-    double x2 = (fabs(df) > 1e-30) ? x1 - f1 * (x1 - x0) / df : mid;
+    double x2 = (fabs(df) > 1e-30)
+                    ? x1 - (FAST ? div_lo(f1 * (x1 - x0), df) : f1 * (x1 - x0) / df)
+                    : mid;
     x2 = ((x2 - x0) * (x2 - x1) < 0.0) ? x2 : mid;
     const double f2 = master_at<FAST>(x2);
     if (!((f0 * f2) <= 0.0)) {

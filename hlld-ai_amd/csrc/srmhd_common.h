@@ -102,6 +102,24 @@ SRMHD_DEV double div_nr(double n, double d) {
 #endif
 }
 
+// n / d correctly rounded where 2^-100 <= |d| <= 2^100 and |n| <= 2^800:
+// div_nr when |n| >= 2^-800 (the quotient is then inside div_nr's range) or
+// n is a zero, the compiler's IEEE division for the rare tiny n.  A zero n
+// gives a zero whose sign is IEEE's except for n = -0 with d > 0 (+0 here,
+// -0 in IEEE): callers have no -0 dividend, or use the quotient only where
+// the sign of a zero cannot matter.
+SRMHD_DEV double div_lo(double n, double d) {
+  if (fabs(n) >= 0x1p-800 || n == 0.0) return div_nr(n, d);
+  return n / d;
+}
+
+// torch.maximum / torch.minimum (tmax, tmin) and clamp(min=) / clamp(max=)
+// (clo, chi_) for operands that are not NaN and not zeros of opposite sign:
+// the hardware max / min then returns the same double (for equal operands,
+// the one value).  Callers establish both.
+SRMHD_DEV double vmax(double a, double b) { return __builtin_fmax(a, b); }
+SRMHD_DEV double vmin(double a, double b) { return __builtin_fmin(a, b); }
+
 // csqrt(x) for 2^-760 <= x <= 2^900, bit for bit: the compiler's f64 sqrt
 // expansion without its argument scaling (an identity in this range) --
 // the hardware reciprocal square root and three Newton corrections -- then
