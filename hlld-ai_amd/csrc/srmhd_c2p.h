@@ -154,7 +154,8 @@ struct KastaunT {
     // +0, so eps_lo = +0 and p = 0 + (eps - 0) rho (g_th - 1) (eos_press)
     const double eps = vmin(vmax(eps0, 0.0), 1e5);
     const double p = e.is_cold ? 0.0 : eps * rho * (e.gamma_th - 1.0);
-    const double a = div_lo(p, rho * (1.0 + eps) + TINY);
+    // rho (1 + eps) >= 1e-12, where adding TINY = 1e-300 rounds to itself
+    const double a = div_lo(p, rho * (1.0 + eps));
     const double nuA = div_nr((1.0 + a) * (1.0 + eps), Wh);
     const double nuB = (1.0 + a) * (1.0 + qb - mu * rb2);
     const double nu = vmax(nuA, nuB);
