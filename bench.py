@@ -68,6 +68,12 @@ VALU_BUSY = {("fp64", "sweep_x_a", 2048): 0.967}
 # the launch's faces) and HBM bytes per launch pair (FETCH_SIZE scaled by the
 # copy kernel's 2.0, plus WRITE_SIZE)
 ADJ_SWEEP_FLOPS_PER_FACE = {512: 6783.0}
+
+# measured HBM bytes per cell and RK3 step of the batched tubes' phased form
+# (4096 tubes, MP5 + HLLD; rocprofv3 --pmc FETCH_SIZE, scaled by the copy
+# kernel's 2.0, and WRITE_SIZE, summed over one step's launches:
+# profiles/tubes4096_counters.md), keyed by (limiter, solver)
+TUBES_TRAFFIC_PER_CELL_STEP = {("mp5", "hlld"): 4968.2}
 ADJ_SWEEP_TRAFFIC = {512: 269.3e6}
 
 # measured HBM bytes per launch (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE,
@@ -253,7 +259,9 @@ def bench_tubes(a):
         "roofline": {"bound": "hbm", "achieved": tube_bytes / (kernel_ms * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": tube_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None,
+                     "traffic": (TUBES_TRAFFIC_PER_CELL_STEP[(a.limiter, a.solver)] * zones *
+                                 a.steps if best == 0 and (a.limiter, a.solver) in
+                                 TUBES_TRAFFIC_PER_CELL_STEP else None),
                      "kernel": ("the phased form's launches, every step" if best == 0 else
                                 f"k_tubes (the persistent launch, block {best}, every step)"),
                      "avg_launch_ms": kernel_ms, "launches": 1,
