@@ -334,8 +334,8 @@ struct KastaunT {
 };
 using Kastaun = KastaunT<double>;
 
-// the pass-A walks test for a fixed point at steps 3, 7, 11, ... (it & 3 == 3)
-#define C2P_FIX_EVERY(it) (((it) & 3) == 3)
+// the pass-A walks test for a fixed point at steps 3, 7, 11, ...
+SRMHD_DEV bool c2p_fix_test(int it) { return (it & 3) == 3; }
 
 // per-element record of pass A: mu_+ (sign = good bracket), state after 60 steps
 struct C2PRec {
@@ -357,7 +357,7 @@ SRMHD_DEV uint64_t c2p_pass_a_impl(const Kastaun& k, C2PRec& rec) {
     // a fixed point: every later step sees this state, so its bits repeat.
     // Tested every fourth step only (a state a step maps to itself stays so:
     // finding it later changes neither the mask nor the final state)
-    if (C2P_FIX_EVERY(it) ? k.secant_step_fixed<FAST>(x0, f0, x1, f1)
+    if (c2p_fix_test(it) ? k.secant_step_fixed<FAST>(x0, f0, x1, f1)
                           : (k.secant_step<FAST>(x0, f0, x1, f1), false)) {
       if (nc) mask |= C2P_ALL & ~(((uint64_t)2 << it) - 1);
       break;
@@ -427,7 +427,7 @@ SRMHD_DEV uint64_t c2p_pass_a_impl(const Kastaun& k, C2PRecK& rec) {
     if (it == C2P_CK2) rec_put(rec.ck[2], x0, f0, x1, f1);
     // (jf may then lie up to three steps past the first stationary step:
     // the state after any K >= jf steps is still the final one)
-    if (C2P_FIX_EVERY(it) ? k.secant_step_fixed<FAST>(x0, f0, x1, f1)
+    if (c2p_fix_test(it) ? k.secant_step_fixed<FAST>(x0, f0, x1, f1)
                           : (k.secant_step<FAST>(x0, f0, x1, f1), false)) {
       if (nc) mask |= C2P_ALL & ~(((uint64_t)2 << it) - 1);
       jf = it;
