@@ -44,8 +44,8 @@ FP64_PEAK_TFLOPS = 78.6
 # SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 on the 2048^2 rotor
 # (profiles/rotor2048_counters.md): flops = 64 lanes x (add + mul + 2 fma +
 # trans) per wave, divided by the elements of the launch.
-FP64_FLOPS_PER_ELEMENT = {("fp64", "sweep_x_a"): 13660.0, ("fp64", "sweep_y_a"): 4131.0,
-                          ("fp64", "c2p_a"): 10246.0}
+FP64_FLOPS_PER_ELEMENT = {("fp64", "sweep_x_a"): 13802.0, ("fp64", "sweep_y_a"): 4131.0,
+                          ("fp64", "c2p_a"): 10388.0}
 
 # the kernel class each form spends most time in (rocprofv3 --stats of the
 # 2048^2 rotor: profiles/rotor2048_weno5z_hlld_kernel_stats.md)
@@ -54,13 +54,13 @@ DOMINANT_KERNEL = {"fp64": "sweep_x_a", "mixed": "sweep_x_b", "mixed_r64": "swee
 
 # wave-level VALU instructions per launch (rocprofv3 --pmc SQ_INSTS_VALU on the
 # 2048^2 rotor, profiles/rotor2048_counters.md)
-VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 1.836e8, ("fp64", "sweep_x_a", 2048): 9.206e8}
+VALU_INSTS_PER_LAUNCH = {("mixed", "sweep_x_b", 2048): 1.836e8, ("fp64", "sweep_x_a", 2048): 9.097e8}
 
 # measured VALU busy fraction of the launch (same profile): SQ_ACTIVE_INST_VALU
 # (quad-cycles in which a SIMD issued a VALU instruction, summed over the
 # chip) / (256 CUs x GRBM_GUI_ACTIVE cycles of one XCD) -- 1 means every SIMD
 # issued a vector instruction every quad-cycle of the launch
-VALU_BUSY = {("fp64", "sweep_x_a", 2048): 0.967}
+VALU_BUSY = {("fp64", "sweep_x_a", 2048): 0.961}
 
 # the adjoint's x sweep at 512^2 (k_adj_flux<0, 1, 2> + k_adj_recon<0, 1>),
 # rocprofv3 --pmc on the Orszag-Tang gradient (profiles/adjoint_ot512_counters.md):
